@@ -1,0 +1,194 @@
+/*
+ * suprema_pdlp.h -- C ABI of the MI355X (gfx950) restarted-PDHG (PDLP) engine.
+ *
+ * This is the drop-in boundary for the reference's GPU PDLP engine.  Every
+ * entry point takes plain pointers and sizes (no C++ or framework types),
+ * returns 0 on success or a negative SUPR_E* code, and leaves a message for
+ * supr_last_error() on failure.  Nothing here throws.
+ *
+ * Entry points and the reference interfaces they replace (paths relative to
+ * the reference repository):
+ *   supr_solve_pdhg_gpu   solve_pdhg_gpu()    include/pdlp_gpu.hpp:22-25,
+ *                                             gpu/pdlp_gpu_solve.cu:261-267
+ *                         (GpuBackend + pdlp_run, include/pdlp_algo.hpp:49-185)
+ *   supr_verify           verify(ranged, x, y) include/verify.hpp:13-20,
+ *                                             src/verify.cpp:18-103
+ *   supr_scale            scale()             include/scaling.hpp:13,
+ *                                             src/scaling.cpp:25-84
+ *   supr_pdlp_form_sizes  build_pdlp_form()   include/pdlp.hpp:26-27,
+ *   supr_pdlp_form                            src/pdlp.cpp:8-82
+ *   supr_extract_solution extract_solution()  include/pdlp.hpp:27-28,
+ *                                             src/pdlp.cpp:84-90
+ *   supr_gen_transport    tools/gen_lp.cpp:49-135 (same LP family, own PRNG)
+ *   supr_gen_random_lp    no reference counterpart: seeded equality-form LP
+ *                         generator (6 nonzeros per column, feasible and bounded
+ *                         by construction) for synthetic workloads
+ *   supr_bench_kernels    no reference counterpart: times the engine's two
+ *                         iteration kernels at a problem's real shapes (callers:
+ *                         benchmarks reporting achieved HBM bandwidth)
+ *   supr_last_error       the what() text of the reference's std::runtime_error
+ *                         (SOVEREIGN_CUDA_CHECK, gpu/pdlp_gpu_solve.cu:32-38)
+ *
+ * Threading: every entry point is re-entrant; solves on different threads
+ * use independent device state and streams.  The error text is thread-local.
+ */
+#ifndef SUPREMA_PDLP_H
+#define SUPREMA_PDLP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- status ----- */
+#define SUPR_OK 0
+#define SUPR_EINVAL (-1)    /* bad argument (sizes, indices, options) */
+#define SUPR_EDEVICE (-2)   /* HIP runtime / device error */
+#define SUPR_ENOMEM (-3)    /* host or device allocation failed */
+#define SUPR_ENODEV (-4)    /* no gfx950 device visible */
+#define SUPR_EINTERNAL (-5)
+
+const char* supr_last_error(void);
+/* Library version string. */
+const char* supr_version(void);
+/* Number of HIP devices visible (0 when none); never fails. */
+int supr_device_count(void);
+
+/* ------------------------------------------------------------- LP view -- */
+/* One LP "min c'x s.t. rL <= Ax <= rU, l <= x <= u" with A in COO order
+ * (row_idx, col_idx, val), the reference's RangedLP (include/ranged_lp.hpp:
+ * 10-25).  Infinite bounds are +-INFINITY.  All host memory, read only. */
+typedef struct supr_lp {
+    int32_t m, n, nnz;
+    const int32_t* row_idx;
+    const int32_t* col_idx;
+    const double* val;
+    const double* c;
+    const double* rL;
+    const double* rU;
+    const double* l;
+    const double* u;
+    double obj_offset;
+} supr_lp;
+
+/* --------------------------------------------------------- solve modes -- */
+/* SUPR_MODE_PLAIN_FP64: the reference GPU engine's design on MI355X: FP64
+ *   everywhere, the host decides every step (one blocking read of the step
+ *   statistics per attempt) and runs the KKT check on the host every
+ *   check_every accepted steps.
+ * SUPR_MODE_FAST_FP64: device-resident loop (no host sync inside the loop,
+ *   KKT residuals, restarts and step-size rule on the device, graph-replayed
+ *   iterations), FP64 iterates.
+ * SUPR_MODE_MIXED: device-resident loop with FP32 iterates / FP32 SpMV and
+ *   FP64 step statistics, KKT, restart rule; switches to the FP64 iterate
+ *   loop (the polish) once the FP32 iterate stalls or reaches switch_tol. */
+#define SUPR_MODE_PLAIN_FP64 0
+#define SUPR_MODE_FAST_FP64 1
+#define SUPR_MODE_MIXED 2
+
+typedef struct supr_pdlp_options {
+    int32_t mode;              /* SUPR_MODE_* */
+    int32_t max_iterations;    /* attempts, accepted or not (reference default 200000) */
+    int32_t check_every;       /* KKT check cadence in accepted steps (reference: 64) */
+    double tol;                /* converged iff max(eps_P, eps_D, eps_G) <= tol */
+    double switch_tol;         /* MIXED: FP32 -> FP64 when merit <= switch_tol (0: auto) */
+    int32_t stall_checks;      /* MIXED: ... or after this many checks without a new best (0: auto) */
+    int32_t graph_attempts;    /* device-resident modes: attempts per graph replay (0: auto) */
+    int32_t spmv_lanes_A;      /* lanes per row for A (0: autotune; 1..64 powers of two, or 256 = one block per row) */
+    int32_t spmv_lanes_AT;     /* lanes per row for A^T (0: autotune) */
+    int32_t device;            /* HIP device ordinal */
+    int32_t verbose;           /* 0 quiet, 1 phase summary on stderr */
+} supr_pdlp_options;
+
+/* Fills the reference's defaults: mode MIXED, max_iterations 200000,
+ * check_every 64, tol 1e-6, the rest automatic. */
+void supr_pdlp_default_options(supr_pdlp_options* opt);
+
+typedef struct supr_pdlp_info {
+    int32_t iterations;        /* attempts, as PdlpResult::iterations */
+    int32_t restarts;
+    int32_t converged;
+    int32_t has_point;         /* 0: no finite-merit point was seen; x, y untouched */
+    double eps_P, eps_D, eps_G;
+    double objective;          /* c'x + obj_offset of the returned point (unscaled form) */
+    int32_t iterations_fp32;   /* MIXED: attempts spent in the FP32 loop */
+    int32_t checks;            /* KKT checks performed */
+    int32_t lanes_A, lanes_AT; /* SpMV schedules used */
+    double seconds_setup;      /* device upload + layout + autotune */
+    double seconds_solve;      /* first iteration to verified result on the host clock */
+    double seconds_total;
+} supr_pdlp_info;
+
+/* Stop callback: polled by the host loop (at least every check_every
+ * attempts for the plain mode, every graph replay otherwise); nonzero stops
+ * the solve, which then returns the best point seen with converged = 0.
+ * The reference's std::atomic<bool>* stop_flag (include/pdlp.hpp:30-38). */
+typedef int (*supr_stop_fn)(void* ctx);
+
+/* solve_pdhg_gpu(): unscaled and scaled PDLP forms share one COO pattern
+ * (scaled = diag(Dr) A diag(Dc) with scaled c, bounds, rows as produced by
+ * supr_scale); x_out (n) and y_out (m) receive the returned point in
+ * UNSCALED PDLP-form coordinates, exactly like PdlpResult::x / y. */
+int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
+                        const double* Dr, const double* Dc, const supr_pdlp_options* opt,
+                        supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
+                        supr_pdlp_info* info);
+
+/* Times the two iteration kernels (primal half: A^T y + projection +
+ * averages; dual half: A x + dual step + step decision) at this LP's shapes,
+ * each launched reps times after a warm-up with every launch doing an
+ * accepted step's full work.  out8 = ms per primal-half launch (FP32), ms per
+ * dual-half launch (FP32), the same two in FP64, lanes-per-row of A and of
+ * A^T (FP64 choice), ELL width of A and of A^T (0 = CSR). */
+int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
+                       const double* Dr, const double* Dc, const supr_pdlp_options* opt,
+                       int32_t reps, double* out8);
+
+/* ------------------------------------------------------- host-side form -- */
+/* verify(): out7 = eps_P, eps_D, eps_G, objective, dual_objective,
+ * max_row_violation, max_bound_violation (KKTReport, include/verify.hpp:7-11). */
+int supr_verify(const supr_lp* lp, const double* x, const double* y, double* out7);
+
+/* scale(): Ruiz passes then Pock-Chambolle; integer may be NULL.  Outputs
+ * have the input's sizes (val_s: nnz in the input's COO order). */
+int supr_scale(const supr_lp* lp, const char* integer, int32_t ruiz_passes, double* val_s,
+               double* c_s, double* rL_s, double* rU_s, double* l_s, double* u_s, double* Dr,
+               double* Dc);
+
+/* build_pdlp_form(): dims4 = m2, n2, nnz2, n_slack. */
+int supr_pdlp_form_sizes(const supr_lp* lp, int32_t* dims4);
+int supr_pdlp_form(const supr_lp* lp, int32_t* row_idx2, int32_t* col_idx2, double* val2,
+                   double* c2, double* rL2, double* rU2, double* l2, double* u2,
+                   int32_t* row_perm, double* row_signs);
+int supr_extract_solution(int32_t n_orig, int32_t m_orig, int32_t m2, const int32_t* row_perm,
+                          const double* row_signs, const double* x2, const double* y2,
+                          double* x, double* y);
+
+/* ----------------------------------------------------------- generators -- */
+/* Balanced transportation LP (tools/gen_lp.cpp:49-135 family): m = S + D
+ * equality rows (suppliers, then demanders), n = S*D columns x_ij in i-major
+ * order, 2 nonzeros (+1, +1) per column, nnz = 2*S*D.  Supplies and raw
+ * demands are integers U[50,150]; demands are rescaled so the totals match
+ * exactly (remainder on the last).  Costs are integers U[1, cost_max];
+ * 0 <= x < inf.  Arrays sized by the caller. */
+int supr_gen_transport(int32_t S, int32_t D, uint64_t seed, int32_t cost_max, int32_t* row_idx,
+                       int32_t* col_idx, double* val, double* c, double* rL, double* rU,
+                       double* l, double* u);
+
+/* Random equality-form LP: A is m x n with exactly col_nnz nonzeros per
+ * column at distinct uniform rows, values N(0,1) (or +-log-uniform
+ * magnitudes in [10^-mag_decades, 10^mag_decades] when mag_decades > 0);
+ * x0 ~ U[0, x0_hi]^n, b = A x0 (rL = rU = b); y0 ~ N(0,1)^m, s0 ~ U[0,1]^n,
+ * c = A'y0 + s0; bounds [0, u_hi].  nnz = n * col_nnz.  Feasible (x0) and
+ * bounded (box).  If col_deg is non-NULL it gives per-column degrees instead
+ * of col_nnz (nnz = sum). */
+int supr_gen_random_lp(int32_t m, int32_t n, int32_t col_nnz, const int32_t* col_deg,
+                       uint64_t seed, double x0_hi, double u_hi, int32_t mag_decades,
+                       int32_t* row_idx, int32_t* col_idx, double* val, double* c, double* rL,
+                       double* rU, double* l, double* u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUPREMA_PDLP_H */

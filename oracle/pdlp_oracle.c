@@ -1,0 +1,564 @@
+/*
+ * pdlp_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64, single-threaded restatement of the reference's CPU PDLP
+ * path.  It is the checker the parity tests, the smoke test and the bench's
+ * CPU-baseline leg compare the HIP engine against; nothing in the product
+ * (team-suprema_amd/) links, loads or calls it.
+ *
+ * Every function follows one reference function and cites it (paths are
+ * relative to the reference repository):
+ *   orc_csr_build         include/sparse.hpp:31-57   (stable counting sort)
+ *   orc_spmv              src/pdlp.cpp:112-118, include/sparse.hpp:60-66
+ *   orc_verify            src/verify.cpp:6-103
+ *   orc_scale             src/scaling.cpp:4-84 (LP part; QP terms absent)
+ *   orc_pdlp_form_*       src/pdlp.cpp:8-82
+ *   orc_extract_solution  src/pdlp.cpp:84-90
+ *   orc_pdlp              include/pdlp_algo.hpp:49-185 with CpuBackend,
+ *                         src/pdlp.cpp:97-152
+ *
+ * The arithmetic is written in the reference's evaluation order (left-to-
+ * right row sums, COO-order accumulations, (v*a)*b products) and must be
+ * compiled without floating-point contraction (-ffp-contract=off), so that
+ * its results are bit-identical to the reference's own CPU engine.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   - bit-for-bit against the reference CPU engine compiled from its own
+ *     sources into oracle/_ref (scale, form, verify, full PDLP trajectories:
+ *     iterations, restarts, eps, x, y) via committed golden fixtures;
+ *   - against published Netlib optima (tools/audit_lp.cpp:148-160);
+ *   - against closed forms: dense brute-force matvec, a row off by delta
+ *     giving eps_P = delta/(1+|b|inf), infinite-bound sign cases, Ruiz
+ *     invariants.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_INF (1.0 / 0.0)
+/* std::max / std::min exactly as libstdc++ defines them (signed zeros and
+ * ties resolve the same way). */
+#define SMAX(a, b) ((a) < (b) ? (b) : (a))
+#define SMIN(a, b) ((b) < (a) ? (b) : (a))
+
+typedef struct {
+    int rows, cols, nnz;
+    int* indptr;
+    int* indices;
+    double* data;
+} orc_csr;
+
+/* include/sparse.hpp:31-47: counting sort by row, entries of a row kept in
+ * COO order.  Transposed CSR (sparse.hpp:50-57) = the same with the roles of
+ * the index arrays swapped. */
+static int csr_build(int rows, int cols, int nnz, const int* ri, const int* ci,
+                     const double* v, orc_csr* out) {
+    out->rows = rows; out->cols = cols; out->nnz = nnz;
+    out->indptr = (int*)calloc((size_t)rows + 1, sizeof(int));
+    out->indices = (int*)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    out->data = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    int* cursor = (int*)malloc(sizeof(int) * (size_t)(rows > 0 ? rows : 1));
+    if (!out->indptr || !out->indices || !out->data || !cursor) { free(cursor); return -1; }
+    for (int k = 0; k < nnz; ++k) out->indptr[ri[k] + 1]++;
+    for (int i = 0; i < rows; ++i) out->indptr[i + 1] += out->indptr[i];
+    for (int i = 0; i < rows; ++i) cursor[i] = out->indptr[i];
+    for (int k = 0; k < nnz; ++k) {
+        int pos = cursor[ri[k]]++;
+        out->indices[pos] = ci[k];
+        out->data[pos] = v[k];
+    }
+    free(cursor);
+    return 0;
+}
+
+static void csr_free(orc_csr* a) {
+    free(a->indptr); free(a->indices); free(a->data);
+    a->indptr = NULL; a->indices = NULL; a->data = NULL;
+}
+
+/* src/pdlp.cpp:112-118 (CpuBackend::spmv) == include/sparse.hpp:60-66. */
+static void spmv(const orc_csr* M, const double* v, double* out) {
+    for (int i = 0; i < M->rows; ++i) {
+        double s = 0.0;
+        for (int p = M->indptr[i]; p < M->indptr[i + 1]; ++p) s += M->data[p] * v[M->indices[p]];
+        out[i] = s;
+    }
+}
+
+/* Exported CSR builder + SpMV, so tests can pin the HIP SpMV kernels. */
+int orc_csr_spmv(int rows, int cols, int nnz, const int* ri, const int* ci, const double* v,
+                 int transposed, const double* x, double* out) {
+    orc_csr M;
+    int rc = transposed ? csr_build(cols, rows, nnz, ci, ri, v, &M)
+                        : csr_build(rows, cols, nnz, ri, ci, v, &M);
+    if (rc) return rc;
+    spmv(&M, x, out);
+    csr_free(&M);
+    return 0;
+}
+
+/* Exported: the CSR arrays themselves (indptr rows+1, indices/data nnz). */
+int orc_csr_arrays(int rows, int cols, int nnz, const int* ri, const int* ci, const double* v,
+                   int* indptr, int* indices, double* data) {
+    orc_csr M;
+    if (csr_build(rows, cols, nnz, ri, ci, v, &M)) return -1;
+    memcpy(indptr, M.indptr, sizeof(int) * ((size_t)rows + 1));
+    if (nnz > 0) {
+        memcpy(indices, M.indices, sizeof(int) * (size_t)nnz);
+        memcpy(data, M.data, sizeof(double) * (size_t)nnz);
+    }
+    csr_free(&M);
+    return 0;
+}
+
+/* ------------------------------------------------------------ verify ----- */
+
+/* src/verify.cpp:6-10 */
+static double finite_inf_norm(const double* v, int len) {
+    double m = 0.0; int any = 0;
+    for (int k = 0; k < len; ++k)
+        if (isfinite(v[k])) { m = SMAX(m, fabs(v[k])); any = 1; }
+    return any ? m : 0.0;
+}
+
+/* src/verify.cpp:11-16: the two parts are summed in order a, then b. */
+static double vec_norm2(const double* a, int na, const double* b, int nb) {
+    double s = 0.0;
+    for (int k = 0; k < na; ++k) s += a[k] * a[k];
+    for (int k = 0; k < nb; ++k) s += b[k] * b[k];
+    return sqrt(s);
+}
+
+typedef struct {
+    int m, n;
+    const double *c, *rL, *rU, *l, *u;
+    double obj_offset;
+} orc_lpv;
+
+/* out[0..6] = eps_P, eps_D, eps_G, objective, dual_objective,
+ * max_row_violation, max_bound_violation  (verify.hpp:7-11 field order). */
+static void verify_csr(const orc_lpv* P, const orc_csr* A, const orc_csr* AT,
+                       const double* x, const double* y, double* out) {
+    const int m = P->m, n = P->n;
+    const double *c = P->c, *rL = P->rL, *rU = P->rU, *l = P->l, *u = P->u;
+    double* a = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* aty = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* z = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* rv = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* bv = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    spmv(A, x, a);       /* matvec(Acsr, x): same sum order as spmv */
+    spmv(AT, y, aty);    /* matvec_T(ATcsr, y) */
+    for (int j = 0; j < n; ++j) z[j] = c[j] - aty[j];
+
+    /* eps_P  (verify.cpp:33-40) */
+    for (int i = 0; i < m; ++i) {
+        rv[i] = SMAX(rL[i] - a[i], 0.0) + SMAX(a[i] - rU[i], 0.0);
+    }
+    for (int j = 0; j < n; ++j) {
+        bv[j] = SMAX(l[j] - x[j], 0.0) + SMAX(x[j] - u[j], 0.0);
+    }
+    double frl = finite_inf_norm(rL, m), fru = finite_inf_norm(rU, m);
+    double denom_p = 1.0 + SMAX(frl, fru);
+    double eps_P = vec_norm2(rv, m, bv, n) / denom_p;
+    double max_rv = 0.0, max_bv = 0.0;
+    if (m > 0) { max_rv = rv[0]; for (int i = 1; i < m; ++i) if (rv[i] > max_rv) max_rv = rv[i]; }
+    if (n > 0) { max_bv = bv[0]; for (int j = 1; j < n; ++j) if (bv[j] > max_bv) max_bv = bv[j]; }
+
+    /* eps_D  (verify.cpp:42-57); rv / bv are reused as the sign-violation vectors */
+    for (int i = 0; i < m; ++i) {
+        double v = 0.0;
+        if (!isfinite(rL[i]) && y[i] > 0) v += y[i];
+        if (!isfinite(rU[i]) && y[i] < 0) v += -y[i];
+        rv[i] = v;
+    }
+    for (int j = 0; j < n; ++j) {
+        double v = 0.0;
+        if (!isfinite(l[j]) && z[j] > 0) v += z[j];
+        if (!isfinite(u[j]) && z[j] < 0) v += -z[j];
+        bv[j] = v;
+    }
+    double denom_d = 1.0 + finite_inf_norm(c, n);
+    double eps_D = vec_norm2(rv, m, bv, n) / denom_d;
+
+    /* dual objective  (verify.cpp:59-87) */
+    double D = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double yv = y[i];
+        if (!isfinite(rL[i]) && yv > 0) yv = 0.0;
+        if (!isfinite(rU[i]) && yv < 0) yv = 0.0;
+        double y_pos = SMAX(yv, 0.0), y_neg = SMAX(-yv, 0.0);
+        if (y_pos > 0 && isfinite(rL[i])) D += rL[i] * y_pos;
+        if (y_neg > 0 && isfinite(rU[i])) D += -rU[i] * y_neg;
+        if (y_pos > 0 && !isfinite(rL[i])) D = -ORC_INF;
+        if (y_neg > 0 && !isfinite(rU[i])) D = -ORC_INF;
+    }
+    for (int j = 0; j < n; ++j) {
+        double zv = z[j];
+        if (!isfinite(l[j]) && zv > 0) zv = 0.0;
+        if (!isfinite(u[j]) && zv < 0) zv = 0.0;
+        double z_pos = SMAX(zv, 0.0), z_neg = SMAX(-zv, 0.0);
+        if (z_pos > 0 && isfinite(l[j])) D += l[j] * z_pos;
+        if (z_neg > 0 && isfinite(u[j])) D += -u[j] * z_neg;
+        if (z_pos > 0 && !isfinite(l[j])) D = -ORC_INF;
+        if (z_neg > 0 && !isfinite(u[j])) D = -ORC_INF;
+    }
+
+    /* objective and gap  (verify.cpp:89-94) */
+    double cx = 0.0;
+    for (int j = 0; j < n; ++j) cx += c[j] * x[j];
+    double eps_G = !isfinite(D) ? ORC_INF : fabs(cx - D) / (1.0 + fabs(cx) + fabs(D));
+
+    out[0] = eps_P; out[1] = eps_D; out[2] = eps_G;
+    out[3] = cx + P->obj_offset;
+    out[4] = isfinite(D) ? D + P->obj_offset : D;
+    out[5] = max_rv; out[6] = max_bv;
+    free(a); free(aty); free(z); free(rv); free(bv);
+}
+
+/* Exported verify(ranged, x, y)  (verify.cpp:18-22). */
+int orc_verify(int m, int n, int nnz, const int* ri, const int* ci, const double* v,
+               const double* c, const double* rL, const double* rU, const double* l,
+               const double* u, double obj_offset, const double* x, const double* y,
+               double* out7) {
+    orc_csr A, AT;
+    if (csr_build(m, n, nnz, ri, ci, v, &A)) return -1;
+    if (csr_build(n, m, nnz, ci, ri, v, &AT)) { csr_free(&A); return -1; }
+    orc_lpv P = { m, n, c, rL, rU, l, u, obj_offset };
+    verify_csr(&P, &A, &AT, x, y, out7);
+    csr_free(&A); csr_free(&AT);
+    return 0;
+}
+
+/* ------------------------------------------------------------- scale ----- */
+
+/* src/scaling.cpp:25-84, LP part.  integer may be NULL (pure LP).
+ * Outputs: val_s (nnz, same COO order), c_s, rL_s, rU_s (m), l_s, u_s (n),
+ * Dr (m), Dc (n). */
+int orc_scale(int m, int n, int nnz, const int* ri, const int* ci, const double* v,
+              const double* c, const double* rL, const double* rU, const double* l,
+              const double* u, const char* integer, int ruiz_passes,
+              double* val_s, double* c_s, double* rL_s, double* rU_s, double* l_s,
+              double* u_s, double* Dr, double* Dc) {
+    double* A = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    double* rs = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* cs = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!A || !rs || !cs) { free(A); free(rs); free(cs); return -1; }
+    memcpy(A, v, sizeof(double) * (size_t)nnz);
+    for (int i = 0; i < m; ++i) Dr[i] = 1.0;
+    for (int j = 0; j < n; ++j) Dc[j] = 1.0;
+
+    for (int pass = 0; pass < ruiz_passes; ++pass) {
+        /* row_max_abs_all (scaling.cpp:4-8) */
+        for (int i = 0; i < m; ++i) rs[i] = 0.0;
+        for (int k = 0; k < nnz; ++k) rs[ri[k]] = SMAX(rs[ri[k]], fabs(A[k]));
+        for (int i = 0; i < m; ++i) rs[i] = rs[i] > 0 ? 1.0 / sqrt(rs[i]) : 1.0;
+        for (int k = 0; k < nnz; ++k) A[k] = A[k] * rs[ri[k]] * 1.0;
+        for (int i = 0; i < m; ++i) Dr[i] *= rs[i];
+
+        for (int j = 0; j < n; ++j) cs[j] = 0.0;
+        for (int k = 0; k < nnz; ++k) cs[ci[k]] = SMAX(cs[ci[k]], fabs(A[k]));
+        for (int j = 0; j < n; ++j) {
+            int keep = integer && integer[j];
+            cs[j] = (cs[j] > 0 && !keep) ? 1.0 / sqrt(cs[j]) : 1.0;
+        }
+        for (int k = 0; k < nnz; ++k) A[k] = A[k] * 1.0 * cs[ci[k]];
+        for (int j = 0; j < n; ++j) Dc[j] *= cs[j];
+    }
+
+    /* Pock-Chambolle alpha = 1 (scaling.cpp:56-66); abs sums in COO order */
+    for (int i = 0; i < m; ++i) rs[i] = 0.0;
+    for (int j = 0; j < n; ++j) cs[j] = 0.0;
+    for (int k = 0; k < nnz; ++k) rs[ri[k]] += fabs(A[k]);
+    for (int k = 0; k < nnz; ++k) cs[ci[k]] += fabs(A[k]);
+    for (int i = 0; i < m; ++i) { double sg = rs[i] > 0 ? 1.0 / rs[i] : 1.0; rs[i] = sqrt(sg); }
+    for (int j = 0; j < n; ++j) {
+        double t = cs[j] > 0 ? 1.0 / cs[j] : 1.0;
+        cs[j] = (integer && integer[j]) ? 1.0 : sqrt(t);
+    }
+    for (int k = 0; k < nnz; ++k) A[k] = A[k] * rs[ri[k]] * cs[ci[k]];
+    for (int i = 0; i < m; ++i) Dr[i] *= rs[i];
+    for (int j = 0; j < n; ++j) Dc[j] *= cs[j];
+
+    /* scaled form (scaling.cpp:68-76) */
+    memcpy(val_s, A, sizeof(double) * (size_t)nnz);
+    for (int j = 0; j < n; ++j) c_s[j] = Dc[j] * c[j];
+    for (int i = 0; i < m; ++i) { rL_s[i] = Dr[i] * rL[i]; rU_s[i] = Dr[i] * rU[i]; }
+    for (int j = 0; j < n; ++j) { l_s[j] = l[j] / Dc[j]; u_s[j] = u[j] / Dc[j]; }
+    free(A); free(rs); free(cs);
+    return 0;
+}
+
+/* -------------------------------------------------------- PDLP form ------ */
+
+/* src/pdlp.cpp:18-37: classify rows.  kind[i]: 0 equality, 1 ranged (eq row
+ * plus a slack column), 2 <= (ineq, sign -1), 3 >= (ineq, sign +1), 4 free. */
+static int row_kind(double lo, double hi) {
+    if (lo == hi) return 0;
+    if (isfinite(lo) && isfinite(hi)) return 1;
+    if (isfinite(hi)) return 2;
+    if (isfinite(lo)) return 3;
+    return 4;
+}
+
+/* Sizes of the PDLP form: dims[0] = m2, dims[1] = n2, dims[2] = nnz2,
+ * dims[3] = n_slack. */
+int orc_pdlp_form_sizes(int m, int n, int nnz, const int* ri, const double* rL,
+                        const double* rU, int* dims) {
+    int n_eq = 0, n_ineq = 0, n_slack = 0, nnz2 = 0;
+    for (int i = 0; i < m; ++i) {
+        int k = row_kind(rL[i], rU[i]);
+        if (k <= 1) n_eq++;
+        if (k == 1) n_slack++;
+        if (k == 2 || k == 3) n_ineq++;
+    }
+    for (int k = 0; k < nnz; ++k) if (row_kind(rL[ri[k]], rU[ri[k]]) != 4) nnz2++;
+    dims[0] = n_eq + n_ineq; dims[1] = n + n_slack; dims[2] = nnz2 + n_slack; dims[3] = n_slack;
+    return 0;
+}
+
+/* src/pdlp.cpp:8-82.  Output arrays sized by orc_pdlp_form_sizes.
+ * row_perm (m2) and row_signs (m2) are the PdlpFormMeta fields. */
+int orc_pdlp_form(int m, int n, int nnz, const int* ri, const int* ci, const double* v,
+                  const double* c, const double* rL, const double* rU, const double* l,
+                  const double* u,
+                  int* ri2, int* ci2, double* v2, double* c2, double* rL2, double* rU2,
+                  double* l2, double* u2, int* row_perm, double* row_signs) {
+    int dims[4];
+    orc_pdlp_form_sizes(m, n, nnz, ri, rL, rU, dims);
+    const int m2 = dims[0];
+    int* pos_of = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+    int* slack_of_eq = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+    double* slo = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* shi = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    if (!pos_of || !slack_of_eq || !slo || !shi) { free(pos_of); free(slack_of_eq); free(slo); free(shi); return -1; }
+    int n_eq = 0, n_slack = 0;
+    for (int i = 0; i < m; ++i) pos_of[i] = -1;
+    /* equality (and ranged) rows first, in original order, then inequalities */
+    for (int i = 0; i < m; ++i) {
+        int k = row_kind(rL[i], rU[i]);
+        if (k > 1) continue;
+        row_perm[n_eq] = i; row_signs[n_eq] = 1.0;
+        if (k == 0) { rL2[n_eq] = rL[i]; rU2[n_eq] = rL[i]; slack_of_eq[n_eq] = -1; }
+        else {
+            rL2[n_eq] = 0.0; rU2[n_eq] = 0.0;
+            slo[n_slack] = rL[i]; shi[n_slack] = rU[i];
+            slack_of_eq[n_eq] = n_slack++;
+        }
+        pos_of[i] = n_eq++;
+    }
+    int p = n_eq;
+    for (int i = 0; i < m; ++i) {
+        int k = row_kind(rL[i], rU[i]);
+        if (k == 2) { row_perm[p] = i; row_signs[p] = -1.0; rL2[p] = -rU[i]; rU2[p] = ORC_INF; pos_of[i] = p++; }
+        else if (k == 3) { row_perm[p] = i; row_signs[p] = 1.0; rL2[p] = rL[i]; rU2[p] = ORC_INF; pos_of[i] = p++; }
+    }
+    (void)m2;
+    int q = 0;
+    for (int k = 0; k < nnz; ++k) {
+        int pos = pos_of[ri[k]];
+        if (pos < 0) continue;
+        ri2[q] = pos; ci2[q] = ci[k]; v2[q] = row_signs[pos] * v[k]; q++;
+    }
+    for (int pos = 0; pos < n_eq; ++pos) {
+        int s = slack_of_eq[pos];
+        if (s >= 0) { ri2[q] = pos; ci2[q] = n + s; v2[q] = -1.0; q++; }
+    }
+    for (int j = 0; j < n; ++j) { c2[j] = c[j]; l2[j] = l[j]; u2[j] = u[j]; }
+    for (int s = 0; s < n_slack; ++s) { c2[n + s] = 0.0; l2[n + s] = slo[s]; u2[n + s] = shi[s]; }
+    free(pos_of); free(slack_of_eq); free(slo); free(shi);
+    return 0;
+}
+
+/* src/pdlp.cpp:84-90 */
+int orc_extract_solution(int n_orig, int m_orig, int m2, const int* row_perm,
+                         const double* row_signs, const double* x2, const double* y2,
+                         double* x, double* y) {
+    for (int j = 0; j < n_orig; ++j) x[j] = x2[j];
+    for (int i = 0; i < m_orig; ++i) y[i] = 0.0;
+    for (int pos = 0; pos < m2; ++pos) y[row_perm[pos]] = row_signs[pos] * y2[pos];
+    return 0;
+}
+
+/* -------------------------------------------------------------- PDLP ----- */
+
+static double merit3(const double* r) {
+    return sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+}
+
+/* include/pdlp_algo.hpp:49-185 with the CpuBackend arithmetic of
+ * src/pdlp.cpp:97-142.  The unscaled and scaled forms share one COO pattern
+ * (scale() only rescales values).  stop may be NULL; it is polled once per
+ * outer iteration like the reference's relaxed stop_flag load.
+ * info_i[0..3] = iterations (attempts), restarts, converged, has_point
+ * (0 only when no finite-merit point was ever seen: the reference then
+ * returns empty x / y and x_out, y_out are left untouched);
+ * info_d[0..2] = eps_P, eps_D, eps_G of the returned point. */
+int orc_pdlp(int m, int n, int nnz, const int* ri, const int* ci,
+             const double* val_u, const double* c_u, const double* rL_u, const double* rU_u,
+             const double* l_u, const double* u_u, double obj_offset,
+             const double* val_s, const double* c_s, const double* rL_s, const double* rU_s,
+             const double* l_s, const double* u_s,
+             const double* Dr, const double* Dc,
+             int max_iterations, int check_every, double tol, const volatile int* stop,
+             double* x_out, double* y_out, int* info_i, double* info_d) {
+    orc_csr A, AT, Au, AuT;
+    if (csr_build(m, n, nnz, ri, ci, val_s, &A)) return -1;
+    if (csr_build(n, m, nnz, ci, ri, val_s, &AT)) return -1;
+    if (csr_build(m, n, nnz, ri, ci, val_u, &Au)) return -1;
+    if (csr_build(n, m, nnz, ci, ri, val_u, &AuT)) return -1;
+    orc_lpv PU = { m, n, c_u, rL_u, rU_u, l_u, u_u, obj_offset };
+    char* is_ineq = (char*)malloc((size_t)(m > 0 ? m : 1));
+    for (int i = 0; i < m; ++i) is_ineq[i] = rL_s[i] != rU_s[i];
+    const double* b = rL_s;
+
+    size_t bn = sizeof(double) * (size_t)(n > 0 ? n : 1), bm = sizeof(double) * (size_t)(m > 0 ? m : 1);
+    double *x = malloc(bn), *xn = malloc(bn), *aty = malloc(bn), *xsum = calloc(n > 0 ? n : 1, sizeof(double));
+    double *y = calloc(m > 0 ? m : 1, sizeof(double)), *yn = malloc(bm), *ax = malloc(bm), *axn = malloc(bm);
+    double *ysum = calloc(m > 0 ? m : 1, sizeof(double));
+    double *hxs = malloc(bn), *hys = malloc(bm), *xu = malloc(bn), *yu = malloc(bm);
+    double *xu2 = malloc(bn), *yu2 = malloc(bm), *hx_last = malloc(bn), *hy_last = malloc(bm);
+    double *best_x = malloc(bn), *best_y = malloc(bm);
+
+    /* initial point, step size, primal weight (pdlp_algo.hpp:67-77) */
+    for (int j = 0; j < n; ++j) x[j] = SMIN(SMAX(0.0, l_s[j]), u_s[j]);
+    double amax = 0.0;
+    for (int k = 0; k < nnz; ++k) amax = SMAX(amax, fabs(val_s[k]));
+    double eta = amax > 0 ? 1.0 / amax : 1.0;
+    double cn = 0.0, bnrm = 0.0;
+    for (int j = 0; j < n; ++j) cn += c_s[j] * c_s[j];
+    for (int i = 0; i < m; ++i) if (isfinite(rL_s[i])) bnrm += rL_s[i] * rL_s[i];
+    cn = sqrt(cn); bnrm = sqrt(bnrm);
+    double omega = (cn > 1e-10 && bnrm > 1e-10) ? cn / bnrm : 1.0;
+
+    spmv(&A, x, ax);
+    spmv(&AT, y, aty);
+    double wsum = 0.0;
+    memcpy(hx_last, x, bn); memcpy(hy_last, y, bm);
+    double merit_last_restart = ORC_INF, merit_prev_cand = ORC_INF;
+    int since_restart = 0, restarts = 0, accepted = 0, attempts = 0;
+    double best_merit = ORC_INF, best_eps[3] = { 0, 0, 0 };
+    int have_best = 0, converged = 0;
+    double rc[7], ra[7];
+
+    while (attempts < max_iterations) {
+        if (stop && *stop) break;
+        while (attempts < max_iterations) {
+            ++attempts;
+            const double tau = eta / omega, sigma = eta * omega;
+            for (int j = 0; j < n; ++j) {           /* primal_step */
+                xn[j] = SMIN(SMAX(x[j] - tau * (c_s[j] - aty[j]), l_s[j]), u_s[j]);
+            }
+            spmv(&A, xn, axn);
+            for (int i = 0; i < m; ++i) {           /* dual_step */
+                double t = y[i] + sigma * (b[i] - (2.0 * axn[i] - ax[i]));
+                yn[i] = (is_ineq[i] && t < 0.0) ? 0.0 : t;
+            }
+            double dx2 = 0.0, dy2 = 0.0, inter = 0.0;   /* step_stats */
+            for (int j = 0; j < n; ++j) { double d = xn[j] - x[j]; dx2 += d * d; }
+            for (int i = 0; i < m; ++i) { double d = yn[i] - y[i]; dy2 += d * d; inter += d * (axn[i] - ax[i]); }
+            const double num = omega * dx2 + dy2 / omega;
+            const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : ORC_INF;
+            const double k1 = accepted + 2.0;
+            const double grow = (1.0 + pow(k1, -0.6)) * eta;
+            const double eta_next = isfinite(eta_bar) ? SMIN((1.0 - pow(k1, -0.3)) * eta_bar, grow) : grow;
+            if (eta <= eta_bar) {
+                double* t;
+                t = x; x = xn; xn = t;
+                t = y; y = yn; yn = t;
+                t = ax; ax = axn; axn = t;
+                spmv(&AT, y, aty);
+                for (int j = 0; j < n; ++j) xsum[j] += eta * x[j];
+                for (int i = 0; i < m; ++i) ysum[i] += eta * y[i];
+                wsum += eta;
+                eta = eta_next;
+                ++accepted; ++since_restart;
+                break;
+            }
+            eta = eta_next;
+        }
+
+        if (accepted % check_every != 0 && attempts < max_iterations) continue;
+
+        /* KKT check on the unscaled problem: current vs average (:135-153) */
+        if (wsum > 0) {
+            for (int j = 0; j < n; ++j) hxs[j] = xsum[j] / wsum;
+            for (int i = 0; i < m; ++i) hys[i] = ysum[i] / wsum;
+        } else { memcpy(hxs, x, bn); memcpy(hys, y, bm); }
+        for (int j = 0; j < n; ++j) { xu[j] = Dc[j] * x[j]; xu2[j] = Dc[j] * hxs[j]; }
+        for (int i = 0; i < m; ++i) { yu[i] = Dr[i] * y[i]; yu2[i] = Dr[i] * hys[i]; }
+        verify_csr(&PU, &Au, &AuT, xu, yu, rc);
+        verify_csr(&PU, &Au, &AuT, xu2, yu2, ra);
+        /* keep_best, current then average (:94-100, :142-143) */
+        if (merit3(rc) < best_merit) {
+            best_merit = merit3(rc); memcpy(best_x, xu, bn); memcpy(best_y, yu, bm);
+            best_eps[0] = rc[0]; best_eps[1] = rc[1]; best_eps[2] = rc[2]; have_best = 1;
+        }
+        if (merit3(ra) < best_merit) {
+            best_merit = merit3(ra); memcpy(best_x, xu2, bn); memcpy(best_y, yu2, bm);
+            best_eps[0] = ra[0]; best_eps[1] = ra[1]; best_eps[2] = ra[2]; have_best = 1;
+        }
+        const int use_avg = merit3(ra) < merit3(rc);
+        const double* rep = use_avg ? ra : rc;
+        const double mc = merit3(rep);
+        const double worst = SMAX(SMAX(rep[0], rep[1]), rep[2]);   /* std::max({...}) */
+        if (worst <= tol) {
+            memcpy(x_out, use_avg ? xu2 : xu, sizeof(double) * (size_t)n);
+            memcpy(y_out, use_avg ? yu2 : yu, sizeof(double) * (size_t)m);
+            info_d[0] = rep[0]; info_d[1] = rep[1]; info_d[2] = rep[2];
+            converged = 1;
+            break;
+        }
+
+        /* restart decision and primal weight (:155-174) */
+        const int sufficient = mc <= 0.2 * merit_last_restart;
+        const int necessary = mc <= 0.8 * merit_last_restart && mc > merit_prev_cand;
+        const int artificial = since_restart >= 0.36 * accepted;
+        merit_prev_cand = mc;
+        if (sufficient || necessary || artificial) {
+            const double* cx = use_avg ? hxs : x;
+            const double* cy = use_avg ? hys : y;
+            double ddx = 0.0, ddy = 0.0;
+            for (int j = 0; j < n; ++j) ddx += (cx[j] - hx_last[j]) * (cx[j] - hx_last[j]);
+            for (int i = 0; i < m; ++i) ddy += (cy[i] - hy_last[i]) * (cy[i] - hy_last[i]);
+            ddx = sqrt(ddx); ddy = sqrt(ddy);
+            if (ddx > 1e-10 && ddy > 1e-10) omega = exp(0.5 * log(ddy / ddx) + 0.5 * log(omega));
+            memcpy(hx_last, cx, bn); memcpy(hy_last, cy, bm);
+            if (use_avg) {
+                memcpy(x, hxs, bn); memcpy(y, hys, bm);
+                spmv(&A, x, ax); spmv(&AT, y, aty);
+            }
+            memset(xsum, 0, bn); memset(ysum, 0, bm); wsum = 0.0;
+            since_restart = 0; ++restarts;
+            merit_last_restart = mc;
+            merit_prev_cand = ORC_INF;
+        }
+    }
+
+    if (!converged) {
+        /* stopped: the best point seen (:177-184) */
+        if (!isfinite(best_merit)) {
+            for (int j = 0; j < n; ++j) xu[j] = Dc[j] * x[j];
+            for (int i = 0; i < m; ++i) yu[i] = Dr[i] * y[i];
+            verify_csr(&PU, &Au, &AuT, xu, yu, rc);
+            if (merit3(rc) < best_merit) {    /* keep_best: an infinite merit keeps nothing */
+                memcpy(best_x, xu, bn); memcpy(best_y, yu, bm);
+                best_eps[0] = rc[0]; best_eps[1] = rc[1]; best_eps[2] = rc[2];
+                have_best = 1;
+            }
+        }
+        if (have_best) {
+            memcpy(x_out, best_x, sizeof(double) * (size_t)n);
+            memcpy(y_out, best_y, sizeof(double) * (size_t)m);
+        }
+        info_d[0] = best_eps[0]; info_d[1] = best_eps[1]; info_d[2] = best_eps[2];
+    }
+    info_i[0] = attempts; info_i[1] = restarts; info_i[2] = converged;
+    info_i[3] = converged || have_best;
+
+    csr_free(&A); csr_free(&AT); csr_free(&Au); csr_free(&AuT);
+    free(is_ineq);
+    free(x); free(xn); free(aty); free(xsum); free(y); free(yn); free(ax); free(axn); free(ysum);
+    free(hxs); free(hys); free(xu); free(yu); free(xu2); free(yu2); free(hx_last); free(hy_last);
+    free(best_x); free(best_y);
+    return 0;
+}

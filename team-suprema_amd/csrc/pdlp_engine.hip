@@ -1,0 +1,1076 @@
+// pdlp_engine.hip -- the MI355X restarted-PDHG engine behind
+// supr_solve_pdhg_gpu (the reference's solve_pdhg_gpu,
+// gpu/pdlp_gpu_solve.cu:261-267 + include/pdlp_algo.hpp:49-185).
+//
+// Modes (see suprema_pdlp.h):
+//  * PLAIN_FP64 -- the reference GPU backend's design on this GPU: FP64
+//    kernels, the host reads the step statistics after every attempt and
+//    runs the KKT check (product verify on the host) every check_every
+//    accepted steps.  It is the honest "plain FP64 GPU PDLP" baseline.
+//  * FAST_FP64 / MIXED -- the device-resident loop: the algorithm's control
+//    flow lives in a DevState block on the device, iterations and checks are
+//    replayed from a captured hipGraph, the host only polls a 200-byte
+//    state copy once per replay.  MIXED runs FP32 iterates / FP32 SpMV with
+//    FP64 statistics, KKT and restart rule, then continues in FP64 from the
+//    FP32 loop's candidate (the polish) until the tolerance is verified.
+//
+// Memory layout (per LP, all resident in HBM for the whole solve):
+//   A    : CSR (row pointers, int32 column indices) or ELL for uniform short
+//          rows; values as FP32 scaled, FP64 scaled, FP64 unscaled arrays in
+//          the same order (one index stream serves all three).
+//   A^T  : the same for the stored transpose (reference keeps both,
+//          gpu/pdlp_gpu_solve.cu:194).
+//   iterates: x, y, A x double-buffered (swap = flip of DevState::cur),
+//          A^T y, ergodic sums, restart points; c, l, u, b, inequality flags.
+//   KKT  : unscaled c, l, u, rL, rU, Dr, Dc; unscaled candidates, averages,
+//          best point (FP64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_lp.hpp"
+#include "pdlp_kernels.hpp"
+#include "suprema_pdlp.h"
+
+namespace supr {
+
+namespace {
+
+struct HipFailure : std::runtime_error {
+    explicit HipFailure(const std::string& m) : std::runtime_error(m) {}
+};
+struct BadInput : std::runtime_error {
+    explicit BadInput(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define SUPR_HIP(expr)                                                                       \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            throw HipFailure(std::string("HIP error at " __FILE__ ":") +                    \
+                             std::to_string(__LINE__) + " (" #expr ") -- " +               \
+                             hipGetErrorString(_e));                                         \
+    } while (0)
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t0) {
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+// Owning device allocation (move-only).
+class DevBuf {
+   public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
+    DevBuf& operator=(DevBuf&& o) noexcept { std::swap(p_, o.p_); return *this; }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p_) (void)hipFree(p_); }
+    void alloc(size_t bytes) {
+        if (p_) { (void)hipFree(p_); p_ = nullptr; }
+        SUPR_HIP(hipMalloc(&p_, std::max<size_t>(bytes, 256)));
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p_); }
+    explicit operator bool() const { return p_ != nullptr; }
+
+   private:
+    void* p_ = nullptr;
+};
+
+// Synchronous upload: the host vector may be a temporary.
+template <typename T>
+DevBuf upload(const std::vector<T>& h, hipStream_t s) {
+    DevBuf d(h.size() * sizeof(T));
+    if (!h.empty()) {
+        SUPR_HIP(hipMemcpyAsync(d.as<void>(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+    }
+    return d;
+}
+
+template <typename T>
+DevBuf device_vec(size_t len) { return DevBuf(len * sizeof(T)); }
+
+// Row-loop grids are rounded up to a multiple of the XCD count (the kernels'
+// XCD-aware block mapping needs it; extra blocks find no rows).
+int grid_rows(int rows, int L) {
+    const int per = L >= kBlock ? 1 : kBlock / L;
+    long g = ((long)rows + per - 1) / per;
+    g = std::max(1L, std::min<long>(g, kMaxBlocks));
+    if (g >= kXcds) g = (g + kXcds - 1) / kXcds * kXcds;
+    return (int)g;
+}
+int grid_elems(long len) {
+    return (int)std::max(1L, std::min<long>((len + kBlock - 1) / kBlock, kMaxBlocks));
+}
+
+#define SUPR_LANES(L, ...)                                                        \
+    switch (L) {                                                                  \
+        case 1: { constexpr int LL = 1; __VA_ARGS__; } break;                     \
+        case 2: { constexpr int LL = 2; __VA_ARGS__; } break;                     \
+        case 4: { constexpr int LL = 4; __VA_ARGS__; } break;                     \
+        case 8: { constexpr int LL = 8; __VA_ARGS__; } break;                     \
+        case 16: { constexpr int LL = 16; __VA_ARGS__; } break;                   \
+        case 32: { constexpr int LL = 32; __VA_ARGS__; } break;                   \
+        case 64: { constexpr int LL = 64; __VA_ARGS__; } break;                   \
+        case 256: { constexpr int LL = 256; __VA_ARGS__; } break;                 \
+        default: throw BadInput("unsupported lanes-per-row " + std::to_string(L)); \
+    }
+
+bool valid_lanes(int L) {
+    return L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64 || L == kBlock;
+}
+
+// --------------------------------------------------------- device matrix --
+
+struct DevMatrix {
+    int rows = 0, cols = 0, nnz = 0, ell_w = 0, lanes = 1;
+    DevBuf ptr, idx, v32, v64, vu;
+    template <typename V> MatView<V> view(const DevBuf& v) const {
+        MatView<V> m;
+        m.rows = rows;
+        m.ell_w = ell_w;
+        m.ptr = ptr.as<int32_t>();
+        m.idx = idx.as<int32_t>();
+        m.val = v.as<V>();
+        return m;
+    }
+    MatView<float> f32() const { return view<float>(v32); }
+    MatView<double> f64() const { return view<double>(v64); }
+    MatView<double> unscaled() const { return view<double>(vu); }
+};
+
+constexpr int kMaxEllWidth = 16;
+
+// Lays out H (CSR of the scaled matrix; H.perm maps to COO positions) on the
+// device, as ELL when every row has the same length 1..16.
+void build_dev_matrix(DevMatrix& D, const Csr& H, const double* unscaled_coo, bool want32,
+                      bool want64, bool wantu, hipStream_t s) {
+    D.rows = H.rows;
+    D.cols = H.cols;
+    D.nnz = (int)H.idx.size();
+    int w = -1;
+    bool uniform = H.rows > 0;
+    for (int r = 0; r < H.rows && uniform; ++r) {
+        const int len = H.ptr[r + 1] - H.ptr[r];
+        if (w < 0) w = len;
+        uniform = (len == w);
+    }
+    D.ell_w = (uniform && w >= 1 && w <= kMaxEllWidth) ? w : 0;
+    const size_t nnz = H.idx.size();
+    std::vector<int32_t> idx(nnz);
+    std::vector<size_t> src(nnz);   // slot -> CSR position
+    if (D.ell_w > 0) {
+        for (int r = 0; r < H.rows; ++r)
+            for (int k = 0; k < D.ell_w; ++k) src[(size_t)k * H.rows + r] = (size_t)H.ptr[r] + k;
+    } else {
+        for (size_t p = 0; p < nnz; ++p) src[p] = p;
+        D.ptr = upload(H.ptr, s);
+    }
+    for (size_t q = 0; q < nnz; ++q) idx[q] = H.idx[src[q]];
+    D.idx = upload(idx, s);
+    if (want32) {
+        std::vector<float> v(nnz);
+        for (size_t q = 0; q < nnz; ++q) v[q] = (float)H.val[src[q]];
+        D.v32 = upload(v, s);
+    }
+    if (want64) {
+        std::vector<double> v(nnz);
+        for (size_t q = 0; q < nnz; ++q) v[q] = H.val[src[q]];
+        D.v64 = upload(v, s);
+    }
+    if (wantu) {
+        std::vector<double> v(nnz);
+        for (size_t q = 0; q < nnz; ++q) v[q] = unscaled_coo[H.perm[src[q]]];
+        D.vu = upload(v, s);
+    }
+    SUPR_HIP(hipStreamSynchronize(s));   // host staging vectors die here
+}
+
+template <typename V>
+void launch_spmv(const DevMatrix& M, const MatView<V>& view, const V* x, V* y, hipStream_t s) {
+    SUPR_LANES(M.lanes, hipLaunchKernelGGL((k_spmv<LL, V, V, V>), dim3(grid_rows(M.rows, LL)),
+                                           dim3(kBlock), 0, s, view, x, y));
+    SUPR_HIP(hipGetLastError());
+}
+
+// Picks lanes-per-row for M by timing every candidate on the device.
+template <typename V>
+int autotune_lanes(DevMatrix& M, const MatView<V>& view, const V* x, V* y, hipStream_t s) {
+    if (M.ell_w > 0) return 1;
+    const double mean = M.rows > 0 ? (double)M.nnz / M.rows : 0.0;
+    hipEvent_t e0, e1;
+    SUPR_HIP(hipEventCreate(&e0));
+    SUPR_HIP(hipEventCreate(&e1));
+    int best = 1;
+    float best_ms = std::numeric_limits<float>::infinity();
+    for (int L = 1; L <= kBlock; L = (L == 64 ? kBlock : 2 * L)) {
+        if (L > 1 && mean < 0.25 * L) break;   // rows far shorter than the group
+        M.lanes = L;
+        launch_spmv(M, view, x, y, s);
+        SUPR_HIP(hipEventRecord(e0, s));
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) launch_spmv(M, view, x, y, s);
+        SUPR_HIP(hipEventRecord(e1, s));
+        SUPR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SUPR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = L; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    M.lanes = best;
+    return best;
+}
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    StreamGuard() { SUPR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~StreamGuard() { if (s) (void)hipStreamDestroy(s); }
+};
+
+// ---------------------------------------------------------------- solver --
+
+double finite_inf_norm(const double* v, int len) {
+    double m = 0.0;
+    for (int k = 0; k < len; ++k)
+        if (std::isfinite(v[k])) m = std::max(m, std::fabs(v[k]));
+    return m;
+}
+
+double merit_of(const double* e) { return std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]); }
+
+class Solver {
+   public:
+    Solver(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc,
+           const supr_pdlp_options& o)
+        : U_(U), S_(S), Dr_(Dr), Dc_(Dc), o_(o), m_(S.m), n_(S.n) {}
+
+    void solve(supr_stop_fn stop, void* ctx, double* x_out, double* y_out, supr_pdlp_info* info);
+    void bench_kernels(int reps, double* out);
+
+   private:
+    // shared
+    const supr_lp& U_;
+    const supr_lp& S_;
+    const double* Dr_;
+    const double* Dc_;
+    supr_pdlp_options o_;
+    const int m_, n_;
+    StreamGuard stream_;
+    DevMatrix A_, AT_;
+    double eta0_ = 1.0, omega0_ = 1.0;
+    std::vector<double> x0_;
+
+    void initial_point();
+    void setup_matrices(bool want32, bool want64, bool wantu);
+
+    // plain mode
+    void solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_out, supr_pdlp_info* info);
+
+    // device-resident modes
+    template <typename T> struct Iter {
+        DevBuf x[2], y[2], ax[2], aty, xsum, ysum, xlast, ylast, c, l, u, b;
+        IterBufs<T> view(const uint8_t* ineq) const {
+            IterBufs<T> B;
+            for (int k = 0; k < 2; ++k) {
+                B.x[k] = x[k].template as<T>();
+                B.y[k] = y[k].template as<T>();
+                B.ax[k] = ax[k].template as<T>();
+            }
+            B.aty = aty.template as<T>();
+            B.xsum = xsum.template as<T>();
+            B.ysum = ysum.template as<T>();
+            B.xlast = xlast.template as<T>();
+            B.ylast = ylast.template as<T>();
+            B.c = c.template as<T>();
+            B.l = l.template as<T>();
+            B.u = u.template as<T>();
+            B.b = b.template as<T>();
+            B.ineq = ineq;
+            return B;
+        }
+    };
+    Iter<float> F_;
+    Iter<double> D_;
+    DevBuf ineq_, state_, parts_;
+    DevBuf kc_, kl_, ku_, krL_, krU_, kDr_, kDc_, xu_, xu2_, xavg_, yu_, yu2_, yavg_, bestx_, besty_;
+    KktBufs K_{};
+    double *part_n_ = nullptr, *part_m_ = nullptr, *part_kn_ = nullptr, *part_km_ = nullptr,
+           *part_rn_ = nullptr, *part_rm_ = nullptr;
+
+    template <typename T> void alloc_iter(Iter<T>& I, bool init_point);
+    void setup_device_resident(bool mixed);
+    DevState initial_state(bool mixed) const;
+    template <typename T> void enqueue_attempts(const Iter<T>& I, int count);
+    template <typename T> void enqueue_check(const Iter<T>& I);
+    template <typename T> void enqueue_ax_fix(const Iter<T>& I);
+    template <typename T> hipGraphExec_t capture_graph(const Iter<T>& I, int attempts);
+    template <typename T>
+    int run_graph_loop(const Iter<T>& I, supr_stop_fn stop, void* ctx, DevState& hs, bool& stopped);
+    void solve_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
+                               double* y_out, supr_pdlp_info* info);
+};
+
+void Solver::initial_point() {
+    // include/pdlp_algo.hpp:67-77: x0 = clip(0, l, u), eta0 = 1 / max|a_ij|,
+    // omega0 = |c| / |finite rL| (1 when either is tiny).
+    x0_.resize(n_);
+    for (int j = 0; j < n_; ++j) x0_[j] = smin(smax(0.0, S_.l[j]), S_.u[j]);
+    double amax = 0.0;
+    for (int k = 0; k < S_.nnz; ++k) amax = smax(amax, std::fabs(S_.val[k]));
+    eta0_ = amax > 0 ? 1.0 / amax : 1.0;
+    double cn = 0.0, bn = 0.0;
+    for (int j = 0; j < n_; ++j) cn += S_.c[j] * S_.c[j];
+    for (int i = 0; i < m_; ++i)
+        if (std::isfinite(S_.rL[i])) bn += S_.rL[i] * S_.rL[i];
+    cn = std::sqrt(cn);
+    bn = std::sqrt(bn);
+    omega0_ = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+}
+
+void Solver::setup_matrices(bool want32, bool want64, bool wantu) {
+    const Csr A = build_csr(S_, S_.val, false);
+    build_dev_matrix(A_, A, U_.val, want32, want64, wantu, stream_.s);
+    const Csr AT = build_csr(S_, S_.val, true);
+    build_dev_matrix(AT_, AT, U_.val, want32, want64, wantu, stream_.s);
+}
+
+// ---------------------------------------------------------- plain FP64 --
+
+void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_out,
+                         supr_pdlp_info* info) {
+    const auto t0 = Clock::now();
+    hipStream_t s = stream_.s;
+    const int n = n_, m = m_;
+    setup_matrices(false, true, false);
+    initial_point();
+    // host CSR of the unscaled form for the host verifier (pdlp_algo.hpp:57)
+    const Csr Au = build_csr(U_, U_.val, false), AuT = build_csr(U_, U_.val, true);
+
+    auto vec = [&](const double* p, int len) { return upload(std::vector<double>(p, p + len), s); };
+    DevBuf c = vec(S_.c, n), l = vec(S_.l, n), u = vec(S_.u, n), b = vec(S_.rL, m);
+    std::vector<uint8_t> hineq(m);
+    for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
+    DevBuf ineq = upload(hineq, s);
+    DevBuf x = device_vec<double>(n), xn = device_vec<double>(n), aty = device_vec<double>(n),
+           xsum = device_vec<double>(n), y = device_vec<double>(m), yn = device_vec<double>(m),
+           ax = device_vec<double>(m), axn = device_vec<double>(m), ysum = device_vec<double>(m),
+           stats = device_vec<double>(3 * kMaxPlain);
+    double* px = x.as<double>(); double* pxn = xn.as<double>();
+    double* py = y.as<double>(); double* pyn = yn.as<double>();
+    double* pax = ax.as<double>(); double* paxn = axn.as<double>();
+
+    // SpMV schedules
+    if (o_.spmv_lanes_A > 0) A_.lanes = o_.spmv_lanes_A;
+    else autotune_lanes(A_, A_.f64(), px, pax, s);
+    if (o_.spmv_lanes_AT > 0) AT_.lanes = o_.spmv_lanes_AT;
+    else autotune_lanes(AT_, AT_.f64(), py, aty.as<double>(), s);
+    if (A_.ell_w > 0) A_.lanes = 1;
+    if (AT_.ell_w > 0) AT_.lanes = 1;
+
+    std::vector<double> hx = x0_, hy(m, 0.0);
+    SUPR_HIP(hipMemcpyAsync(px, hx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    SUPR_HIP(hipMemsetAsync(py, 0, sizeof(double) * m, s));
+    SUPR_HIP(hipMemsetAsync(xsum.as<void>(), 0, sizeof(double) * n, s));
+    SUPR_HIP(hipMemsetAsync(ysum.as<void>(), 0, sizeof(double) * m, s));
+    launch_spmv(A_, A_.f64(), px, pax, s);
+    launch_spmv(AT_, AT_.f64(), py, aty.as<double>(), s);
+    SUPR_HIP(hipStreamSynchronize(s));
+    const double t_setup = seconds_since(t0);
+    const auto t1 = Clock::now();
+
+    double eta = eta0_, omega = omega0_, wsum = 0.0;
+    std::vector<double> hx_last = hx, hy_last = hy, hxs(n), hys(m), xu(n), yu(m), xu2(n), yu2(m);
+    double merit_last_restart = INFINITY, merit_prev_cand = INFINITY;
+    int since_restart = 0, restarts = 0, accepted = 0, attempts = 0, checks = 0;
+    std::vector<double> best_x, best_y;
+    double best_merit = INFINITY, best_eps[3] = {0, 0, 0};
+    double stats_h[3 * kMaxPlain];
+    const int gb_n = (n + kBlock - 1) / kBlock, gb_m = (m + kBlock - 1) / kBlock;
+    const int rb_n = std::min(kMaxPlain, std::max(1, gb_n)), rb_m = std::min(kMaxPlain, std::max(1, gb_m));
+    double rc[7], ra[7];
+    bool converged = false;
+    std::vector<double> res_x, res_y;
+    double res_eps[3] = {0, 0, 0};
+
+    auto unscaled_kkt = [&](const std::vector<double>& xs, const std::vector<double>& ys,
+                            std::vector<double>& xuo, std::vector<double>& yuo, double* rep) {
+        for (int j = 0; j < n; ++j) xuo[j] = Dc_[j] * xs[j];
+        for (int i = 0; i < m; ++i) yuo[i] = Dr_[i] * ys[i];
+        verify_csr(U_, Au, AuT, xuo.data(), yuo.data(), rep);
+    };
+    auto keep_best = [&](const std::vector<double>& xv, const std::vector<double>& yv, const double* r) {
+        const double mm = merit_of(r);
+        if (mm < best_merit) {
+            best_merit = mm; best_x = xv; best_y = yv;
+            best_eps[0] = r[0]; best_eps[1] = r[1]; best_eps[2] = r[2];
+        }
+    };
+
+    while (attempts < o_.max_iterations) {
+        if (stop && stop(ctx)) break;
+        while (attempts < o_.max_iterations) {
+            ++attempts;
+            const double tau = eta / omega, sigma = eta * omega;
+            hipLaunchKernelGGL(k_plain_primal, dim3(std::max(1, gb_n)), dim3(kBlock), 0, s, n, px,
+                               c.as<double>(), aty.as<double>(), l.as<double>(), u.as<double>(), tau, pxn);
+            launch_spmv(A_, A_.f64(), pxn, paxn, s);
+            hipLaunchKernelGGL(k_plain_dual, dim3(std::max(1, gb_m)), dim3(kBlock), 0, s, m, py,
+                               b.as<double>(), paxn, pax, ineq.as<uint8_t>(), sigma, pyn);
+            hipLaunchKernelGGL(k_plain_stats_n, dim3(rb_n), dim3(kBlock), 0, s, n, px, pxn, stats.as<double>());
+            hipLaunchKernelGGL(k_plain_stats_m, dim3(rb_m), dim3(kBlock), 0, s, m, py, pyn, pax, paxn,
+                               stats.as<double>());
+            SUPR_HIP(hipGetLastError());
+            SUPR_HIP(hipMemcpyAsync(stats_h, stats.as<double>(), sizeof(stats_h), hipMemcpyDeviceToHost, s));
+            SUPR_HIP(hipStreamSynchronize(s));
+            double dx2 = 0.0, dy2 = 0.0, inter = 0.0;
+            for (int k = 0; k < rb_n; ++k) dx2 += stats_h[k];
+            for (int k = 0; k < rb_m; ++k) { dy2 += stats_h[kMaxPlain + k]; inter += stats_h[2 * kMaxPlain + k]; }
+            const double num = omega * dx2 + dy2 / omega;
+            const double eta_bar = std::fabs(inter) > 0 ? num / (2.0 * std::fabs(inter)) : INFINITY;
+            const double k1 = accepted + 2.0;
+            const double grow = (1.0 + std::pow(k1, -0.6)) * eta;
+            const double eta_next = std::isfinite(eta_bar) ? smin((1.0 - std::pow(k1, -0.3)) * eta_bar, grow) : grow;
+            if (eta <= eta_bar) {
+                std::swap(px, pxn); std::swap(py, pyn); std::swap(pax, paxn);
+                launch_spmv(AT_, AT_.f64(), py, aty.as<double>(), s);
+                hipLaunchKernelGGL(k_plain_axpy, dim3(std::max(1, gb_n)), dim3(kBlock), 0, s, n, eta, px, xsum.as<double>());
+                hipLaunchKernelGGL(k_plain_axpy, dim3(std::max(1, gb_m)), dim3(kBlock), 0, s, m, eta, py, ysum.as<double>());
+                wsum += eta;
+                eta = eta_next;
+                ++accepted; ++since_restart;
+                break;
+            }
+            eta = eta_next;
+        }
+        if (accepted % o_.check_every != 0 && attempts < o_.max_iterations) continue;
+
+        ++checks;
+        SUPR_HIP(hipMemcpyAsync(hx.data(), px, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(hy.data(), py, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(hxs.data(), xsum.as<double>(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(hys.data(), ysum.as<double>(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+        if (wsum > 0) { for (auto& v : hxs) v /= wsum; for (auto& v : hys) v /= wsum; }
+        else { hxs = hx; hys = hy; }
+        unscaled_kkt(hx, hy, xu, yu, rc);
+        unscaled_kkt(hxs, hys, xu2, yu2, ra);
+        keep_best(xu, yu, rc);
+        keep_best(xu2, yu2, ra);
+        const bool use_avg = merit_of(ra) < merit_of(rc);
+        const double* rep = use_avg ? ra : rc;
+        const double mc = merit_of(rep);
+        if (smax(smax(rep[0], rep[1]), rep[2]) <= o_.tol) {
+            res_x = use_avg ? xu2 : xu; res_y = use_avg ? yu2 : yu;
+            res_eps[0] = rep[0]; res_eps[1] = rep[1]; res_eps[2] = rep[2];
+            converged = true;
+            break;
+        }
+        const bool sufficient = mc <= 0.2 * merit_last_restart;
+        const bool necessary = mc <= 0.8 * merit_last_restart && mc > merit_prev_cand;
+        const bool artificial = since_restart >= 0.36 * accepted;
+        merit_prev_cand = mc;
+        if (sufficient || necessary || artificial) {
+            const std::vector<double>& cx = use_avg ? hxs : hx;
+            const std::vector<double>& cy = use_avg ? hys : hy;
+            double ddx = 0.0, ddy = 0.0;
+            for (int j = 0; j < n; ++j) ddx += (cx[j] - hx_last[j]) * (cx[j] - hx_last[j]);
+            for (int i = 0; i < m; ++i) ddy += (cy[i] - hy_last[i]) * (cy[i] - hy_last[i]);
+            ddx = std::sqrt(ddx); ddy = std::sqrt(ddy);
+            if (ddx > 1e-10 && ddy > 1e-10) omega = std::exp(0.5 * std::log(ddy / ddx) + 0.5 * std::log(omega));
+            hx_last = cx; hy_last = cy;
+            if (use_avg) {
+                SUPR_HIP(hipMemcpyAsync(px, cx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+                SUPR_HIP(hipMemcpyAsync(py, cy.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+                launch_spmv(A_, A_.f64(), px, pax, s);
+                launch_spmv(AT_, AT_.f64(), py, aty.as<double>(), s);
+            }
+            SUPR_HIP(hipMemsetAsync(xsum.as<void>(), 0, sizeof(double) * n, s));
+            SUPR_HIP(hipMemsetAsync(ysum.as<void>(), 0, sizeof(double) * m, s));
+            wsum = 0.0;
+            since_restart = 0; ++restarts;
+            merit_last_restart = mc;
+            merit_prev_cand = INFINITY;
+        }
+    }
+    SUPR_HIP(hipStreamSynchronize(s));
+    bool has_point = converged;
+    if (!converged) {
+        if (!std::isfinite(best_merit)) {
+            SUPR_HIP(hipMemcpy(hx.data(), px, sizeof(double) * n, hipMemcpyDeviceToHost));
+            SUPR_HIP(hipMemcpy(hy.data(), py, sizeof(double) * m, hipMemcpyDeviceToHost));
+            unscaled_kkt(hx, hy, xu, yu, rc);
+            keep_best(xu, yu, rc);
+        }
+        has_point = std::isfinite(best_merit);
+        res_x = best_x; res_y = best_y;
+        std::copy(best_eps, best_eps + 3, res_eps);
+    }
+    if (has_point) {
+        std::copy(res_x.begin(), res_x.end(), x_out);
+        std::copy(res_y.begin(), res_y.end(), y_out);
+    }
+    info->iterations = attempts;
+    info->restarts = restarts;
+    info->converged = converged ? 1 : 0;
+    info->has_point = has_point ? 1 : 0;
+    info->eps_P = res_eps[0]; info->eps_D = res_eps[1]; info->eps_G = res_eps[2];
+    double obj = U_.obj_offset;
+    if (has_point) { obj = 0.0; for (int j = 0; j < n; ++j) obj += U_.c[j] * x_out[j]; obj += U_.obj_offset; }
+    info->objective = obj;
+    info->iterations_fp32 = 0;
+    info->checks = checks;
+    info->lanes_A = A_.lanes; info->lanes_AT = AT_.lanes;
+    info->seconds_setup = t_setup;
+    info->seconds_solve = seconds_since(t1);
+}
+
+// ---------------------------------------------------- device-resident ---
+
+template <typename T>
+void Solver::alloc_iter(Iter<T>& I, bool init_point) {
+    hipStream_t s = stream_.s;
+    const int n = n_, m = m_;
+    std::vector<T> c(n), l(n), u(n), b(m), x0(n);
+    for (int j = 0; j < n; ++j) {
+        c[j] = (T)S_.c[j]; l[j] = (T)S_.l[j]; u[j] = (T)S_.u[j]; x0[j] = (T)x0_[j];
+    }
+    for (int i = 0; i < m; ++i) b[i] = (T)S_.rL[i];
+    I.c = upload(c, s); I.l = upload(l, s); I.u = upload(u, s); I.b = upload(b, s);
+    for (int k = 0; k < 2; ++k) {
+        I.x[k] = device_vec<T>(n);
+        I.y[k] = device_vec<T>(m);
+        I.ax[k] = device_vec<T>(m);
+    }
+    I.aty = device_vec<T>(n);
+    I.xsum = device_vec<T>(n);
+    I.ysum = device_vec<T>(m);
+    I.xlast = device_vec<T>(n);
+    I.ylast = device_vec<T>(m);
+    if (init_point) {
+        SUPR_HIP(hipMemcpyAsync(I.x[0].template as<T>(), x0.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
+        SUPR_HIP(hipMemcpyAsync(I.xlast.template as<T>(), x0.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
+        SUPR_HIP(hipMemsetAsync(I.y[0].template as<void>(), 0, sizeof(T) * m, s));
+        SUPR_HIP(hipMemsetAsync(I.ylast.template as<void>(), 0, sizeof(T) * m, s));
+        SUPR_HIP(hipMemsetAsync(I.xsum.template as<void>(), 0, sizeof(T) * n, s));
+        SUPR_HIP(hipMemsetAsync(I.ysum.template as<void>(), 0, sizeof(T) * m, s));
+    }
+    SUPR_HIP(hipStreamSynchronize(s));   // host staging vectors die here
+}
+
+void Solver::setup_device_resident(bool mixed) {
+    hipStream_t s = stream_.s;
+    const int n = n_, m = m_;
+    setup_matrices(mixed, true, true);
+    initial_point();
+    std::vector<uint8_t> hineq(m);
+    for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
+    ineq_ = upload(hineq, s);
+    if (mixed) alloc_iter(F_, true);
+    alloc_iter(D_, !mixed);
+    auto vec = [&](const double* p, int len) { return upload(std::vector<double>(p, p + len), s); };
+    kc_ = vec(U_.c, n); kl_ = vec(U_.l, n); ku_ = vec(U_.u, n);
+    krL_ = vec(U_.rL, m); krU_ = vec(U_.rU, m);
+    kDr_ = vec(Dr_, m); kDc_ = vec(Dc_, n);
+    xu_ = device_vec<double>(n); xu2_ = device_vec<double>(n); xavg_ = device_vec<double>(n);
+    yu_ = device_vec<double>(m); yu2_ = device_vec<double>(m); yavg_ = device_vec<double>(m);
+    bestx_ = device_vec<double>(n); besty_ = device_vec<double>(m);
+    K_.c = kc_.as<double>(); K_.l = kl_.as<double>(); K_.u = ku_.as<double>();
+    K_.rL = krL_.as<double>(); K_.rU = krU_.as<double>();
+    K_.Dr = kDr_.as<double>(); K_.Dc = kDc_.as<double>();
+    K_.xu = xu_.as<double>(); K_.xu2 = xu2_.as<double>(); K_.xavg = xavg_.as<double>();
+    K_.yu = yu_.as<double>(); K_.yu2 = yu2_.as<double>(); K_.yavg = yavg_.as<double>();
+    K_.best_x = bestx_.as<double>(); K_.best_y = besty_.as<double>();
+    // partial-sum scratch: 1 + 2 + 8 + 6 + 1 + 1 slots per block
+    parts_ = device_vec<double>((size_t)19 * kMaxBlocks);
+    double* p = parts_.as<double>();
+    part_n_ = p; p += kMaxBlocks;
+    part_m_ = p; p += 2 * kMaxBlocks;
+    part_kn_ = p; p += 8 * kMaxBlocks;
+    part_km_ = p; p += 6 * kMaxBlocks;
+    part_rn_ = p; p += kMaxBlocks;
+    part_rm_ = p;
+    state_ = device_vec<DevState>(1);
+
+    // SpMV schedules, timed in the precision of the first loop
+    auto tune = [&](DevMatrix& M, int fixed, auto view, auto* xv, auto* yv) {
+        if (M.ell_w > 0) M.lanes = 1;
+        else if (fixed > 0) M.lanes = fixed;
+        else autotune_lanes(M, view, xv, yv, s);
+    };
+    if (mixed) {
+        tune(A_, o_.spmv_lanes_A, A_.f32(), F_.x[1].as<float>(), F_.ax[1].as<float>());
+        tune(AT_, o_.spmv_lanes_AT, AT_.f32(), F_.y[1].as<float>(), F_.aty.as<float>());
+    } else {
+        tune(A_, o_.spmv_lanes_A, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>());
+        tune(AT_, o_.spmv_lanes_AT, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>());
+    }
+    if (!valid_lanes(A_.lanes) || !valid_lanes(AT_.lanes)) throw BadInput("bad lanes-per-row option");
+    DevState st = initial_state(mixed);
+    SUPR_HIP(hipMemcpyAsync(state_.as<void>(), &st, sizeof(st), hipMemcpyHostToDevice, s));
+    SUPR_HIP(hipStreamSynchronize(s));
+}
+
+DevState Solver::initial_state(bool mixed) const {
+    DevState st;
+    std::memset(&st, 0, sizeof(st));
+    st.eta = eta0_;
+    st.omega = omega0_;
+    st.merit_last_restart = INFINITY;
+    st.merit_prev_cand = INFINITY;
+    st.best_merit = INFINITY;
+    st.stall_ref = INFINITY;
+    st.tol = o_.tol;
+    double sw = o_.switch_tol;
+    if (sw <= 0) sw = o_.tol < 1e-5 ? 1e-5 : 0.0;   // FP32 reaches ~1e-5; looser tolerances finish in FP32
+    st.switch_tol = sw;
+    st.stall_checks = o_.stall_checks > 0 ? o_.stall_checks : 6;
+    st.denom_p = 1.0 + std::max(finite_inf_norm(U_.rL, m_), finite_inf_norm(U_.rU, m_));
+    st.denom_d = 1.0 + finite_inf_norm(U_.c, n_);
+    st.obj_offset = U_.obj_offset;
+    st.max_iter = o_.max_iterations;
+    st.check_every = o_.check_every;
+    st.active = 1;
+    st.need_aty = 1;
+    st.need_ax = 1;
+    st.phase = mixed ? 0 : 1;
+    return st;
+}
+
+template <typename T> MatView<T> iter_view(const DevMatrix& M);
+template <> MatView<float> iter_view<float>(const DevMatrix& M) { return M.f32(); }
+template <> MatView<double> iter_view<double>(const DevMatrix& M) { return M.f64(); }
+
+template <typename T>
+void Solver::enqueue_attempts(const Iter<T>& I, int count) {
+    hipStream_t s = stream_.s;
+    DevState* st = state_.as<DevState>();
+    const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
+    const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
+    const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
+    for (int k = 0; k < count; ++k) {
+        SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, st, AT, B, part_n_));
+        SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, st, A, B, part_m_));
+        hipLaunchKernelGGL(k_decide, dim3(1), dim3(kBlock), 0, s, st, part_n_, gn, part_m_, gm);
+    }
+    SUPR_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Solver::enqueue_ax_fix(const Iter<T>& I) {
+    hipStream_t s = stream_.s;
+    const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
+    const MatView<T> A = iter_view<T>(A_);
+    SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_ax_fix<LL, T>), dim3(grid_rows(m_, LL)), dim3(kBlock), 0, s,
+                                            state_.as<DevState>(), A, B));
+    SUPR_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Solver::enqueue_check(const Iter<T>& I) {
+    hipStream_t s = stream_.s;
+    DevState* st = state_.as<DevState>();
+    const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
+    const int en = grid_elems(n_), em = grid_elems(m_);
+    hipLaunchKernelGGL((k_kkt_prep_n<T>), dim3(en), dim3(kBlock), 0, s, st, n_, B, K_);
+    hipLaunchKernelGGL((k_kkt_prep_m<T>), dim3(em), dim3(kBlock), 0, s, st, m_, B, K_);
+    const int gkn = grid_rows(n_, AT_.lanes), gkm = grid_rows(m_, A_.lanes);
+    const MatView<double> AuT = AT_.unscaled(), Au = A_.unscaled();
+    SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_kkt_n<LL>), dim3(gkn), dim3(kBlock), 0, s, st, AuT, K_, part_kn_));
+    SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_kkt_m<LL>), dim3(gkm), dim3(kBlock), 0, s, st, Au, K_, part_km_));
+    hipLaunchKernelGGL(k_kkt_fin, dim3(1), dim3(kBlock), 0, s, st, part_kn_, gkn, part_km_, gkm);
+    hipLaunchKernelGGL((k_restart_n<T>), dim3(en), dim3(kBlock), 0, s, st, n_, B, K_, part_rn_);
+    hipLaunchKernelGGL((k_restart_m<T>), dim3(em), dim3(kBlock), 0, s, st, m_, B, K_, part_rm_);
+    hipLaunchKernelGGL(k_restart_fin, dim3(1), dim3(kBlock), 0, s, st, part_rn_, en, part_rm_, em);
+    SUPR_HIP(hipGetLastError());
+    enqueue_ax_fix(I);
+}
+
+template <typename T>
+hipGraphExec_t Solver::capture_graph(const Iter<T>& I, int attempts) {
+    hipStream_t s = stream_.s;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    SUPR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_attempts(I, attempts);
+        enqueue_check(I);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    SUPR_HIP(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SUPR_HIP(e);
+    return exec;
+}
+
+// Replays the graph until the device reports done / a precision switch, or
+// the stop callback fires.  A ring of pinned state copies keeps a few
+// replays in flight while the host polls the oldest one.
+template <typename T>
+int Solver::run_graph_loop(const Iter<T>& I, supr_stop_fn stop, void* ctx, DevState& hs,
+                           bool& stopped) {
+    hipStream_t s = stream_.s;
+    int G = o_.graph_attempts > 0 ? o_.graph_attempts : 16;
+    hipGraphExec_t exec = capture_graph(I, G);
+    constexpr int R = 4;
+    DevState* ring = nullptr;
+    hipEvent_t ev[R] = {};
+    int replays = 0;
+    try {
+        SUPR_HIP(hipHostMalloc((void**)&ring, sizeof(DevState) * R, hipHostMallocDefault));
+        for (auto& e : ev) SUPR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        int last_attempts = -1, last_checks = -1, idle = 0;
+        while (true) {
+            if (replays >= R) {
+                const int slot = replays % R;
+                SUPR_HIP(hipEventSynchronize(ev[slot]));
+                hs = ring[slot];
+                if (hs.done || hs.switch_req) break;
+                if (stop && stop(ctx)) { stopped = true; break; }
+                // A replay always advances attempts or checks; a loop that
+                // stops moving is a logic error -- fail instead of spinning.
+                if (hs.attempts == last_attempts && hs.checks == last_checks) {
+                    if (++idle > 64) throw std::runtime_error("device loop made no progress");
+                } else {
+                    idle = 0;
+                }
+                last_attempts = hs.attempts;
+                last_checks = hs.checks;
+            }
+            const int slot = replays % R;
+            SUPR_HIP(hipGraphLaunch(exec, s));
+            SUPR_HIP(hipMemcpyAsync(&ring[slot], state_.as<void>(), sizeof(DevState), hipMemcpyDeviceToHost, s));
+            SUPR_HIP(hipEventRecord(ev[slot], s));
+            ++replays;
+        }
+        SUPR_HIP(hipStreamSynchronize(s));
+        SUPR_HIP(hipMemcpy(&hs, state_.as<void>(), sizeof(DevState), hipMemcpyDeviceToHost));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (ring) (void)hipHostFree(ring);
+        (void)hipGraphExecDestroy(exec);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipHostFree(ring);
+    (void)hipGraphExecDestroy(exec);
+    return replays;
+}
+
+void Solver::solve_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
+                                   double* y_out, supr_pdlp_info* info) {
+    const auto t0 = Clock::now();
+    hipStream_t s = stream_.s;
+    setup_device_resident(mixed);
+    if (mixed) enqueue_ax_fix(F_);
+    else enqueue_ax_fix(D_);
+    SUPR_HIP(hipStreamSynchronize(s));
+    const double t_setup = seconds_since(t0);
+    const auto t1 = Clock::now();
+
+    DevState hs;
+    bool stopped = false;
+    int fp32_attempts = 0;
+    if (mixed) {
+        run_graph_loop(F_, stop, ctx, hs, stopped);
+        fp32_attempts = hs.attempts;
+        if (o_.verbose)
+            std::fprintf(stderr, "[suprema] fp32 loop: attempts=%d restarts=%d checks=%d merit=%.3e best=%.3e switch=%d done=%d\n",
+                         hs.attempts, hs.restarts, hs.checks, hs.cand_merit, hs.best_merit, hs.switch_req, hs.done);
+        if (hs.switch_req && !stopped) {
+            // Continue in FP64 from the FP32 loop's candidate, as a restart.
+            const IterBufs<float> Fv = F_.view(ineq_.as<uint8_t>());
+            const IterBufs<double> Dv = D_.view(ineq_.as<uint8_t>());
+            hipLaunchKernelGGL(k_switch_n, dim3(grid_elems(n_)), dim3(kBlock), 0, s, state_.as<DevState>(), n_, Fv, Dv, K_);
+            hipLaunchKernelGGL(k_switch_m, dim3(grid_elems(m_)), dim3(kBlock), 0, s, state_.as<DevState>(), m_, Fv, Dv, K_);
+            SUPR_HIP(hipGetLastError());
+            hs.phase = 1;
+            hs.cur = 0;
+            hs.switch_req = 0;
+            hs.wsum = 0.0;
+            hs.since_restart = 0;
+            hs.restarts += 1;
+            hs.merit_last_restart = hs.cand_merit;
+            hs.merit_prev_cand = INFINITY;
+            hs.pending = 0;
+            hs.need_check = 0;
+            hs.need_aty = 1;
+            hs.need_ax = 1;
+            hs.active = 1;
+            hs.stall = 0;
+            SUPR_HIP(hipMemcpyAsync(state_.as<void>(), &hs, sizeof(hs), hipMemcpyHostToDevice, s));
+            // FP64 schedules (the FP64 stream is twice the bytes; re-time)
+            if (o_.spmv_lanes_A <= 0 && A_.ell_w == 0) autotune_lanes(A_, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>(), s);
+            if (o_.spmv_lanes_AT <= 0 && AT_.ell_w == 0) autotune_lanes(AT_, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>(), s);
+            enqueue_ax_fix(D_);
+            run_graph_loop(D_, stop, ctx, hs, stopped);
+        }
+    } else {
+        run_graph_loop(D_, stop, ctx, hs, stopped);
+    }
+    const double t_solve = seconds_since(t1);
+
+    bool has_point = hs.has_best != 0;
+    if (has_point) {
+        SUPR_HIP(hipMemcpy(x_out, bestx_.as<void>(), sizeof(double) * n_, hipMemcpyDeviceToHost));
+        SUPR_HIP(hipMemcpy(y_out, besty_.as<void>(), sizeof(double) * m_, hipMemcpyDeviceToHost));
+    } else {
+        // No check ran yet (stopped early): judge the current point on the host.
+        std::vector<double> xh(n_), yh(m_);
+        if (hs.phase == 0) {
+            std::vector<float> xf(n_), yf(m_);
+            SUPR_HIP(hipMemcpy(xf.data(), F_.x[hs.cur].as<void>(), sizeof(float) * n_, hipMemcpyDeviceToHost));
+            SUPR_HIP(hipMemcpy(yf.data(), F_.y[hs.cur].as<void>(), sizeof(float) * m_, hipMemcpyDeviceToHost));
+            for (int j = 0; j < n_; ++j) xh[j] = Dc_[j] * (double)xf[j];
+            for (int i = 0; i < m_; ++i) yh[i] = Dr_[i] * (double)yf[i];
+        } else {
+            SUPR_HIP(hipMemcpy(xh.data(), D_.x[hs.cur].as<void>(), sizeof(double) * n_, hipMemcpyDeviceToHost));
+            SUPR_HIP(hipMemcpy(yh.data(), D_.y[hs.cur].as<void>(), sizeof(double) * m_, hipMemcpyDeviceToHost));
+            for (int j = 0; j < n_; ++j) xh[j] *= Dc_[j];
+            for (int i = 0; i < m_; ++i) yh[i] *= Dr_[i];
+        }
+        double rep[7];
+        const Csr Au = build_csr(U_, U_.val, false), AuT = build_csr(U_, U_.val, true);
+        verify_csr(U_, Au, AuT, xh.data(), yh.data(), rep);
+        if (std::isfinite(merit_of(rep))) {
+            has_point = true;
+            std::copy(xh.begin(), xh.end(), x_out);
+            std::copy(yh.begin(), yh.end(), y_out);
+            hs.best_eps[0] = rep[0]; hs.best_eps[1] = rep[1]; hs.best_eps[2] = rep[2];
+        }
+    }
+    info->iterations = hs.attempts;
+    info->restarts = hs.restarts;
+    info->converged = hs.done == 1 ? 1 : 0;
+    info->has_point = has_point ? 1 : 0;
+    info->eps_P = hs.best_eps[0]; info->eps_D = hs.best_eps[1]; info->eps_G = hs.best_eps[2];
+    double obj = U_.obj_offset;
+    if (has_point) { obj = 0.0; for (int j = 0; j < n_; ++j) obj += U_.c[j] * x_out[j]; obj += U_.obj_offset; }
+    info->objective = obj;
+    info->iterations_fp32 = fp32_attempts;
+    info->checks = hs.checks;
+    info->lanes_A = A_.lanes; info->lanes_AT = AT_.lanes;
+    info->seconds_setup = t_setup;
+    info->seconds_solve = t_solve;
+    if (o_.verbose)
+        std::fprintf(stderr, "[suprema] done: attempts=%d (fp32 %d) restarts=%d checks=%d eps=%.2e/%.2e/%.2e conv=%d solve=%.4fs setup=%.4fs lanes A=%d AT=%d ell A=%d AT=%d\n",
+                     hs.attempts, fp32_attempts, hs.restarts, hs.checks, info->eps_P, info->eps_D, info->eps_G,
+                     info->converged, t_solve, t_setup, A_.lanes, AT_.lanes, A_.ell_w, AT_.ell_w);
+}
+
+void Solver::solve(supr_stop_fn stop, void* ctx, double* x_out, double* y_out,
+                   supr_pdlp_info* info) {
+    const auto t0 = Clock::now();
+    switch (o_.mode) {
+        case SUPR_MODE_PLAIN_FP64: solve_plain(stop, ctx, x_out, y_out, info); break;
+        case SUPR_MODE_FAST_FP64: solve_device_resident(false, stop, ctx, x_out, y_out, info); break;
+        case SUPR_MODE_MIXED: solve_device_resident(true, stop, ctx, x_out, y_out, info); break;
+        default: throw BadInput("unknown mode " + std::to_string(o_.mode));
+    }
+    info->seconds_total = seconds_since(t0);
+}
+
+// Standalone timing of the two iteration kernels at their real shapes (the
+// kernels the graph replays), for the roofline figure:
+//   out[0] ms per k_primal, out[1] ms per k_dual, out[2] ms per k_primal
+//   (FP64), out[3] ms per k_dual (FP64), out[4] lanes A, out[5] lanes A^T,
+//   out[6] ELL width of A, out[7] ELL width of A^T.
+// The loop control is opened up (no checks, no limit) so every launch does
+// a full accepted-step's work: A^T y, averages, projections, A x.
+void Solver::bench_kernels(int reps, double* out) {
+    hipStream_t s = stream_.s;
+    setup_device_resident(true);
+    enqueue_ax_fix(F_);
+    DevState st = initial_state(true);
+    st.check_every = std::numeric_limits<int32_t>::max();
+    st.max_iter = std::numeric_limits<int32_t>::max();
+    hipEvent_t e0, e1;
+    SUPR_HIP(hipEventCreate(&e0));
+    SUPR_HIP(hipEventCreate(&e1));
+    auto time_kernel = [&](auto&& launch) {
+        for (int r = 0; r < 3; ++r) launch();
+        SUPR_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) launch();
+        SUPR_HIP(hipEventRecord(e1, s));
+        SUPR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SUPR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / reps;
+    };
+    auto run = [&](auto& I, auto tag, int slot) {
+        using T = decltype(tag);
+        const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
+        const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
+        const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
+        DevState* d = state_.as<DevState>();
+        SUPR_HIP(hipMemcpyAsync(d, &st, sizeof(st), hipMemcpyHostToDevice, s));
+        // need_aty and pending stay set: every k_primal does the accepted-step work
+        DevState st2 = st;
+        st2.pending = 1;
+        st2.eta_acc = st.eta;
+        SUPR_HIP(hipMemcpyAsync(d, &st2, sizeof(st2), hipMemcpyHostToDevice, s));
+        out[slot] = time_kernel([&] {
+            SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, d, AT, B, part_n_));
+        });
+        out[slot + 1] = time_kernel([&] {
+            SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, d, A, B, part_m_));
+        });
+        SUPR_HIP(hipGetLastError());
+    };
+    run(F_, float{}, 0);
+    // FP64 buffers were left uninitialised in the mixed setup: give them a point
+    {
+        std::vector<double> x0 = x0_;
+        SUPR_HIP(hipMemcpyAsync(D_.x[0].as<void>(), x0.data(), sizeof(double) * n_, hipMemcpyHostToDevice, s));
+        SUPR_HIP(hipMemcpyAsync(D_.x[1].as<void>(), x0.data(), sizeof(double) * n_, hipMemcpyHostToDevice, s));
+        SUPR_HIP(hipMemsetAsync(D_.y[0].as<void>(), 0, sizeof(double) * m_, s));
+        SUPR_HIP(hipMemsetAsync(D_.y[1].as<void>(), 0, sizeof(double) * m_, s));
+        SUPR_HIP(hipMemsetAsync(D_.ax[0].as<void>(), 0, sizeof(double) * m_, s));
+        SUPR_HIP(hipMemsetAsync(D_.ax[1].as<void>(), 0, sizeof(double) * m_, s));
+        SUPR_HIP(hipMemsetAsync(D_.xsum.as<void>(), 0, sizeof(double) * n_, s));
+        SUPR_HIP(hipMemsetAsync(D_.ysum.as<void>(), 0, sizeof(double) * m_, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+    }
+    if (A_.ell_w == 0) autotune_lanes(A_, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>(), s);
+    if (AT_.ell_w == 0) autotune_lanes(AT_, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>(), s);
+    run(D_, double{}, 2);
+    SUPR_HIP(hipStreamSynchronize(s));
+    out[4] = A_.lanes; out[5] = AT_.lanes; out[6] = A_.ell_w; out[7] = AT_.ell_w;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+int map_exception() {
+    try {
+        throw;
+    } catch (const BadInput& e) {
+        set_error(e.what());
+        return SUPR_EINVAL;
+    } catch (const HipFailure& e) {
+        set_error(e.what());
+        return SUPR_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return SUPR_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SUPR_EINTERNAL;
+    } catch (...) {
+        set_error("unknown exception");
+        return SUPR_EINTERNAL;
+    }
+}
+
+int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const double* Dc,
+                     const supr_pdlp_options* opt) {
+    if (!check_lp(U, "unscaled_form") || !check_lp(S, "scaled_form")) return SUPR_EINVAL;
+    if (U->m != S->m || U->n != S->n || U->nnz != S->nnz) {
+        set_error("unscaled and scaled forms differ in shape");
+        return SUPR_EINVAL;
+    }
+    for (int k = 0; k < S->nnz; ++k)
+        if (U->row_idx[k] != S->row_idx[k] || U->col_idx[k] != S->col_idx[k]) {
+            set_error("unscaled and scaled forms must share one COO pattern (entry " + std::to_string(k) + ")");
+            return SUPR_EINVAL;
+        }
+    if (S->m < 1 || S->n < 1) {
+        set_error("the PDLP form needs at least one row and one column");
+        return SUPR_EINVAL;
+    }
+    if (!Dr || !Dc || !opt) { set_error("null Dr, Dc or options"); return SUPR_EINVAL; }
+    if (opt->max_iterations < 0 || opt->check_every < 1 || !(opt->tol >= 0)) {
+        set_error("bad options: max_iterations >= 0, check_every >= 1, tol >= 0 required");
+        return SUPR_EINVAL;
+    }
+    if ((opt->spmv_lanes_A != 0 && !valid_lanes(opt->spmv_lanes_A)) ||
+        (opt->spmv_lanes_AT != 0 && !valid_lanes(opt->spmv_lanes_AT))) {
+        set_error("spmv lanes must be 0 (auto), a power of two in [1, 64], or 256");
+        return SUPR_EINVAL;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        set_error("no HIP device visible");
+        return SUPR_ENODEV;
+    }
+    if (opt->device < 0 || opt->device >= count) {
+        set_error("device ordinal out of range");
+        return SUPR_EINVAL;
+    }
+    return SUPR_OK;
+}
+
+}  // namespace
+}  // namespace supr
+
+extern "C" {
+
+int supr_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    return count;
+}
+
+void supr_pdlp_default_options(supr_pdlp_options* opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->mode = SUPR_MODE_MIXED;
+    opt->max_iterations = 200000;
+    opt->check_every = 64;
+    opt->tol = 1e-6;
+}
+
+int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
+                        const double* Dr, const double* Dc, const supr_pdlp_options* opt,
+                        supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
+                        supr_pdlp_info* info) {
+    using namespace supr;
+    int rc = check_solve_args(unscaled_form, scaled_form, Dr, Dc, opt);
+    if (rc != SUPR_OK) return rc;
+    if (!x_out || !y_out || !info) { set_error("null output"); return SUPR_EINVAL; }
+    try {
+        SUPR_HIP(hipSetDevice(opt->device));
+        std::memset(info, 0, sizeof(*info));
+        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt);
+        solver.solve(stop, stop_ctx, x_out, y_out, info);
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+// Times the iteration kernels (see Solver::bench_kernels).  out: 8 doubles.
+int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
+                       const double* Dr, const double* Dc, const supr_pdlp_options* opt,
+                       int32_t reps, double* out8) {
+    using namespace supr;
+    int rc = check_solve_args(unscaled_form, scaled_form, Dr, Dc, opt);
+    if (rc != SUPR_OK) return rc;
+    if (!out8 || reps < 1) { set_error("null output or reps < 1"); return SUPR_EINVAL; }
+    try {
+        SUPR_HIP(hipSetDevice(opt->device));
+        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt);
+        solver.bench_kernels(reps, out8);
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,802 @@
+// pdlp_kernels.hpp -- CDNA4 (gfx950, wave64) kernels of the restarted-PDHG
+// engine.  Included by pdlp_engine.hip only.
+//
+// Two families:
+//  * plain kernels (k_spmv, k_plain_*): one operation per launch, the shape of
+//    the reference GPU backend (gpu/pdlp_gpu_solve.cu:42-138) -- used by the
+//    host-driven plain FP64 mode and by the SpMV autotuner;
+//  * device-resident kernels (k_primal, k_dual, k_kkt_*, k_restart_*): the
+//    whole PDHG loop of include/pdlp_algo.hpp:103-175 with its step-size rule,
+//    ergodic averages, KKT check and restart rule evaluated on the device.
+//    Control flow lives in a DevState block in device memory; each kernel
+//    reads it at entry and returns at once when it has nothing to do, so a
+//    fixed sequence of launches (a replayed hipGraph) can run a variable
+//    number of accepted / rejected steps and checks without a host round trip.
+//
+// SpMV schedules.  A row is handled by a group of L lanes: L = 1, 2, ..., 64
+// lanes of one wavefront (lanes stride the row's nonzeros, an xor-shuffle
+// tree combines them), or L = 256, the whole block (the LDS-staged row split
+// for long rows: four wavefront sums combined through LDS).  Matrices whose rows all have the same short
+// length (A^T of a transportation LP: 2 per row) are stored as ELL instead
+// (column-major slots, coalesced across the wavefront, no row pointers).  The
+// engine picks L per matrix by timing the candidates (autotune) or from the
+// mean row length.
+//
+// Reductions are deterministic: each block writes its partial sums and a
+// one-block kernel adds them in a fixed order.  The hot kernels carry no
+// atomics and no fences: kernel boundaries order everything.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace supr {
+
+constexpr int kBlock = 256;       // 4 wavefronts
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBlocks = 2048;  // grid-stride cap: 8 blocks per CU on 256 CUs
+
+// std::max / std::min semantics (ties and signed zeros as libstdc++).
+template <typename T> __host__ __device__ __forceinline__ T smax(T a, T b) { return a < b ? b : a; }
+template <typename T> __host__ __device__ __forceinline__ T smin(T a, T b) { return b < a ? b : a; }
+
+// Control block of the device-resident loop (one per LP).
+struct DevState {
+    // step size, primal weight, averaging
+    double eta, omega, wsum, eta_acc;
+    // restart bookkeeping
+    double merit_last_restart, merit_prev_cand, cand_merit;
+    double best_merit, stall_ref;
+    double best_eps[3];
+    double last_eps[3];      // eps of the candidate chosen at the last check
+    double last_obj;
+    double tol, switch_tol;
+    double denom_p, denom_d, obj_offset;
+    int32_t attempts, accepted, since_restart, restarts, checks;
+    int32_t max_iter, check_every, stall_checks, stall;
+    int32_t cur;             // which half of each double buffer is current
+    int32_t active;          // iterations run (0: paused for a check, or finished)
+    int32_t need_aty;        // next k_primal recomputes A^T y
+    int32_t need_ax;         // k_ax_fix recomputes A x
+    int32_t pending;         // last accepted step not yet added to the averages
+    int32_t need_check;
+    int32_t done;            // 1 converged, 2 iteration limit
+    int32_t has_best, best_which, use_avg, restart_now;
+    int32_t switch_req, phase;   // switch_req: reason code (1 merit, 2 stall near floor, 3 long stall); phase 0: FP32 iterates, 1: FP64
+    int32_t pad0, pad1;
+};
+
+template <typename V> struct MatView {
+    int32_t rows;
+    int32_t ell_w;           // > 0: ELL with this width (idx / val column-major)
+    const int32_t* ptr;      // CSR row pointers (unused for ELL)
+    const int32_t* idx;
+    const V* val;
+};
+
+// Iterate buffers of one precision.
+template <typename T> struct IterBufs {
+    T* x[2];
+    T* y[2];
+    T* ax[2];
+    T* aty;
+    T* xsum;
+    T* ysum;
+    T* xlast;
+    T* ylast;
+    const T* c;
+    const T* l;
+    const T* u;
+    const T* b;
+    const uint8_t* ineq;
+};
+
+// Vectors of the FP64 KKT check (unscaled PDLP form).
+struct KktBufs {
+    const double* c;   // unscaled c, l, u (n) and rL, rU (m)
+    const double* l;
+    const double* u;
+    const double* rL;
+    const double* rU;
+    const double* Dr;
+    const double* Dc;
+    double* xu;        // Dc * x, Dc * x_avg
+    double* xu2;
+    double* xavg;      // scaled average
+    double* yu;
+    double* yu2;
+    double* yavg;
+    double* best_x;
+    double* best_y;
+};
+
+// ------------------------------------------------------------- helpers --
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Sums N values over the block; result valid in thread 0.
+template <int N> __device__ __forceinline__ void block_sum(double (&v)[N]) {
+    __shared__ double lds[N][kWaves];
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) lds[q][w] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            double s = lds[q][0];
+            for (int k = 1; k < kWaves; ++k) s += lds[q][k];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Deterministic sum of N-wide partials of nb <= kMaxBlocks blocks by one
+// block (a separate launch, so the producers' stores are visible): thread t
+// loads blocks t, t+256, ... with independent loads, adds them in that order,
+// then a fixed tree.  Valid in thread 0.
+template <int N>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb,
+                                             double (&out)[N]) {
+    constexpr int K = kMaxBlocks / kBlock;
+    double v[K][N];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int b = threadIdx.x + k * kBlock;
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[k][q] = b < nb ? part[(size_t)b * N + q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        out[q] = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[q] += v[k][q];
+    }
+    block_sum<N>(out);
+}
+
+// Sum over a group of L lanes (L <= 64: xor shuffles inside the wavefront;
+// L = 256: the whole block, wavefront sums combined through LDS).  Every lane
+// of the group gets the result.  Called with block-uniform control flow.
+template <int L, typename Acc> __device__ __forceinline__ Acc group_sum(Acc s) {
+    if constexpr (L <= 64) {
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+        return s;
+    } else {
+        static_assert(L == kBlock, "block-wide groups span the whole block");
+        __shared__ Acc red[kWaves];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        Acc t = red[0];
+#pragma unroll
+        for (int k = 1; k < kWaves; ++k) t += red[k];
+        __syncthreads();
+        return t;
+    }
+}
+
+// Row dot product with a group of L lanes; every lane of the group gets the
+// sum.  valid = false gives 0 but still takes part in the group reduction.
+// The CSR loop issues U independent index / value / gather loads before the
+// multiply-adds, so a lane has U gathers in flight instead of one.
+template <int L, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict__ x, int row,
+                                       int lane, bool valid) {
+    constexpr int U = 4;
+    Acc s = Acc(0);
+    if (M.ell_w > 0) {
+        if (valid) {
+            const size_t R = (size_t)M.rows;
+            for (int k = lane; k < M.ell_w; k += L) {
+                const size_t q = (size_t)k * R + row;
+                s += (Acc)M.val[q] * (Acc)x[M.idx[q]];
+            }
+        }
+    } else {
+        int b = 0, e = 0;
+        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+        int p = b + lane;
+        for (; p + (U - 1) * L < e; p += U * L) {
+            int c[U];
+            V v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
+            X xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
+        }
+        for (; p < e; p += L) s += (Acc)M.val[p] * (Acc)x[M.idx[p]];
+    }
+    return group_sum<L, Acc>(s);
+}
+
+// Two right-hand sides sharing the index stream (FP64, KKT check).
+template <int L, typename V>
+__device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __restrict__ x1,
+                                         const double* __restrict__ x2, int row, int lane,
+                                         bool valid, double& s1, double& s2) {
+    constexpr int U = 4;
+    s1 = 0.0; s2 = 0.0;
+    if (M.ell_w > 0) {
+        if (valid) {
+            const size_t R = (size_t)M.rows;
+            for (int k = lane; k < M.ell_w; k += L) {
+                const size_t q = (size_t)k * R + row;
+                const int c = M.idx[q];
+                const double a = (double)M.val[q];
+                s1 += a * x1[c];
+                s2 += a * x2[c];
+            }
+        }
+    } else {
+        int b = 0, e = 0;
+        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+        int p = b + lane;
+        for (; p + (U - 1) * L < e; p += U * L) {
+            int c[U];
+            double a[U], u1[U], u2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = (double)M.val[p + u * L]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { u1[u] = x1[c[u]]; u2[u] = x2[c[u]]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { s1 += a[u] * u1[u]; s2 += a[u] * u2[u]; }
+        }
+        for (; p < e; p += L) {
+            const int c = M.idx[p];
+            const double a = (double)M.val[p];
+            s1 += a * x1[c];
+            s2 += a * x2[c];
+        }
+    }
+    s1 = group_sum<L, double>(s1);
+    s2 = group_sum<L, double>(s2);
+}
+
+// Grid-stride loop over rows, L lanes per row; body(row, lane, valid) runs
+// with block-uniform trip counts so group reductions always see full groups.
+// XCD-aware mapping: the hardware deals blocks round-robin to the 8 XCDs
+// (block b -> XCD b % 8); when the grid is a multiple of 8, block b takes
+// the logical slot (b % 8) * (G / 8) + b / 8, so each XCD sweeps one
+// contiguous range of rows and the x entries those rows gather stay in that
+// XCD's L2.
+constexpr int kXcds = 8;
+template <int L, typename F>
+__device__ __forceinline__ void for_rows(int rows, F&& body) {
+    const int gpb = kBlock / L;
+    const int g = threadIdx.x / L;
+    const int lane = threadIdx.x % L;
+    const int G = gridDim.x;
+    const int lb = (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
+                                    : blockIdx.x;
+    for (int r0 = lb * gpb; r0 < rows; r0 += G * gpb) {
+        const int row = r0 + g;
+        body(row, lane, row < rows);
+    }
+}
+
+// ------------------------------------------------------- plain kernels --
+
+// y = M x  (the reference's sov_spmv_csr_* pair, generalised to L lanes)
+template <int L, typename V, typename X, typename Y>
+__global__ __launch_bounds__(kBlock) void k_spmv(MatView<V> M, const X* __restrict__ x,
+                                                 Y* __restrict__ y) {
+    for_rows<L>(M.rows, [&](int row, int lane, bool valid) {
+        const V s = row_dot<L, V>(M, x, row, lane, valid);
+        if (valid && lane == 0) y[row] = (Y)s;
+    });
+}
+
+// xn = clip(x - tau (c - aty), l, u)          gpu/pdlp_gpu_solve.cu:71-80
+__global__ __launch_bounds__(kBlock) void k_plain_primal(int n, const double* __restrict__ x,
+                                                         const double* __restrict__ c,
+                                                         const double* __restrict__ aty,
+                                                         const double* __restrict__ l,
+                                                         const double* __restrict__ u,
+                                                         double tau, double* __restrict__ xn) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j < n) xn[j] = smin(smax(x[j] - tau * (c[j] - aty[j]), l[j]), u[j]);
+}
+
+// yn = y + sigma (b - (2 axn - ax)), clipped at 0 on inequality rows   :82-92
+__global__ __launch_bounds__(kBlock) void k_plain_dual(int m, const double* __restrict__ y,
+                                                       const double* __restrict__ b,
+                                                       const double* __restrict__ axn,
+                                                       const double* __restrict__ ax,
+                                                       const uint8_t* __restrict__ ineq,
+                                                       double sigma, double* __restrict__ yn) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) {
+        const double v = y[i] + sigma * (b[i] - (2.0 * axn[i] - ax[i]));
+        yn[i] = (ineq[i] && v < 0.0) ? 0.0 : v;
+    }
+}
+
+// Per-block partials of sum (xn-x)^2 and of sum (yn-y)^2, sum (yn-y)(axn-ax):
+// part[b], part[kMaxPlain + b], part[2 kMaxPlain + b]          :105-131
+constexpr int kMaxPlain = 256;
+__global__ __launch_bounds__(kBlock) void k_plain_stats_n(int n, const double* __restrict__ x,
+                                                          const double* __restrict__ xn,
+                                                          double* __restrict__ part) {
+    double v[1] = {0.0};
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const double d = xn[j] - x[j];
+        v[0] += d * d;
+    }
+    block_sum<1>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_plain_stats_m(int m, const double* __restrict__ y,
+                                                          const double* __restrict__ yn,
+                                                          const double* __restrict__ ax,
+                                                          const double* __restrict__ axn,
+                                                          double* __restrict__ part) {
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const double d = yn[i] - y[i];
+        v[0] += d * d;
+        v[1] += d * (axn[i] - ax[i]);
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[kMaxPlain + blockIdx.x] = v[0];
+        part[2 * kMaxPlain + blockIdx.x] = v[1];
+    }
+}
+
+// y += a x                                                     :133-138
+__global__ __launch_bounds__(kBlock) void k_plain_axpy(int len, double a,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ y) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j < len) y[j] += a * x[j];
+}
+
+// ------------------------------------------- device-resident PDHG loop --
+
+// The step-size rule of include/pdlp_algo.hpp:113-131 on the device.
+__device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2,
+                                            double inter) {
+    st->attempts += 1;
+    st->pending = 0;
+    st->need_ax = 0;
+    const double eta = st->eta, omega = st->omega;
+    const double num = omega * dx2 + dy2 / omega;
+    const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : INFINITY;
+    const double k1 = st->accepted + 2.0;
+    const double grow = (1.0 + pow(k1, -0.6)) * eta;
+    const double eta_next = isfinite(eta_bar) ? smin((1.0 - pow(k1, -0.3)) * eta_bar, grow) : grow;
+    if (eta <= eta_bar) {
+        st->cur ^= 1;
+        st->pending = 1;
+        st->eta_acc = eta;
+        st->wsum += eta;
+        st->accepted += 1;
+        st->since_restart += 1;
+        st->need_aty = 1;
+        if (st->accepted % st->check_every == 0) st->need_check = 1;
+    } else {
+        st->need_aty = 0;
+    }
+    st->eta = eta_next;
+    if (st->attempts >= st->max_iter) st->need_check = 1;
+    if (st->need_check) st->active = 0;
+}
+
+// Primal half of one attempt, over the columns (rows of A^T):
+//   [after an accept] aty = A^T y, xsum += eta_acc x  (pending average term)
+//   xn = clip(x - tau (c - aty), l, u);  partial sums of (xn - x)^2 in FP64.
+template <int L, typename T>
+__global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ st, MatView<T> AT,
+                                                   IterBufs<T> B, double* __restrict__ part_n) {
+    if (!st->active) return;
+    const int cur = st->cur;
+    const bool need_aty = st->need_aty != 0;
+    const bool pending = st->pending != 0;
+    const T eta_acc = (T)st->eta_acc;
+    const T tau = (T)(st->eta / st->omega);
+    const T* __restrict__ x = B.x[cur];
+    T* __restrict__ xn = B.x[cur ^ 1];
+    const T* __restrict__ y = B.y[cur];
+    double acc[1] = {0.0};
+    for_rows<L>(AT.rows, [&](int row, int lane, bool valid) {
+        T a = T(0);
+        if (need_aty) {
+            a = row_dot<L, T>(AT, y, row, lane, valid);
+            if (valid && lane == 0) B.aty[row] = a;
+        }
+        if (valid && lane == 0) {
+            if (!need_aty) a = B.aty[row];
+            const T xj = x[row];
+            if (pending) B.xsum[row] += eta_acc * xj;
+            const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
+            xn[row] = v;
+            const double d = (double)v - (double)xj;
+            acc[0] += d * d;
+        }
+    });
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part_n[blockIdx.x] = acc[0];
+}
+
+// Dual half of one attempt, over the rows of A:
+//   axn = A xn;  [pending] ysum += eta_acc y;
+//   yn = proj(y + sigma (b - (2 axn - ax)));  partials of (yn-y)^2 and
+//   (yn-y)(axn-ax).
+template <int L, typename T>
+__global__ __launch_bounds__(kBlock) void k_dual(DevState* __restrict__ st, MatView<T> A,
+                                                 IterBufs<T> B, double* __restrict__ part_m) {
+    if (!st->active) return;
+    const int cur = st->cur;
+    const bool pending = st->pending != 0;
+    const T eta_acc = (T)st->eta_acc;
+    const T sigma = (T)(st->eta * st->omega);
+    const T* __restrict__ xn = B.x[cur ^ 1];
+    const T* __restrict__ y = B.y[cur];
+    T* __restrict__ yn = B.y[cur ^ 1];
+    const T* __restrict__ ax = B.ax[cur];
+    T* __restrict__ axn = B.ax[cur ^ 1];
+    double acc[2] = {0.0, 0.0};
+    for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
+        const T a = row_dot<L, T>(A, xn, row, lane, valid);
+        if (valid && lane == 0) {
+            axn[row] = a;
+            const T yi = y[row];
+            if (pending) B.ysum[row] += eta_acc * yi;
+            const T axi = ax[row];
+            T v = yi + sigma * (B.b[row] - (T(2) * a - axi));
+            if (B.ineq[row] && v < T(0)) v = T(0);
+            yn[row] = v;
+            const double d = (double)v - (double)yi;
+            acc[0] += d * d;
+            acc[1] += d * ((double)a - (double)axi);
+        }
+    });
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part_m[(size_t)blockIdx.x * 2] = acc[0];
+        part_m[(size_t)blockIdx.x * 2 + 1] = acc[1];
+    }
+}
+
+// Step decision of one attempt (one block): sums the primal and dual
+// partials in a fixed order and applies the step-size rule.
+__global__ __launch_bounds__(kBlock) void k_decide(DevState* __restrict__ st,
+                                                   const double* __restrict__ part_n, int nb_n,
+                                                   const double* __restrict__ part_m, int nb_m) {
+    if (!st->active) return;
+    double sn[1], sm[2];
+    sum_partials<1>(part_n, nb_n, sn);
+    sum_partials<2>(part_m, nb_m, sm);
+    if (threadIdx.x == 0) decide_step(st, sn[0], sm[0], sm[1]);
+}
+
+// A x for the current iterate (after a restart to the average, at start-up
+// and after the FP32 -> FP64 switch).
+template <int L, typename T>
+__global__ __launch_bounds__(kBlock) void k_ax_fix(DevState* __restrict__ st, MatView<T> A,
+                                                   IterBufs<T> B) {
+    if (!st->need_ax) return;
+    const T* __restrict__ x = B.x[st->cur];
+    T* __restrict__ ax = B.ax[st->cur];
+    for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
+        const T a = row_dot<L, T>(A, x, row, lane, valid);
+        if (valid && lane == 0) ax[row] = a;
+    });
+}
+
+// KKT check, step 1 (columns): add the pending average term, form the
+// average and the unscaled candidates xu = Dc x, xu2 = Dc x_avg.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_kkt_prep_n(DevState* __restrict__ st, int n,
+                                                       IterBufs<T> B, KktBufs K) {
+    if (!st->need_check) return;
+    const int cur = st->cur;
+    const bool pending = st->pending != 0;
+    const T eta_acc = (T)st->eta_acc;
+    const double wsum = st->wsum;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const T xj = B.x[cur][j];
+        T xs = B.xsum[j];
+        if (pending) { xs += eta_acc * xj; B.xsum[j] = xs; }
+        const double avg = wsum > 0 ? (double)xs / wsum : (double)xj;
+        K.xavg[j] = avg;
+        K.xu[j] = K.Dc[j] * (double)xj;
+        K.xu2[j] = K.Dc[j] * avg;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_kkt_prep_m(DevState* __restrict__ st, int m,
+                                                       IterBufs<T> B, KktBufs K) {
+    if (!st->need_check) return;
+    const int cur = st->cur;
+    const bool pending = st->pending != 0;
+    const T eta_acc = (T)st->eta_acc;
+    const double wsum = st->wsum;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const T yi = B.y[cur][i];
+        T ys = B.ysum[i];
+        if (pending) { ys += eta_acc * yi; B.ysum[i] = ys; }
+        const double avg = wsum > 0 ? (double)ys / wsum : (double)yi;
+        K.yavg[i] = avg;
+        K.yu[i] = K.Dr[i] * (double)yi;
+        K.yu2[i] = K.Dr[i] * avg;
+    }
+}
+
+// KKT check, step 2 (columns of the unscaled A, both candidates): bound
+// violations, dual sign violations of z = c - A^T y, the column part of the
+// dual objective, c'x.  8 partials per block (src/verify.cpp:24-94).
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_kkt_n(DevState* __restrict__ st,
+                                                  MatView<double> AuT, KktBufs K,
+                                                  double* __restrict__ part) {
+    if (!st->need_check) return;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for_rows<L>(AuT.rows, [&](int row, int lane, bool valid) {
+        double s1, s2;
+        row_dot2<L>(AuT, K.yu, K.yu2, row, lane, valid, s1, s2);
+        if (valid && lane == 0) {
+            const double cj = K.c[row], lj = K.l[row], uj = K.u[row];
+            const bool lf = isfinite(lj), uf = isfinite(uj);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const double xv = k ? K.xu2[row] : K.xu[row];
+                const double z = cj - (k ? s2 : s1);
+                const double bv = smax(lj - xv, 0.0) + smax(xv - uj, 0.0);
+                double sv = 0.0;
+                if (!lf && z > 0) sv += z;
+                if (!uf && z < 0) sv += -z;
+                double zv = z, dcol = 0.0;
+                if (!lf && zv > 0) zv = 0.0;
+                if (!uf && zv < 0) zv = 0.0;
+                if (zv > 0 && lf) dcol += lj * zv;
+                if (zv < 0 && uf) dcol += -uj * -zv;
+                acc[4 * k + 0] += bv * bv;
+                acc[4 * k + 1] += sv * sv;
+                acc[4 * k + 2] += dcol;
+                acc[4 * k + 3] += cj * xv;
+            }
+        }
+    });
+    block_sum<8>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) part[(size_t)blockIdx.x * 8 + q] = acc[q];
+    }
+}
+
+// KKT check, step 3 (rows of the unscaled A, both candidates): row
+// violations, sign violations of y on infinite row bounds, the row part of
+// the dual objective.  6 partials per block.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_kkt_m(DevState* __restrict__ st, MatView<double> Au,
+                                                  KktBufs K, double* __restrict__ part) {
+    if (!st->need_check) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for_rows<L>(Au.rows, [&](int row, int lane, bool valid) {
+        double a1, a2;
+        row_dot2<L>(Au, K.xu, K.xu2, row, lane, valid, a1, a2);
+        if (valid && lane == 0) {
+            const double lo = K.rL[row], hi = K.rU[row];
+            const bool lf = isfinite(lo), hf = isfinite(hi);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const double a = k ? a2 : a1;
+                const double yv0 = k ? K.yu2[row] : K.yu[row];
+                const double rv = smax(lo - a, 0.0) + smax(a - hi, 0.0);
+                double sv = 0.0;
+                if (!lf && yv0 > 0) sv += yv0;
+                if (!hf && yv0 < 0) sv += -yv0;
+                double yv = yv0, drow = 0.0;
+                if (!lf && yv > 0) yv = 0.0;
+                if (!hf && yv < 0) yv = 0.0;
+                if (yv > 0 && lf) drow += lo * yv;
+                if (yv < 0 && hf) drow += -hi * -yv;
+                acc[3 * k + 0] += rv * rv;
+                acc[3 * k + 1] += sv * sv;
+                acc[3 * k + 2] += drow;
+            }
+        }
+    });
+    block_sum<6>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) part[(size_t)blockIdx.x * 6 + q] = acc[q];
+    }
+}
+
+__device__ __forceinline__ double merit3(const double* e) {
+    return sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+}
+
+// KKT check, step 4 (one block): eps of both candidates, keep-best,
+// convergence, restart decision (include/pdlp_algo.hpp:135-174), and in the
+// mixed mode the request to continue in FP64.
+__global__ __launch_bounds__(kBlock) void k_kkt_fin(DevState* __restrict__ st,
+                                                    const double* __restrict__ part_n, int nb_n,
+                                                    const double* __restrict__ part_m, int nb_m) {
+    if (!st->need_check) return;
+    double sn[8], sm[6];
+    sum_partials<8>(part_n, nb_n, sn);
+    sum_partials<6>(part_m, nb_m, sm);
+    if (threadIdx.x != 0) return;
+    double eps[2][3], obj[2];
+    for (int k = 0; k < 2; ++k) {
+        const double bv2 = sn[4 * k + 0], svc = sn[4 * k + 1], dcol = sn[4 * k + 2], cx = sn[4 * k + 3];
+        const double rv2 = sm[3 * k + 0], svr = sm[3 * k + 1], drow = sm[3 * k + 2];
+        const double D = drow + dcol;
+        eps[k][0] = sqrt(rv2 + bv2) / st->denom_p;
+        eps[k][1] = sqrt(svr + svc) / st->denom_d;
+        eps[k][2] = isfinite(D) ? fabs(cx - D) / (1.0 + fabs(cx) + fabs(D)) : INFINITY;
+        obj[k] = cx + st->obj_offset;
+    }
+    st->checks += 1;
+    const double mc_cur = merit3(eps[0]), mc_avg = merit3(eps[1]);
+    int bw = 0;
+    if (mc_cur < st->best_merit) { st->best_merit = mc_cur; bw = 1; }
+    if (mc_avg < st->best_merit) { st->best_merit = mc_avg; bw = 2; }
+    if (bw) {
+        for (int q = 0; q < 3; ++q) st->best_eps[q] = eps[bw - 1][q];
+        st->has_best = 1;
+    }
+    const int use_avg = mc_avg < mc_cur;
+    const double* rep = eps[use_avg];
+    const double mc = use_avg ? mc_avg : mc_cur;
+    for (int q = 0; q < 3; ++q) st->last_eps[q] = rep[q];
+    st->last_obj = obj[use_avg];
+    st->use_avg = use_avg;
+    st->cand_merit = mc;
+    st->restart_now = 0;
+    if (smax(smax(rep[0], rep[1]), rep[2]) <= st->tol) {
+        // converged: the returned point is this candidate
+        st->done = 1;
+        st->best_which = use_avg ? 2 : 1;
+        for (int q = 0; q < 3; ++q) st->best_eps[q] = rep[q];
+        st->has_best = 1;
+        return;
+    }
+    st->best_which = bw;
+    if (st->best_merit < 0.9 * st->stall_ref) { st->stall_ref = st->best_merit; st->stall = 0; }
+    else st->stall += 1;
+    const bool sufficient = mc <= 0.2 * st->merit_last_restart;
+    const bool necessary = mc <= 0.8 * st->merit_last_restart && mc > st->merit_prev_cand;
+    const bool artificial = st->since_restart >= 0.36 * st->accepted;
+    st->merit_prev_cand = mc;
+    if (st->phase == 0 && st->attempts < st->max_iter) {
+        // FP32 -> FP64: the merit reached switch_tol, or the best merit stopped
+        // improving near the FP32 floor (within 100x switch_tol), or it has
+        // stopped improving for four times the stall budget anywhere.
+        int why = 0;
+        if (mc <= st->switch_tol) why = 1;
+        else if (st->stall_checks > 0 && st->stall >= st->stall_checks &&
+                 st->best_merit <= 100.0 * st->switch_tol) why = 2;
+        else if (st->stall_checks > 0 && st->stall >= 4 * st->stall_checks) why = 3;
+        if (why) {
+            st->switch_req = why;   // the host restarts this candidate in FP64
+            return;
+        }
+    }
+    st->restart_now = (sufficient || necessary || artificial) ? 1 : 0;
+    if (st->attempts >= st->max_iter) st->done = 2;
+}
+
+// KKT check, step 5 (columns): copy a new best point; on a restart measure
+// |candidate - last restart point| (scaled), move the restart point, move x
+// to the average if that is the candidate, and clear the sums.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restart_n(DevState* __restrict__ st, int n,
+                                                      IterBufs<T> B, KktBufs K,
+                                                      double* __restrict__ part) {
+    if (!st->need_check) return;
+    const int bw = st->best_which, rs = st->restart_now, ua = st->use_avg, cur = st->cur;
+    double acc[1] = {0.0};
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        if (bw) K.best_x[j] = bw == 1 ? K.xu[j] : K.xu2[j];
+        if (rs) {
+            const double cand = ua ? K.xavg[j] : (double)B.x[cur][j];
+            const double d = cand - (double)B.xlast[j];
+            acc[0] += d * d;
+            B.xlast[j] = (T)cand;
+            if (ua) B.x[cur][j] = (T)cand;
+            B.xsum[j] = T(0);
+        }
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restart_m(DevState* __restrict__ st, int m,
+                                                      IterBufs<T> B, KktBufs K,
+                                                      double* __restrict__ part) {
+    if (!st->need_check) return;
+    const int bw = st->best_which, rs = st->restart_now, ua = st->use_avg, cur = st->cur;
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        if (bw) K.best_y[i] = bw == 1 ? K.yu[i] : K.yu2[i];
+        if (rs) {
+            const double cand = ua ? K.yavg[i] : (double)B.y[cur][i];
+            const double d = cand - (double)B.ylast[i];
+            acc[0] += d * d;
+            B.ylast[i] = (T)cand;
+            if (ua) B.y[cur][i] = (T)cand;
+            B.ysum[i] = T(0);
+        }
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// KKT check, step 6 (one block): primal-weight update and restart counters;
+// resumes the iterations unless finished or switching precision.
+__global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ st,
+                                                        const double* __restrict__ part_n, int nb_n,
+                                                        const double* __restrict__ part_m, int nb_m) {
+    if (!st->need_check) return;
+    double sn[1], sm[1];
+    sum_partials<1>(part_n, nb_n, sn);
+    sum_partials<1>(part_m, nb_m, sm);
+    if (threadIdx.x != 0) return;
+    if (st->restart_now) {
+        const double ddx = sqrt(sn[0]), ddy = sqrt(sm[0]);
+        if (ddx > 1e-10 && ddy > 1e-10)
+            st->omega = exp(0.5 * log(ddy / ddx) + 0.5 * log(st->omega));
+        st->wsum = 0.0;
+        st->since_restart = 0;
+        st->restarts += 1;
+        st->merit_last_restart = st->cand_merit;
+        st->merit_prev_cand = INFINITY;
+        if (st->use_avg) { st->need_ax = 1; st->need_aty = 1; }
+    }
+    st->pending = 0;
+    st->need_check = 0;
+    st->best_which = 0;
+    st->restart_now = 0;
+    if (!st->done && !st->switch_req) st->active = 1;
+}
+
+// Precision switch (mixed mode): the FP64 iterate starts at the FP32 loop's
+// chosen candidate, as a restart (sums cleared, restart point moved).
+__global__ __launch_bounds__(kBlock) void k_switch_n(const DevState* __restrict__ st, int n,
+                                                     IterBufs<float> F, IterBufs<double> D,
+                                                     KktBufs K) {
+    const int ua = st->use_avg, cur = st->cur;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const double cand = ua ? K.xavg[j] : (double)F.x[cur][j];
+        D.x[0][j] = cand;
+        D.xlast[j] = cand;
+        D.xsum[j] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_switch_m(const DevState* __restrict__ st, int m,
+                                                     IterBufs<float> F, IterBufs<double> D,
+                                                     KktBufs K) {
+    const int ua = st->use_avg, cur = st->cur;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const double cand = ua ? K.yavg[i] : (double)F.y[cur][i];
+        D.y[0][i] = cand;
+        D.ylast[i] = cand;
+        D.ysum[i] = 0.0;
+    }
+}
+
+}  // namespace supr

@@ -1,0 +1,263 @@
+"""Python host mirror of the MI355X PDLP engine's C ABI.
+
+Mirrors the reference's PDLP interface (include/pdlp.hpp, include/pdlp_gpu.hpp,
+include/verify.hpp, include/scaling.hpp): the same names, argument meanings
+and result fields (PdlpResult: x, y, iterations, converged, eps_P, eps_D,
+eps_G, restarts), over numpy arrays.  Errors from the library raise
+SupremaError (a RuntimeError, as the reference's std::runtime_error).
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  # noqa: F401
+                   check, lib)
+
+__all__ = [
+    "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "verify", "scale", "build_pdlp_form",
+    "extract_solution", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
+    "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
+]
+
+_i32 = np.int32
+_f64 = np.float64
+
+
+def _dp(a):
+    return a.ctypes.data_as(_lib.c_double_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_lib.c_int32_p)
+
+
+@dataclass
+class LP:
+    """min c'x s.t. rL <= Ax <= rU, l <= x <= u, A in COO order (the
+    reference's RangedLP, include/ranged_lp.hpp:10-25)."""
+    m: int
+    n: int
+    row: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    c: np.ndarray
+    rL: np.ndarray
+    rU: np.ndarray
+    l: np.ndarray
+    u: np.ndarray
+    obj_offset: float = 0.0
+    name: str = "lp"
+
+    def __post_init__(self):
+        self.row = np.ascontiguousarray(self.row, dtype=_i32)
+        self.col = np.ascontiguousarray(self.col, dtype=_i32)
+        for k in ("val", "c", "rL", "rU", "l", "u"):
+            setattr(self, k, np.ascontiguousarray(getattr(self, k), dtype=_f64))
+        if self.row.shape != self.col.shape or self.row.shape != self.val.shape:
+            raise ValueError("COO arrays differ in length")
+        if self.c.shape != (self.n,) or self.l.shape != (self.n,) or self.u.shape != (self.n,):
+            raise ValueError("column arrays must have length n")
+        if self.rL.shape != (self.m,) or self.rU.shape != (self.m,):
+            raise ValueError("row arrays must have length m")
+
+    @property
+    def nnz(self):
+        return int(self.val.shape[0])
+
+    def cstruct(self):
+        return _lib.SuprLp(self.m, self.n, self.nnz, _ip(self.row), _ip(self.col), _dp(self.val),
+                           _dp(self.c), _dp(self.rL), _dp(self.rU), _dp(self.l), _dp(self.u),
+                           float(self.obj_offset))
+
+    def with_values(self, val, c, rL, rU, l, u, name=None):
+        return LP(self.m, self.n, self.row, self.col, val, c, rL, rU, l, u, self.obj_offset,
+                  name or self.name)
+
+
+@dataclass
+class KKTReport:
+    """include/verify.hpp:7-11"""
+    eps_P: float
+    eps_D: float
+    eps_G: float
+    objective: float
+    dual_objective: float
+    max_row_violation: float
+    max_bound_violation: float
+
+
+@dataclass
+class PdlpResult:
+    """include/pdlp.hpp:18-24, plus the engine's run information."""
+    x: np.ndarray
+    y: np.ndarray
+    iterations: int = 0
+    converged: bool = False
+    eps_P: float = 0.0
+    eps_D: float = 0.0
+    eps_G: float = 0.0
+    restarts: int = 0
+    info: dict = field(default_factory=dict)
+
+
+def device_count():
+    return int(lib().supr_device_count())
+
+
+def verify(lp, x, y):
+    """verify(ranged, x, y) -- src/verify.cpp:18-103."""
+    x = np.ascontiguousarray(x, dtype=_f64)
+    y = np.ascontiguousarray(y, dtype=_f64)
+    if x.shape != (lp.n,) or y.shape != (lp.m,):
+        raise ValueError("x must have length n and y length m")
+    out = np.zeros(7)
+    s = lp.cstruct()
+    check(lib().supr_verify(ctypes.byref(s), _dp(x), _dp(y), _dp(out)))
+    return KKTReport(*[float(v) for v in out])
+
+
+def scale(lp, ruiz_passes=10, integer=None):
+    """scale() -- src/scaling.cpp:25-84.  Returns (scaled LP, Dr, Dc)."""
+    outs = [np.zeros(lp.nnz), np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.m), np.zeros(lp.n),
+            np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.n)]
+    integ = None
+    if integer is not None:
+        integ = np.ascontiguousarray(integer, dtype=np.int8).tobytes()
+    s = lp.cstruct()
+    check(lib().supr_scale(ctypes.byref(s), integ, int(ruiz_passes), *[_dp(a) for a in outs]))
+    val, c, rL, rU, l, u, Dr, Dc = outs
+    return lp.with_values(val, c, rL, rU, l, u, lp.name + " (scaled)"), Dr, Dc
+
+
+@dataclass
+class PdlpFormMeta:
+    """include/pdlp.hpp:11-16"""
+    n_orig: int
+    m_orig: int
+    n_slack: int
+    row_perm: np.ndarray
+    row_signs: np.ndarray
+
+
+def build_pdlp_form(lp):
+    """build_pdlp_form() -- src/pdlp.cpp:8-82.  Returns (form LP, meta)."""
+    s = lp.cstruct()
+    dims = np.zeros(4, dtype=_i32)
+    check(lib().supr_pdlp_form_sizes(ctypes.byref(s), _ip(dims)))
+    m2, n2, nnz2, n_slack = (int(v) for v in dims)
+    ri, ci = np.zeros(nnz2, dtype=_i32), np.zeros(nnz2, dtype=_i32)
+    v = np.zeros(nnz2)
+    c, l, u = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+    rL, rU = np.zeros(m2), np.zeros(m2)
+    perm, signs = np.zeros(m2, dtype=_i32), np.zeros(m2)
+    check(lib().supr_pdlp_form(ctypes.byref(s), _ip(ri), _ip(ci), _dp(v), _dp(c), _dp(rL),
+                               _dp(rU), _dp(l), _dp(u), _ip(perm), _dp(signs)))
+    form = LP(m2, n2, ri, ci, v, c, rL, rU, l, u, lp.obj_offset, lp.name + " (pdlp-form)")
+    return form, PdlpFormMeta(lp.n, lp.m, n_slack, perm, signs)
+
+
+def extract_solution(meta, x2, y2):
+    """extract_solution() -- src/pdlp.cpp:84-90."""
+    x2 = np.ascontiguousarray(x2, dtype=_f64)
+    y2 = np.ascontiguousarray(y2, dtype=_f64)
+    x, y = np.zeros(meta.n_orig), np.zeros(meta.m_orig)
+    check(lib().supr_extract_solution(meta.n_orig, meta.m_orig, len(meta.row_perm),
+                                      _ip(meta.row_perm), _dp(meta.row_signs), _dp(x2), _dp(y2),
+                                      _dp(x), _dp(y)))
+    return x, y
+
+
+def _options(mode, max_iterations, check_every, tol, **kw):
+    o = _lib.SuprOptions()
+    lib().supr_pdlp_default_options(ctypes.byref(o))
+    o.mode = int(mode)
+    o.max_iterations = int(max_iterations)
+    o.check_every = int(check_every)
+    o.tol = float(tol)
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, type(getattr(o, k))(v))
+    return o
+
+
+def solve_pdhg_gpu(unscaled_form, scaled_form, Dr, Dc, eta=0.99, max_iterations=200000,
+                   check_every=64, tol=1e-6, stop_flag=None, mode=MODE_MIXED, **options):
+    """solve_pdhg_gpu() -- include/pdlp_gpu.hpp:22-25.
+
+    eta is accepted and ignored, as in the reference (the step size is
+    adaptive).  stop_flag: a callable returning True, or an object with
+    is_set() (threading.Event); polled by the host loop.  Returns PdlpResult
+    with x, y in unscaled PDLP-form coordinates."""
+    del eta
+    Dr = np.ascontiguousarray(Dr, dtype=_f64)
+    Dc = np.ascontiguousarray(Dc, dtype=_f64)
+    if Dr.shape != (scaled_form.m,) or Dc.shape != (scaled_form.n,):
+        raise ValueError("Dr must have length m and Dc length n")
+    o = _options(mode, max_iterations, check_every, tol, **options)
+    x = np.zeros(scaled_form.n)
+    y = np.zeros(scaled_form.m)
+    info = _lib.SuprInfo()
+    if stop_flag is None:
+        cb = _lib.STOP_FN()
+    else:
+        poll = stop_flag.is_set if hasattr(stop_flag, "is_set") else stop_flag
+        cb = _lib.STOP_FN(lambda _ctx: 1 if poll() else 0)
+    su, ss = unscaled_form.cstruct(), scaled_form.cstruct()
+    check(lib().supr_solve_pdhg_gpu(ctypes.byref(su), ctypes.byref(ss), _dp(Dr), _dp(Dc),
+                                    ctypes.byref(o), cb, None, _dp(x), _dp(y),
+                                    ctypes.byref(info)))
+    d = {k: getattr(info, k) for k, _ in _lib.SuprInfo._fields_}
+    res = PdlpResult(x if info.has_point else np.zeros(0), y if info.has_point else np.zeros(0),
+                     int(info.iterations), bool(info.converged), float(info.eps_P),
+                     float(info.eps_D), float(info.eps_G), int(info.restarts), d)
+    return res
+
+
+def bench_kernels(unscaled_form, scaled_form, Dr, Dc, reps=50, **options):
+    """Per-launch time (ms) of the primal-half and dual-half iteration kernels
+    (FP32 and FP64) at this LP's shapes; see supr_bench_kernels."""
+    Dr = np.ascontiguousarray(Dr, dtype=_f64)
+    Dc = np.ascontiguousarray(Dc, dtype=_f64)
+    o = _options(MODE_MIXED, 200000, 64, 1e-6, **options)
+    out = np.zeros(8)
+    su, ss = unscaled_form.cstruct(), scaled_form.cstruct()
+    check(lib().supr_bench_kernels(ctypes.byref(su), ctypes.byref(ss), _dp(Dr), _dp(Dc),
+                                   ctypes.byref(o), int(reps), _dp(out)))
+    keys = ["primal_ms_fp32", "dual_ms_fp32", "primal_ms_fp64", "dual_ms_fp64", "lanes_A",
+            "lanes_AT", "ell_A", "ell_AT"]
+    return dict(zip(keys, (float(v) for v in out)))
+
+
+def gen_transport(S, D, seed=42, cost_max=100):
+    """Balanced transportation LP (tools/gen_lp.cpp family); see suprema_pdlp.h."""
+    n, m = S * D, S + D
+    nnz = 2 * n
+    ri, ci = np.zeros(nnz, dtype=_i32), np.zeros(nnz, dtype=_i32)
+    v = np.zeros(nnz)
+    c, rL, rU, l, u = np.zeros(n), np.zeros(m), np.zeros(m), np.zeros(n), np.zeros(n)
+    check(lib().supr_gen_transport(int(S), int(D), int(seed), int(cost_max), _ip(ri), _ip(ci),
+                                   _dp(v), _dp(c), _dp(rL), _dp(rU), _dp(l), _dp(u)))
+    return LP(m, n, ri, ci, v, c, rL, rU, l, u, 0.0, f"transport_{S}x{D}")
+
+
+def gen_random_lp(m, n, col_nnz=6, seed=42, x0_hi=5.0, u_hi=10.0, mag_decades=0, col_deg=None):
+    """Random feasible, bounded equality-form LP; see suprema_pdlp.h."""
+    deg = None
+    if col_deg is not None:
+        deg = np.ascontiguousarray(col_deg, dtype=_i32)
+        if deg.shape != (n,):
+            raise ValueError("col_deg must have length n")
+        nnz = int(deg.sum())
+    else:
+        nnz = int(n) * int(col_nnz)
+    ri, ci = np.zeros(nnz, dtype=_i32), np.zeros(nnz, dtype=_i32)
+    v = np.zeros(nnz)
+    c, rL, rU, l, u = np.zeros(n), np.zeros(m), np.zeros(m), np.zeros(n), np.zeros(n)
+    check(lib().supr_gen_random_lp(int(m), int(n), int(col_nnz), _ip(deg) if deg is not None else None,
+                                   int(seed), float(x0_hi), float(u_hi), int(mag_decades),
+                                   _ip(ri), _ip(ci), _dp(v), _dp(c), _dp(rL), _dp(rU), _dp(l),
+                                   _dp(u)))
+    return LP(m, n, ri, ci, v, c, rL, rU, l, u, 0.0, f"random_{m}x{n}")
