@@ -23,6 +23,9 @@
  *   supr_gen_random_lp    no reference counterpart: seeded equality-form LP
  *                         generator (6 nonzeros per column, feasible and bounded
  *                         by construction) for synthetic workloads
+ *   supr_spmv             matvec() include/sparse.hpp:59-71 on the device, i.e.
+ *                         GpuBackend::spmv, gpu/pdlp_gpu_solve.cu:211-222, with a
+ *                         chosen schedule (callers: kernel parity tests)
  *   supr_bench_kernels    no reference counterpart: times the engine's two
  *                         iteration kernels at a problem's real shapes (callers:
  *                         benchmarks reporting achieved HBM bandwidth)
@@ -134,6 +137,12 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
                         const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
                         supr_pdlp_info* info);
+
+/* y = A x (transposed = 0; y has m entries) or A^T x (transposed = 1; y has
+ * n entries) on the device, with lanes-per-row `lanes` (0: autotuned) and
+ * FP32 (fp32 = 1: values, x and the accumulation in FP32) or FP64 arithmetic. */
+int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32,
+              const double* x, double* y);
 
 /* Times the two iteration kernels (primal half: A^T y + projection +
  * averages; dual half: A x + dual step + step decision) at this LP's shapes,

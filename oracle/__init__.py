@@ -84,10 +84,13 @@ class Checker:
         assert rc == 0
         return out
 
-    def scale(self, lp, ruiz_passes=10):
+    def scale(self, lp, ruiz_passes=10, integer=None):
         outs = [np.zeros(lp.nnz), np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.m), np.zeros(lp.n),
                 np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.n)]
-        extra = [None] if self.kind == "oracle" else []
+        if integer is not None and self.kind != "oracle":
+            raise ValueError("integer columns: oracle only")
+        integ = None if integer is None else np.ascontiguousarray(integer, dtype=np.int8).tobytes()
+        extra = [integ] if self.kind == "oracle" else []
         rc = self._scale(*self._lpargs(lp), *extra, int(ruiz_passes), *[_d(a) for a in outs])
         assert rc == 0
         return outs   # val_s, c_s, rL_s, rU_s, l_s, u_s, Dr, Dc

@@ -1055,6 +1055,47 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
     }
 }
 
+// y = A x (transposed = 0) or A^T x (transposed = 1) with one schedule.
+int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32,
+              const double* x, double* y) {
+    using namespace supr;
+    if (!check_lp(lp, "supr_spmv")) return SUPR_EINVAL;
+    if (!x || !y) { set_error("supr_spmv: null x or y"); return SUPR_EINVAL; }
+    if (lanes != 0 && !valid_lanes(lanes)) { set_error("supr_spmv: bad lanes"); return SUPR_EINVAL; }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) { set_error("no HIP device visible"); return SUPR_ENODEV; }
+    try {
+        StreamGuard stream;
+        hipStream_t s = stream.s;
+        const Csr H = build_csr(*lp, lp->val, transposed != 0);
+        DevMatrix M;
+        build_dev_matrix(M, H, lp->val, fp32 != 0, fp32 == 0, false, s);
+        const int rows = M.rows, cols = M.cols;
+        std::vector<double> out(rows);
+        if (fp32) {
+            std::vector<float> xf(x, x + cols);
+            DevBuf dx = upload(xf, s), dy = device_vec<float>(rows);
+            if (lanes == 0) autotune_lanes(M, M.f32(), dx.as<float>(), dy.as<float>(), s);
+            else M.lanes = lanes;
+            launch_spmv(M, M.f32(), dx.as<float>(), dy.as<float>(), s);
+            std::vector<float> yf(rows);
+            SUPR_HIP(hipMemcpyAsync(yf.data(), dy.as<void>(), sizeof(float) * rows, hipMemcpyDeviceToHost, s));
+            SUPR_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < rows; ++i) y[i] = yf[i];
+        } else {
+            DevBuf dx = upload(std::vector<double>(x, x + cols), s), dy = device_vec<double>(rows);
+            if (lanes == 0) autotune_lanes(M, M.f64(), dx.as<double>(), dy.as<double>(), s);
+            else M.lanes = lanes;
+            launch_spmv(M, M.f64(), dx.as<double>(), dy.as<double>(), s);
+            SUPR_HIP(hipMemcpyAsync(y, dy.as<void>(), sizeof(double) * rows, hipMemcpyDeviceToHost, s));
+            SUPR_HIP(hipStreamSynchronize(s));
+        }
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
 // Times the iteration kernels (see Solver::bench_kernels).  out: 8 doubles.
 int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                        const double* Dr, const double* Dc, const supr_pdlp_options* opt,

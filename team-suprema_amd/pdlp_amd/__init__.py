@@ -16,7 +16,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
                    check, lib)
 
 __all__ = [
-    "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "verify", "scale", "build_pdlp_form",
+    "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
     "extract_solution", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
@@ -104,6 +104,19 @@ class PdlpResult:
 
 def device_count():
     return int(lib().supr_device_count())
+
+
+def spmv(lp, x, transposed=False, lanes=0, fp32=False):
+    """A x (or A^T x) on the device with one SpMV schedule (GpuBackend::spmv,
+    gpu/pdlp_gpu_solve.cu:211-222); lanes 0 = autotuned."""
+    x = np.ascontiguousarray(x, dtype=_f64)
+    if x.shape != ((lp.m,) if transposed else (lp.n,)):
+        raise ValueError("x has the wrong length")
+    y = np.zeros(lp.n if transposed else lp.m)
+    s = lp.cstruct()
+    check(lib().supr_spmv(ctypes.byref(s), int(bool(transposed)), int(lanes), int(bool(fp32)),
+                          _dp(x), _dp(y)))
+    return y
 
 
 def verify(lp, x, y):

@@ -70,6 +70,8 @@ SIGNATURES = [
       c_double_p, c_double_p, ctypes.POINTER(SuprInfo)]),
     ("supr_bench_kernels", ctypes.c_int,
      [_LP, _LP, c_double_p, c_double_p, ctypes.POINTER(SuprOptions), ctypes.c_int32, c_double_p]),
+    ("supr_spmv", ctypes.c_int,
+     [_LP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_double_p, c_double_p]),
     ("supr_verify", ctypes.c_int, [_LP, c_double_p, c_double_p, c_double_p]),
     ("supr_scale", ctypes.c_int,
      [_LP, ctypes.c_char_p, ctypes.c_int32] + [c_double_p] * 8),

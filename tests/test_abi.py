@@ -1,0 +1,66 @@
+"""The C ABI library loads and exports every entry point include/*.h declares
+(no device compute here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import pdlp_amd as P
+from pdlp_amd import _lib
+from conftest import ROOT, gpu_available
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(supr_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert "supr_solve_pdhg_gpu" in names and len(names) >= 12
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert names == bound, "the Python binding and the header disagree"
+
+
+def test_version_and_device_count():
+    assert P.lib().supr_version().decode().startswith("suprema-pdlp")
+    assert P.device_count() >= 0
+
+
+def test_default_options_are_the_reference_defaults():
+    o = _lib.SuprOptions()
+    P.lib().supr_pdlp_default_options(ctypes.byref(o))
+    assert (o.max_iterations, o.check_every, o.tol, o.mode) == (200000, 64, 1e-6, P.MODE_MIXED)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_solve_without_device_fails_loudly():
+    lp = P.gen_random_lp(20, 40, 6, seed=1)
+    s, Dr, Dc = P.scale(lp)
+    with pytest.raises(P.SupremaError) as e:
+        P.solve_pdhg_gpu(lp, s, Dr, Dc)
+    assert e.value.code == _lib.SUPR_ENODEV
+
+
+def test_solve_rejects_mismatched_forms():
+    lp = P.gen_random_lp(20, 40, 6, seed=1)
+    s, Dr, Dc = P.scale(lp)
+    other = P.gen_random_lp(20, 40, 6, seed=2)
+    with pytest.raises(P.SupremaError) as e:
+        P.solve_pdhg_gpu(other, s, Dr, Dc)
+    assert e.value.code == _lib.SUPR_EINVAL
+    with pytest.raises(P.SupremaError):
+        P.solve_pdhg_gpu(lp, s, Dr, Dc, check_every=0)
+    with pytest.raises(P.SupremaError):
+        P.solve_pdhg_gpu(lp, s, Dr, Dc, spmv_lanes_A=3)

@@ -1,0 +1,184 @@
+"""GPU parity: the HIP engine through the C ABI against the FP64 oracle.
+
+Tolerances (floating point; the reference path is FP64 throughout):
+  * SpMV, FP64: per row |gpu - oracle| <= (nnz_row + 2) * 2^-52 * (|A||x|)_row;
+    FP32 (values, x and accumulation in FP32): (nnz_row + 2) * 2^-23 * (|A||x|)_row.
+  * Solves: converged, the ORACLE's verifier on the returned point gives
+    max(eps_P, eps_D, eps_G) <= tol (1e-6), and the objective agrees with the
+    reference engine's to |d| / (1 + |obj|) <= 1e-5 (audit_lp warns above 1e-6 and
+    fails above 1e-4, tools/audit_lp.cpp:261-268; two points that both meet a
+    1e-6 KKT tolerance can differ by a few times that).
+"""
+import numpy as np
+import pytest
+
+import pdlp_amd as P
+from conftest import golden_lp, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+OBJ_RTOL = 1e-5
+
+
+def _spmv_cases():
+    rng = np.random.default_rng(5)
+    cases = {
+        "random_c0_small": P.gen_random_lp(200, 500, 6, seed=1),
+        "transport_30x40": P.gen_transport(30, 40, seed=7),
+        "powerlaw_mag3": P.gen_random_lp(300, 700, 0, seed=2, mag_decades=3,
+                                         col_deg=np.minimum(2 + rng.zipf(1.5, 700), 200).clip(2, 250)),
+    }
+    # rows of very different lengths plus empty rows and columns
+    m, n = 90, 1500
+    rows, cols = [], []
+    for i in range(0, m, 3):            # every third row empty
+        k = (i * 37) % 700 + 1
+        cs = rng.choice(n - 10, size=k, replace=False)
+        rows += [i] * k
+        cols += cs.tolist()
+    rows, cols = np.array(rows), np.array(cols)
+    cases["ragged_empty"] = P.LP(m, n, rows, cols, rng.standard_normal(len(rows)), np.zeros(n),
+                                 np.zeros(m), np.zeros(m), np.zeros(n), np.zeros(n))
+    return cases
+
+
+SPMV = _spmv_cases()
+
+
+@pytest.mark.parametrize("name", sorted(SPMV))
+@pytest.mark.parametrize("transposed", [False, True])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_spmv_schedules_match_oracle(oracle, name, transposed, fp32):
+    lp = SPMV[name]
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(lp.m if transposed else lp.n)
+    absA = lp.with_values(np.abs(lp.val), lp.c, lp.rL, lp.rU, lp.l, lp.u)
+    xe = x.astype(np.float32).astype(np.float64) if fp32 else x
+    want = oracle.spmv(lp, xe, transposed)
+    mag = oracle.spmv(absA, np.abs(xe), transposed)
+    rows = lp.col if transposed else lp.row
+    cnt = np.bincount(rows, minlength=lp.n if transposed else lp.m)
+    unit = 2.0 ** -23 if fp32 else 2.0 ** -52
+    bound = (cnt + 2) * unit * mag + 1e-300
+    for lanes in (1, 2, 4, 8, 16, 32, 64, 256, 0):
+        got = P.spmv(lp, x, transposed=transposed, lanes=lanes, fp32=fp32)
+        err = np.abs(got - want)
+        assert np.all(err <= bound), f"lanes={lanes}: worst row {np.argmax(err / bound)}"
+        assert np.all(got[cnt == 0] == 0)
+
+
+def _solve(case, mode, **kw):
+    form = golden_lp(case)
+    scaled, Dr, Dc = P.scale(form)
+    res = P.solve_pdhg_gpu(form, scaled, Dr, Dc, tol=TOL, max_iterations=500000, mode=mode, **kw)
+    return form, res
+
+
+def _check_against_reference(oracle, case, form, res):
+    assert res.converged, f"{case['name']}: not converged after {res.iterations}"
+    rep = oracle.verify(form, res.x, res.y)
+    assert max(rep[0], rep[1], rep[2]) <= TOL, f"{case['name']}: oracle verify {rep[:3]}"
+    # the engine's own eps are the oracle verifier's within FP64 reduction order
+    np.testing.assert_allclose([res.eps_P, res.eps_D, res.eps_G], rep[:3], rtol=1e-6, atol=1e-12)
+    ref_obj = case["ref"]["objective_form"]
+    assert abs(rep[3] - ref_obj) / (1 + abs(ref_obj)) <= OBJ_RTOL, (rep[3], ref_obj)
+    return rep
+
+
+FAST_CASES = ["afiro", "adlittle", "blend", "sc50a", "sc50b", "sc105", "kb2", "scagr7", "israel",
+              "share1b", "beaconfd", "tiny_range", "med", "illcond", "random_200x500_s1",
+              "random_2000x5000_s42", "transport_60x60_s42", "transport_30x40_s7"]
+PLAIN_CASES = ["afiro", "sc50a", "blend", "tiny_range", "transport_30x40_s7", "random_200x500_s1",
+               "med"]
+
+
+@pytest.mark.parametrize("name", FAST_CASES)
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_device_resident_solve_matches_reference(oracle, golden, name, mode):
+    case = golden[name]
+    form, res = _solve(case, mode)
+    _check_against_reference(oracle, case, form, res)
+
+
+@pytest.mark.parametrize("name", PLAIN_CASES)
+def test_plain_fp64_solve_matches_reference(oracle, golden, name):
+    case = golden[name]
+    form, res = _solve(case, P.MODE_PLAIN_FP64)
+    _check_against_reference(oracle, case, form, res)
+    # same algorithm, same arithmetic width: the trajectory length stays close
+    it_ref = case["ref"]["iterations"]
+    assert 0.5 * it_ref <= res.iterations <= 1.5 * it_ref
+
+
+@pytest.mark.parametrize("name", ["afiro", "blend", "israel", "share1b", "beaconfd"])
+def test_netlib_original_objective_matches_published(oracle, golden, name):
+    """The audit path: solve on the GPU, extract_solution, verify on the
+    ORIGINAL LP; published optimum within 1e-4 (tools/audit_lp.cpp:270-284)."""
+    lp, maximize = load_fixture(name)
+    form, meta = P.build_pdlp_form(lp)
+    scaled, Dr, Dc = P.scale(form)
+    res = P.solve_pdhg_gpu(form, scaled, Dr, Dc, tol=TOL, max_iterations=500000)
+    assert res.converged
+    x, y = P.extract_solution(meta, res.x, res.y)
+    rep = oracle.verify(lp, x, y)
+    obj = -rep[3] if maximize else rep[3]
+    pub = golden[name]["published_optimum"]
+    assert abs(obj - pub) / (1 + abs(pub)) <= 1e-4
+    assert max(rep[:3]) <= 1e-5
+
+
+@pytest.fixture(scope="module")
+def transport_1m():
+    lp = P.gen_transport(1000, 1000, seed=42)
+    scaled, Dr, Dc = P.scale(lp)
+    return lp, scaled, Dr, Dc
+
+
+def test_transport_1m_verified_and_modes_agree(oracle, transport_1m):
+    """1,000,000-variable transportation LP: every device-resident mode reaches
+    1e-6 as judged by the oracle's verifier; the objectives agree; a second
+    run is bit-identical (deterministic reductions)."""
+    lp, scaled, Dr, Dc = transport_1m
+    objs = []
+    for mode in (P.MODE_MIXED, P.MODE_FAST_FP64):
+        res = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=mode)
+        assert res.converged
+        rep = oracle.verify(lp, res.x, res.y)
+        assert max(rep[:3]) <= TOL
+        objs.append(rep[3])
+        if mode == P.MODE_MIXED:
+            again = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=mode)
+            assert again.iterations == res.iterations
+            assert np.array_equal(again.x, res.x) and np.array_equal(again.y, res.y)
+    assert abs(objs[0] - objs[1]) / (1 + abs(objs[1])) <= OBJ_RTOL
+    # supplies and demands are integers: the optimal cost is an integer too
+    assert abs(objs[1] - round(objs[1])) / (1 + abs(objs[1])) <= OBJ_RTOL
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64, P.MODE_PLAIN_FP64])
+def test_iteration_limit_returns_best_point(oracle, mode):
+    lp = P.gen_random_lp(200, 500, 6, seed=1)
+    scaled, Dr, Dc = P.scale(lp)
+    res = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, max_iterations=300, mode=mode)
+    assert not res.converged
+    assert res.iterations == 300
+    assert res.info["has_point"] == 1
+    rep = oracle.verify(lp, res.x, res.y)
+    np.testing.assert_allclose([res.eps_P, res.eps_D, res.eps_G], rep[:3], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_PLAIN_FP64])
+def test_stop_flag_cancels(mode):
+    lp = P.gen_random_lp(2000, 5000, 6, seed=42)
+    scaled, Dr, Dc = P.scale(lp)
+    polls = {"n": 0}
+
+    def stop():
+        polls["n"] += 1
+        return polls["n"] > 20
+
+    res = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=1e-9, stop_flag=stop, mode=mode)
+    assert not res.converged
+    assert res.iterations < 200000
+    assert polls["n"] > 20
