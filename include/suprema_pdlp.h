@@ -23,6 +23,8 @@
  *   supr_gen_random_lp    no reference counterpart: seeded equality-form LP
  *                         generator (6 nonzeros per column, feasible and bounded
  *                         by construction) for synthetic workloads
+ *   supr_engine_*         no reference counterpart: a resident engine for repeated
+ *                         solves of one LP (callers: benchmarks, request loops)
  *   supr_spmv             matvec() include/sparse.hpp:59-71 on the device, i.e.
  *                         GpuBackend::spmv, gpu/pdlp_gpu_solve.cu:211-222, with a
  *                         chosen schedule (callers: kernel parity tests)
@@ -137,6 +139,20 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
                         const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
                         supr_pdlp_info* info);
+
+/* A resident engine: the LP is uploaded, laid out and its SpMV schedules
+ * tuned once (supr_engine_create); every supr_engine_solve then runs one
+ * solve from the cold start with the data already in HBM and returns what
+ * supr_solve_pdhg_gpu returns.  One engine is used by one thread at a time;
+ * supr_engine_destroy frees its device memory.  (No reference counterpart:
+ * the reference rebuilds GpuBackend per call; callers: benchmarks and
+ * request loops that solve an LP repeatedly.) */
+typedef struct supr_engine supr_engine;
+int supr_engine_create(const supr_lp* unscaled_form, const supr_lp* scaled_form, const double* Dr,
+                       const double* Dc, const supr_pdlp_options* opt, supr_engine** engine);
+int supr_engine_solve(supr_engine* engine, supr_stop_fn stop, void* stop_ctx, double* x_out,
+                      double* y_out, supr_pdlp_info* info);
+void supr_engine_destroy(supr_engine* engine);
 
 /* y = A x (transposed = 0; y has m entries) or A^T x (transposed = 1; y has
  * n entries) on the device, with lanes-per-row `lanes` (0: autotuned) and

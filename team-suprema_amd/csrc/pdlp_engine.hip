@@ -251,17 +251,42 @@ double finite_inf_norm(const double* v, int len) {
 
 double merit_of(const double* e) { return std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]); }
 
+// Deep copy of a caller's LP, so an engine outlives the caller's arrays.
+struct OwnedLp {
+    std::vector<int32_t> row, col;
+    std::vector<double> val, c, rL, rU, l, u;
+    supr_lp v;
+    explicit OwnedLp(const supr_lp& s)
+        : row(s.row_idx, s.row_idx + s.nnz), col(s.col_idx, s.col_idx + s.nnz),
+          val(s.val, s.val + s.nnz), c(s.c, s.c + s.n), rL(s.rL, s.rL + s.m),
+          rU(s.rU, s.rU + s.m), l(s.l, s.l + s.n), u(s.u, s.u + s.n), v(s) {
+        v.row_idx = row.data(); v.col_idx = col.data(); v.val = val.data();
+        v.c = c.data(); v.rL = rL.data(); v.rU = rU.data(); v.l = l.data(); v.u = u.data();
+    }
+    OwnedLp(const OwnedLp&) = delete;
+    OwnedLp& operator=(const OwnedLp&) = delete;
+};
+
 class Solver {
    public:
     Solver(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc,
            const supr_pdlp_options& o)
-        : U_(U), S_(S), Dr_(Dr), Dc_(Dc), o_(o), m_(S.m), n_(S.n) {}
+        : Uown_(U), Sown_(S), Drv_(Dr, Dr + S.m), Dcv_(Dc, Dc + S.n), U_(Uown_.v), S_(Sown_.v),
+          Dr_(Drv_.data()), Dc_(Dcv_.data()), o_(o), m_(S.m), n_(S.n) {}
+    ~Solver() {
+        for (auto& g : graph_) if (g) (void)hipGraphExecDestroy(g);
+    }
 
-    void solve(supr_stop_fn stop, void* ctx, double* x_out, double* y_out, supr_pdlp_info* info);
+    // Uploads the LP, lays out the matrices, tunes the SpMV schedules.
+    void setup();
+    // One solve from the cold start x = clip(0, l, u), y = 0 (pdlp_algo.hpp:67-69).
+    void run(supr_stop_fn stop, void* ctx, double* x_out, double* y_out, supr_pdlp_info* info);
     void bench_kernels(int reps, double* out);
+    double setup_seconds() const { return setup_s_; }
 
    private:
-    // shared
+    OwnedLp Uown_, Sown_;
+    std::vector<double> Drv_, Dcv_;
     const supr_lp& U_;
     const supr_lp& S_;
     const double* Dr_;
@@ -270,14 +295,21 @@ class Solver {
     const int m_, n_;
     StreamGuard stream_;
     DevMatrix A_, AT_;
+    int lanesA_[2] = {1, 1}, lanesAT_[2] = {1, 1};   // [0] FP32, [1] FP64 schedules
     double eta0_ = 1.0, omega0_ = 1.0;
     std::vector<double> x0_;
+    double setup_s_ = 0.0;
+    bool ready_ = false;
 
     void initial_point();
     void setup_matrices(bool want32, bool want64, bool wantu);
+    void use_lanes(int prec) { A_.lanes = lanesA_[prec]; AT_.lanes = lanesAT_[prec]; }
 
     // plain mode
-    void solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_out, supr_pdlp_info* info);
+    DevBuf pc_, pl_, pu_, pb_, pineq_, px_, pxn_, paty_, pxsum_, py_, pyn_, pax_, paxn_, pysum_, pstats_;
+    std::unique_ptr<Csr> Au_, AuT_;
+    void setup_plain();
+    void run_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_out, supr_pdlp_info* info);
 
     // device-resident modes
     template <typename T> struct Iter {
@@ -305,22 +337,26 @@ class Solver {
     Iter<float> F_;
     Iter<double> D_;
     DevBuf ineq_, state_, parts_;
-    DevBuf kc_, kl_, ku_, krL_, krU_, kDr_, kDc_, xu_, xu2_, xavg_, yu_, yu2_, yavg_, bestx_, besty_;
+    DevBuf kc_, kl_, ku_, krL_, krU_, kDr_, kDc_, xu_, xu2_, xavg_, yu_, yu2_, yavg_, bestx_, besty_,
+        atyavg_, axavg_;
     KktBufs K_{};
     double *part_n_ = nullptr, *part_m_ = nullptr, *part_kn_ = nullptr, *part_km_ = nullptr,
-           *part_rn_ = nullptr, *part_rm_ = nullptr;
+           *part_rn_ = nullptr, *part_rm_ = nullptr, *dx2_slot_ = nullptr;
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};   // per iterate precision
 
-    template <typename T> void alloc_iter(Iter<T>& I, bool init_point);
+    template <typename T> void alloc_iter(Iter<T>& I);
+    template <typename T> void reset_iter(Iter<T>& I);
     void setup_device_resident(bool mixed);
     DevState initial_state(bool mixed) const;
     template <typename T> void enqueue_attempts(const Iter<T>& I, int count);
     template <typename T> void enqueue_check(const Iter<T>& I);
-    template <typename T> void enqueue_ax_fix(const Iter<T>& I);
+    template <typename T> void set_products(const Iter<T>& I, const MatView<T>& A, const MatView<T>& AT);
     template <typename T> hipGraphExec_t capture_graph(const Iter<T>& I, int attempts);
     template <typename T>
-    int run_graph_loop(const Iter<T>& I, supr_stop_fn stop, void* ctx, DevState& hs, bool& stopped);
-    void solve_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
-                               double* y_out, supr_pdlp_info* info);
+    int run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx, DevState& hs,
+                       bool& stopped);
+    void run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
+                             double* y_out, supr_pdlp_info* info);
 };
 
 void Solver::initial_point() {
@@ -347,48 +383,77 @@ void Solver::setup_matrices(bool want32, bool want64, bool wantu) {
     build_dev_matrix(AT_, AT, U_.val, want32, want64, wantu, stream_.s);
 }
 
+void Solver::setup() {
+    const auto t0 = Clock::now();
+    switch (o_.mode) {
+        case SUPR_MODE_PLAIN_FP64: setup_plain(); break;
+        case SUPR_MODE_FAST_FP64: setup_device_resident(false); break;
+        case SUPR_MODE_MIXED: setup_device_resident(true); break;
+        default: throw BadInput("unknown mode " + std::to_string(o_.mode));
+    }
+    ready_ = true;
+    setup_s_ = seconds_since(t0);
+}
+
+void Solver::run(supr_stop_fn stop, void* ctx, double* x_out, double* y_out,
+                 supr_pdlp_info* info) {
+    if (!ready_) throw BadInput("engine is not set up");
+    const auto t0 = Clock::now();
+    std::memset(info, 0, sizeof(*info));
+    if (o_.mode == SUPR_MODE_PLAIN_FP64) run_plain(stop, ctx, x_out, y_out, info);
+    else run_device_resident(o_.mode == SUPR_MODE_MIXED, stop, ctx, x_out, y_out, info);
+    info->seconds_setup = setup_s_;
+    info->seconds_total = seconds_since(t0) + setup_s_;
+}
+
 // ---------------------------------------------------------- plain FP64 --
 
-void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_out,
-                         supr_pdlp_info* info) {
-    const auto t0 = Clock::now();
+void Solver::setup_plain() {
     hipStream_t s = stream_.s;
     const int n = n_, m = m_;
     setup_matrices(false, true, false);
     initial_point();
     // host CSR of the unscaled form for the host verifier (pdlp_algo.hpp:57)
-    const Csr Au = build_csr(U_, U_.val, false), AuT = build_csr(U_, U_.val, true);
-
+    Au_ = std::make_unique<Csr>(build_csr(U_, U_.val, false));
+    AuT_ = std::make_unique<Csr>(build_csr(U_, U_.val, true));
     auto vec = [&](const double* p, int len) { return upload(std::vector<double>(p, p + len), s); };
-    DevBuf c = vec(S_.c, n), l = vec(S_.l, n), u = vec(S_.u, n), b = vec(S_.rL, m);
+    pc_ = vec(S_.c, n); pl_ = vec(S_.l, n); pu_ = vec(S_.u, n); pb_ = vec(S_.rL, m);
     std::vector<uint8_t> hineq(m);
     for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
-    DevBuf ineq = upload(hineq, s);
-    DevBuf x = device_vec<double>(n), xn = device_vec<double>(n), aty = device_vec<double>(n),
-           xsum = device_vec<double>(n), y = device_vec<double>(m), yn = device_vec<double>(m),
-           ax = device_vec<double>(m), axn = device_vec<double>(m), ysum = device_vec<double>(m),
-           stats = device_vec<double>(3 * kMaxPlain);
-    double* px = x.as<double>(); double* pxn = xn.as<double>();
-    double* py = y.as<double>(); double* pyn = yn.as<double>();
-    double* pax = ax.as<double>(); double* paxn = axn.as<double>();
-
-    // SpMV schedules
+    pineq_ = upload(hineq, s);
+    px_ = device_vec<double>(n); pxn_ = device_vec<double>(n); paty_ = device_vec<double>(n);
+    pxsum_ = device_vec<double>(n); py_ = device_vec<double>(m); pyn_ = device_vec<double>(m);
+    pax_ = device_vec<double>(m); paxn_ = device_vec<double>(m); pysum_ = device_vec<double>(m);
+    pstats_ = device_vec<double>(3 * kMaxPlain);
     if (o_.spmv_lanes_A > 0) A_.lanes = o_.spmv_lanes_A;
-    else autotune_lanes(A_, A_.f64(), px, pax, s);
+    else autotune_lanes(A_, A_.f64(), px_.as<double>(), pax_.as<double>(), s);
     if (o_.spmv_lanes_AT > 0) AT_.lanes = o_.spmv_lanes_AT;
-    else autotune_lanes(AT_, AT_.f64(), py, aty.as<double>(), s);
-    if (A_.ell_w > 0) A_.lanes = 1;
-    if (AT_.ell_w > 0) AT_.lanes = 1;
+    else autotune_lanes(AT_, AT_.f64(), py_.as<double>(), paty_.as<double>(), s);
+    if (A_.ell_w > 0 && o_.spmv_lanes_A <= 0) A_.lanes = 1;
+    if (AT_.ell_w > 0 && o_.spmv_lanes_AT <= 0) AT_.lanes = 1;
+    lanesA_[1] = A_.lanes; lanesAT_[1] = AT_.lanes;
+}
 
+// The reference's GpuBackend loop (include/pdlp_algo.hpp:103-175 with
+// gpu/pdlp_gpu_solve.cu): step statistics read back after every attempt,
+// KKT check on the host.
+void Solver::run_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_out,
+                       supr_pdlp_info* info) {
+    hipStream_t s = stream_.s;
+    const int n = n_, m = m_;
+    use_lanes(1);
+    double* px = px_.as<double>(); double* pxn = pxn_.as<double>();
+    double* py = py_.as<double>(); double* pyn = pyn_.as<double>();
+    double* pax = pax_.as<double>(); double* paxn = paxn_.as<double>();
+    double* aty = paty_.as<double>(); double* xsum = pxsum_.as<double>(); double* ysum = pysum_.as<double>();
     std::vector<double> hx = x0_, hy(m, 0.0);
     SUPR_HIP(hipMemcpyAsync(px, hx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     SUPR_HIP(hipMemsetAsync(py, 0, sizeof(double) * m, s));
-    SUPR_HIP(hipMemsetAsync(xsum.as<void>(), 0, sizeof(double) * n, s));
-    SUPR_HIP(hipMemsetAsync(ysum.as<void>(), 0, sizeof(double) * m, s));
+    SUPR_HIP(hipMemsetAsync(xsum, 0, sizeof(double) * n, s));
+    SUPR_HIP(hipMemsetAsync(ysum, 0, sizeof(double) * m, s));
     launch_spmv(A_, A_.f64(), px, pax, s);
-    launch_spmv(AT_, AT_.f64(), py, aty.as<double>(), s);
+    launch_spmv(AT_, AT_.f64(), py, aty, s);
     SUPR_HIP(hipStreamSynchronize(s));
-    const double t_setup = seconds_since(t0);
     const auto t1 = Clock::now();
 
     double eta = eta0_, omega = omega0_, wsum = 0.0;
@@ -404,12 +469,15 @@ void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_
     bool converged = false;
     std::vector<double> res_x, res_y;
     double res_eps[3] = {0, 0, 0};
+    const double* c = pc_.as<double>(); const double* l = pl_.as<double>();
+    const double* u = pu_.as<double>(); const double* b = pb_.as<double>();
+    const uint8_t* ineq = pineq_.as<uint8_t>();
 
     auto unscaled_kkt = [&](const std::vector<double>& xs, const std::vector<double>& ys,
                             std::vector<double>& xuo, std::vector<double>& yuo, double* rep) {
         for (int j = 0; j < n; ++j) xuo[j] = Dc_[j] * xs[j];
         for (int i = 0; i < m; ++i) yuo[i] = Dr_[i] * ys[i];
-        verify_csr(U_, Au, AuT, xuo.data(), yuo.data(), rep);
+        verify_csr(U_, *Au_, *AuT_, xuo.data(), yuo.data(), rep);
     };
     auto keep_best = [&](const std::vector<double>& xv, const std::vector<double>& yv, const double* r) {
         const double mm = merit_of(r);
@@ -424,16 +492,13 @@ void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_
         while (attempts < o_.max_iterations) {
             ++attempts;
             const double tau = eta / omega, sigma = eta * omega;
-            hipLaunchKernelGGL(k_plain_primal, dim3(std::max(1, gb_n)), dim3(kBlock), 0, s, n, px,
-                               c.as<double>(), aty.as<double>(), l.as<double>(), u.as<double>(), tau, pxn);
+            hipLaunchKernelGGL(k_plain_primal, dim3(std::max(1, gb_n)), dim3(kBlock), 0, s, n, px, c, aty, l, u, tau, pxn);
             launch_spmv(A_, A_.f64(), pxn, paxn, s);
-            hipLaunchKernelGGL(k_plain_dual, dim3(std::max(1, gb_m)), dim3(kBlock), 0, s, m, py,
-                               b.as<double>(), paxn, pax, ineq.as<uint8_t>(), sigma, pyn);
-            hipLaunchKernelGGL(k_plain_stats_n, dim3(rb_n), dim3(kBlock), 0, s, n, px, pxn, stats.as<double>());
-            hipLaunchKernelGGL(k_plain_stats_m, dim3(rb_m), dim3(kBlock), 0, s, m, py, pyn, pax, paxn,
-                               stats.as<double>());
+            hipLaunchKernelGGL(k_plain_dual, dim3(std::max(1, gb_m)), dim3(kBlock), 0, s, m, py, b, paxn, pax, ineq, sigma, pyn);
+            hipLaunchKernelGGL(k_plain_stats_n, dim3(rb_n), dim3(kBlock), 0, s, n, px, pxn, pstats_.as<double>());
+            hipLaunchKernelGGL(k_plain_stats_m, dim3(rb_m), dim3(kBlock), 0, s, m, py, pyn, pax, paxn, pstats_.as<double>());
             SUPR_HIP(hipGetLastError());
-            SUPR_HIP(hipMemcpyAsync(stats_h, stats.as<double>(), sizeof(stats_h), hipMemcpyDeviceToHost, s));
+            SUPR_HIP(hipMemcpyAsync(stats_h, pstats_.as<double>(), sizeof(stats_h), hipMemcpyDeviceToHost, s));
             SUPR_HIP(hipStreamSynchronize(s));
             double dx2 = 0.0, dy2 = 0.0, inter = 0.0;
             for (int k = 0; k < rb_n; ++k) dx2 += stats_h[k];
@@ -445,9 +510,9 @@ void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_
             const double eta_next = std::isfinite(eta_bar) ? smin((1.0 - std::pow(k1, -0.3)) * eta_bar, grow) : grow;
             if (eta <= eta_bar) {
                 std::swap(px, pxn); std::swap(py, pyn); std::swap(pax, paxn);
-                launch_spmv(AT_, AT_.f64(), py, aty.as<double>(), s);
-                hipLaunchKernelGGL(k_plain_axpy, dim3(std::max(1, gb_n)), dim3(kBlock), 0, s, n, eta, px, xsum.as<double>());
-                hipLaunchKernelGGL(k_plain_axpy, dim3(std::max(1, gb_m)), dim3(kBlock), 0, s, m, eta, py, ysum.as<double>());
+                launch_spmv(AT_, AT_.f64(), py, aty, s);
+                hipLaunchKernelGGL(k_plain_axpy, dim3(std::max(1, gb_n)), dim3(kBlock), 0, s, n, eta, px, xsum);
+                hipLaunchKernelGGL(k_plain_axpy, dim3(std::max(1, gb_m)), dim3(kBlock), 0, s, m, eta, py, ysum);
                 wsum += eta;
                 eta = eta_next;
                 ++accepted; ++since_restart;
@@ -460,8 +525,8 @@ void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_
         ++checks;
         SUPR_HIP(hipMemcpyAsync(hx.data(), px, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         SUPR_HIP(hipMemcpyAsync(hy.data(), py, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-        SUPR_HIP(hipMemcpyAsync(hxs.data(), xsum.as<double>(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        SUPR_HIP(hipMemcpyAsync(hys.data(), ysum.as<double>(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(hxs.data(), xsum, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(hys.data(), ysum, sizeof(double) * m, hipMemcpyDeviceToHost, s));
         SUPR_HIP(hipStreamSynchronize(s));
         if (wsum > 0) { for (auto& v : hxs) v /= wsum; for (auto& v : hys) v /= wsum; }
         else { hxs = hx; hys = hy; }
@@ -495,10 +560,11 @@ void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_
                 SUPR_HIP(hipMemcpyAsync(px, cx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
                 SUPR_HIP(hipMemcpyAsync(py, cy.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
                 launch_spmv(A_, A_.f64(), px, pax, s);
-                launch_spmv(AT_, AT_.f64(), py, aty.as<double>(), s);
+                launch_spmv(AT_, AT_.f64(), py, aty, s);
             }
-            SUPR_HIP(hipMemsetAsync(xsum.as<void>(), 0, sizeof(double) * n, s));
-            SUPR_HIP(hipMemsetAsync(ysum.as<void>(), 0, sizeof(double) * m, s));
+            SUPR_HIP(hipMemsetAsync(xsum, 0, sizeof(double) * n, s));
+            SUPR_HIP(hipMemsetAsync(ysum, 0, sizeof(double) * m, s));
+            SUPR_HIP(hipStreamSynchronize(s));   // cx / cy are host vectors reused below
             wsum = 0.0;
             since_restart = 0; ++restarts;
             merit_last_restart = mc;
@@ -533,20 +599,17 @@ void Solver::solve_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_
     info->iterations_fp32 = 0;
     info->checks = checks;
     info->lanes_A = A_.lanes; info->lanes_AT = AT_.lanes;
-    info->seconds_setup = t_setup;
     info->seconds_solve = seconds_since(t1);
 }
 
 // ---------------------------------------------------- device-resident ---
 
 template <typename T>
-void Solver::alloc_iter(Iter<T>& I, bool init_point) {
+void Solver::alloc_iter(Iter<T>& I) {
     hipStream_t s = stream_.s;
     const int n = n_, m = m_;
-    std::vector<T> c(n), l(n), u(n), b(m), x0(n);
-    for (int j = 0; j < n; ++j) {
-        c[j] = (T)S_.c[j]; l[j] = (T)S_.l[j]; u[j] = (T)S_.u[j]; x0[j] = (T)x0_[j];
-    }
+    std::vector<T> c(n), l(n), u(n), b(m);
+    for (int j = 0; j < n; ++j) { c[j] = (T)S_.c[j]; l[j] = (T)S_.l[j]; u[j] = (T)S_.u[j]; }
     for (int i = 0; i < m; ++i) b[i] = (T)S_.rL[i];
     I.c = upload(c, s); I.l = upload(l, s); I.u = upload(u, s); I.b = upload(b, s);
     for (int k = 0; k < 2; ++k) {
@@ -559,15 +622,21 @@ void Solver::alloc_iter(Iter<T>& I, bool init_point) {
     I.ysum = device_vec<T>(m);
     I.xlast = device_vec<T>(n);
     I.ylast = device_vec<T>(m);
-    if (init_point) {
-        SUPR_HIP(hipMemcpyAsync(I.x[0].template as<T>(), x0.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
-        SUPR_HIP(hipMemcpyAsync(I.xlast.template as<T>(), x0.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
-        SUPR_HIP(hipMemsetAsync(I.y[0].template as<void>(), 0, sizeof(T) * m, s));
-        SUPR_HIP(hipMemsetAsync(I.ylast.template as<void>(), 0, sizeof(T) * m, s));
-        SUPR_HIP(hipMemsetAsync(I.xsum.template as<void>(), 0, sizeof(T) * n, s));
-        SUPR_HIP(hipMemsetAsync(I.ysum.template as<void>(), 0, sizeof(T) * m, s));
-    }
-    SUPR_HIP(hipStreamSynchronize(s));   // host staging vectors die here
+}
+
+// Cold start: x = x0, y = 0, sums cleared, restart point = start point.
+template <typename T>
+void Solver::reset_iter(Iter<T>& I) {
+    hipStream_t s = stream_.s;
+    std::vector<T> x0(n_);
+    for (int j = 0; j < n_; ++j) x0[j] = (T)x0_[j];
+    SUPR_HIP(hipMemcpyAsync(I.x[0].template as<T>(), x0.data(), sizeof(T) * n_, hipMemcpyHostToDevice, s));
+    SUPR_HIP(hipMemcpyAsync(I.xlast.template as<T>(), x0.data(), sizeof(T) * n_, hipMemcpyHostToDevice, s));
+    SUPR_HIP(hipMemsetAsync(I.y[0].template as<void>(), 0, sizeof(T) * m_, s));
+    SUPR_HIP(hipMemsetAsync(I.ylast.template as<void>(), 0, sizeof(T) * m_, s));
+    SUPR_HIP(hipMemsetAsync(I.xsum.template as<void>(), 0, sizeof(T) * n_, s));
+    SUPR_HIP(hipMemsetAsync(I.ysum.template as<void>(), 0, sizeof(T) * m_, s));
+    SUPR_HIP(hipStreamSynchronize(s));   // x0 is a host temporary
 }
 
 void Solver::setup_device_resident(bool mixed) {
@@ -578,8 +647,8 @@ void Solver::setup_device_resident(bool mixed) {
     std::vector<uint8_t> hineq(m);
     for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
     ineq_ = upload(hineq, s);
-    if (mixed) alloc_iter(F_, true);
-    alloc_iter(D_, !mixed);
+    if (mixed) alloc_iter(F_);
+    alloc_iter(D_);
     auto vec = [&](const double* p, int len) { return upload(std::vector<double>(p, p + len), s); };
     kc_ = vec(U_.c, n); kl_ = vec(U_.l, n); ku_ = vec(U_.u, n);
     krL_ = vec(U_.rL, m); krU_ = vec(U_.rU, m);
@@ -587,39 +656,42 @@ void Solver::setup_device_resident(bool mixed) {
     xu_ = device_vec<double>(n); xu2_ = device_vec<double>(n); xavg_ = device_vec<double>(n);
     yu_ = device_vec<double>(m); yu2_ = device_vec<double>(m); yavg_ = device_vec<double>(m);
     bestx_ = device_vec<double>(n); besty_ = device_vec<double>(m);
+    atyavg_ = device_vec<double>(n); axavg_ = device_vec<double>(m);
     K_.c = kc_.as<double>(); K_.l = kl_.as<double>(); K_.u = ku_.as<double>();
     K_.rL = krL_.as<double>(); K_.rU = krU_.as<double>();
     K_.Dr = kDr_.as<double>(); K_.Dc = kDc_.as<double>();
     K_.xu = xu_.as<double>(); K_.xu2 = xu2_.as<double>(); K_.xavg = xavg_.as<double>();
     K_.yu = yu_.as<double>(); K_.yu2 = yu2_.as<double>(); K_.yavg = yavg_.as<double>();
     K_.best_x = bestx_.as<double>(); K_.best_y = besty_.as<double>();
-    // partial-sum scratch: 1 + 2 + 8 + 6 + 1 + 1 slots per block
-    parts_ = device_vec<double>((size_t)19 * kMaxBlocks);
+    K_.aty_avg = atyavg_.as<double>(); K_.ax_avg = axavg_.as<double>();
+    // partial-sum scratch: 1 + 2 + 8 + 6 + 1 + 1 slots per block, plus dx2
+    parts_ = device_vec<double>((size_t)19 * kMaxBlocks + 64);
     double* p = parts_.as<double>();
     part_n_ = p; p += kMaxBlocks;
     part_m_ = p; p += 2 * kMaxBlocks;
     part_kn_ = p; p += 8 * kMaxBlocks;
     part_km_ = p; p += 6 * kMaxBlocks;
     part_rn_ = p; p += kMaxBlocks;
-    part_rm_ = p;
-    state_ = device_vec<DevState>(1);
+    part_rm_ = p; p += kMaxBlocks;
+    dx2_slot_ = p;
+    state_ = device_vec<DevState>(2);   // double-buffered (see pdlp_kernels.hpp)
 
-    // SpMV schedules, timed in the precision of the first loop
+    // SpMV schedules per iterate precision, timed on scratch vectors
     auto tune = [&](DevMatrix& M, int fixed, auto view, auto* xv, auto* yv) {
-        if (M.ell_w > 0) M.lanes = 1;
-        else if (fixed > 0) M.lanes = fixed;
+        if (fixed > 0) M.lanes = fixed;
+        else if (M.ell_w > 0) M.lanes = 1;
         else autotune_lanes(M, view, xv, yv, s);
+        return M.lanes;
     };
+    if (!valid_lanes(o_.spmv_lanes_A < 0 ? 1 : std::max(1, o_.spmv_lanes_A)) ||
+        !valid_lanes(o_.spmv_lanes_AT < 0 ? 1 : std::max(1, o_.spmv_lanes_AT)))
+        throw BadInput("bad lanes-per-row option");
     if (mixed) {
-        tune(A_, o_.spmv_lanes_A, A_.f32(), F_.x[1].as<float>(), F_.ax[1].as<float>());
-        tune(AT_, o_.spmv_lanes_AT, AT_.f32(), F_.y[1].as<float>(), F_.aty.as<float>());
-    } else {
-        tune(A_, o_.spmv_lanes_A, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>());
-        tune(AT_, o_.spmv_lanes_AT, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>());
+        lanesA_[0] = tune(A_, o_.spmv_lanes_A, A_.f32(), F_.x[1].as<float>(), F_.ax[1].as<float>());
+        lanesAT_[0] = tune(AT_, o_.spmv_lanes_AT, AT_.f32(), F_.y[1].as<float>(), F_.aty.as<float>());
     }
-    if (!valid_lanes(A_.lanes) || !valid_lanes(AT_.lanes)) throw BadInput("bad lanes-per-row option");
-    DevState st = initial_state(mixed);
-    SUPR_HIP(hipMemcpyAsync(state_.as<void>(), &st, sizeof(st), hipMemcpyHostToDevice, s));
+    lanesA_[1] = tune(A_, o_.spmv_lanes_A, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>());
+    lanesAT_[1] = tune(AT_, o_.spmv_lanes_AT, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>());
     SUPR_HIP(hipStreamSynchronize(s));
 }
 
@@ -642,9 +714,9 @@ DevState Solver::initial_state(bool mixed) const {
     st.obj_offset = U_.obj_offset;
     st.max_iter = o_.max_iterations;
     st.check_every = o_.check_every;
-    st.active = 1;
+    st.active = o_.max_iterations > 0 ? 1 : 0;
+    st.need_check = o_.max_iterations > 0 ? 0 : 1;
     st.need_aty = 1;
-    st.need_ax = 1;
     st.phase = mixed ? 0 : 1;
     return st;
 }
@@ -656,46 +728,55 @@ template <> MatView<double> iter_view<double>(const DevMatrix& M) { return M.f64
 template <typename T>
 void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     hipStream_t s = stream_.s;
-    DevState* st = state_.as<DevState>();
+    DevState* S = state_.as<DevState>();
     const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
     const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
     const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
     for (int k = 0; k < count; ++k) {
-        SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, st, AT, B, part_n_));
-        SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, st, A, B, part_m_));
-        hipLaunchKernelGGL(k_decide, dim3(1), dim3(kBlock), 0, s, st, part_n_, gn, part_m_, gm);
+        const int p = k & 1;
+        SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, S, p, AT, B,
+                                                 part_m_, gm, dx2_slot_, part_n_));
+        SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, S, p ^ 1, A, B,
+                                                part_n_, gn, dx2_slot_, part_m_));
     }
     SUPR_HIP(hipGetLastError());
 }
 
+// A x and A^T y of the current iterate with the plain SpMV (start-up and the
+// FP32 -> FP64 switch); outside any graph.
 template <typename T>
-void Solver::enqueue_ax_fix(const Iter<T>& I) {
+void Solver::set_products(const Iter<T>& I, const MatView<T>& A, const MatView<T>& AT) {
     hipStream_t s = stream_.s;
-    const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
-    const MatView<T> A = iter_view<T>(A_);
-    SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_ax_fix<LL, T>), dim3(grid_rows(m_, LL)), dim3(kBlock), 0, s,
-                                            state_.as<DevState>(), A, B));
-    SUPR_HIP(hipGetLastError());
+    launch_spmv(A_, A, I.x[0].template as<T>(), I.ax[0].template as<T>(), s);
+    launch_spmv(AT_, AT, I.y[0].template as<T>(), I.aty.template as<T>(), s);
+    SUPR_HIP(hipStreamSynchronize(s));
+}
+
+template <int LA>
+void launch_check_kkt(int lat, hipStream_t s, const DevState* S, const MatView<double>& AuT,
+                      const MatView<double>& Au, int gkn, int gkm, const KktBufs& K, double* pkn,
+                      double* pkm) {
+    SUPR_LANES(lat, hipLaunchKernelGGL((k_check_kkt<LA, LL>), dim3(gkn + gkm), dim3(kBlock), 0, s, S, AuT,
+                                       Au, gkn, K, pkn, pkm));
 }
 
 template <typename T>
 void Solver::enqueue_check(const Iter<T>& I) {
     hipStream_t s = stream_.s;
-    DevState* st = state_.as<DevState>();
+    DevState* S = state_.as<DevState>();
     const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
     const int en = grid_elems(n_), em = grid_elems(m_);
-    hipLaunchKernelGGL((k_kkt_prep_n<T>), dim3(en), dim3(kBlock), 0, s, st, n_, B, K_);
-    hipLaunchKernelGGL((k_kkt_prep_m<T>), dim3(em), dim3(kBlock), 0, s, st, m_, B, K_);
+    const int gm = grid_rows(m_, A_.lanes);
+    hipLaunchKernelGGL((k_check_prep<T>), dim3(en + em), dim3(kBlock), 0, s, S, n_, m_, en, B, K_, part_m_,
+                       gm, dx2_slot_);
     const int gkn = grid_rows(n_, AT_.lanes), gkm = grid_rows(m_, A_.lanes);
     const MatView<double> AuT = AT_.unscaled(), Au = A_.unscaled();
-    SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_kkt_n<LL>), dim3(gkn), dim3(kBlock), 0, s, st, AuT, K_, part_kn_));
-    SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_kkt_m<LL>), dim3(gkm), dim3(kBlock), 0, s, st, Au, K_, part_km_));
-    hipLaunchKernelGGL(k_kkt_fin, dim3(1), dim3(kBlock), 0, s, st, part_kn_, gkn, part_km_, gkm);
-    hipLaunchKernelGGL((k_restart_n<T>), dim3(en), dim3(kBlock), 0, s, st, n_, B, K_, part_rn_);
-    hipLaunchKernelGGL((k_restart_m<T>), dim3(em), dim3(kBlock), 0, s, st, m_, B, K_, part_rm_);
-    hipLaunchKernelGGL(k_restart_fin, dim3(1), dim3(kBlock), 0, s, st, part_rn_, en, part_rm_, em);
+    SUPR_LANES(A_.lanes, launch_check_kkt<LL>(AT_.lanes, s, S, AuT, Au, gkn, gkm, K_, part_kn_, part_km_));
+    hipLaunchKernelGGL(k_kkt_fin, dim3(1), dim3(kBlock), 0, s, S, part_kn_, gkn, part_km_, gkm);
+    hipLaunchKernelGGL((k_check_restart<T>), dim3(en + em), dim3(kBlock), 0, s, S, n_, m_, en, B, K_,
+                       part_rn_, part_rm_);
+    hipLaunchKernelGGL(k_restart_fin, dim3(1), dim3(kBlock), 0, s, S, part_rn_, en, part_rm_, em);
     SUPR_HIP(hipGetLastError());
-    enqueue_ax_fix(I);
 }
 
 template <typename T>
@@ -723,11 +804,15 @@ hipGraphExec_t Solver::capture_graph(const Iter<T>& I, int attempts) {
 // the stop callback fires.  A ring of pinned state copies keeps a few
 // replays in flight while the host polls the oldest one.
 template <typename T>
-int Solver::run_graph_loop(const Iter<T>& I, supr_stop_fn stop, void* ctx, DevState& hs,
-                           bool& stopped) {
+int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx,
+                           DevState& hs, bool& stopped) {
     hipStream_t s = stream_.s;
-    int G = o_.graph_attempts > 0 ? o_.graph_attempts : 16;
-    hipGraphExec_t exec = capture_graph(I, G);
+    if (!graph_[prec]) {
+        int G = o_.graph_attempts > 0 ? o_.graph_attempts : 16;
+        G += G & 1;   // the state double buffer needs an even number of attempts per replay
+        graph_[prec] = capture_graph(I, G);
+    }
+    hipGraphExec_t exec = graph_[prec];
     constexpr int R = 4;
     DevState* ring = nullptr;
     hipEvent_t ev[R] = {};
@@ -765,31 +850,31 @@ int Solver::run_graph_loop(const Iter<T>& I, supr_stop_fn stop, void* ctx, DevSt
         (void)hipStreamSynchronize(s);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         if (ring) (void)hipHostFree(ring);
-        (void)hipGraphExecDestroy(exec);
         throw;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     (void)hipHostFree(ring);
-    (void)hipGraphExecDestroy(exec);
     return replays;
 }
 
-void Solver::solve_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
-                                   double* y_out, supr_pdlp_info* info) {
-    const auto t0 = Clock::now();
+void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
+                                 double* y_out, supr_pdlp_info* info) {
     hipStream_t s = stream_.s;
-    setup_device_resident(mixed);
-    if (mixed) enqueue_ax_fix(F_);
-    else enqueue_ax_fix(D_);
-    SUPR_HIP(hipStreamSynchronize(s));
-    const double t_setup = seconds_since(t0);
+    const int prec0 = mixed ? 0 : 1;
+    use_lanes(prec0);
+    if (mixed) reset_iter(F_);
+    else reset_iter(D_);
+    const DevState st0[2] = {initial_state(mixed), initial_state(mixed)};
+    SUPR_HIP(hipMemcpyAsync(state_.as<void>(), st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    if (mixed) set_products(F_, A_.f32(), AT_.f32());
+    else set_products(D_, A_.f64(), AT_.f64());
     const auto t1 = Clock::now();
 
     DevState hs;
     bool stopped = false;
     int fp32_attempts = 0;
     if (mixed) {
-        run_graph_loop(F_, stop, ctx, hs, stopped);
+        run_graph_loop(F_, 0, stop, ctx, hs, stopped);
         fp32_attempts = hs.attempts;
         if (o_.verbose)
             std::fprintf(stderr, "[suprema] fp32 loop: attempts=%d restarts=%d checks=%d merit=%.3e best=%.3e switch=%d done=%d\n",
@@ -811,19 +896,18 @@ void Solver::solve_device_resident(bool mixed, supr_stop_fn stop, void* ctx, dou
             hs.merit_prev_cand = INFINITY;
             hs.pending = 0;
             hs.need_check = 0;
-            hs.need_aty = 1;
-            hs.need_ax = 1;
+            hs.need_aty = 0;
+            hs.decide_pending = 0;
             hs.active = 1;
             hs.stall = 0;
-            SUPR_HIP(hipMemcpyAsync(state_.as<void>(), &hs, sizeof(hs), hipMemcpyHostToDevice, s));
-            // FP64 schedules (the FP64 stream is twice the bytes; re-time)
-            if (o_.spmv_lanes_A <= 0 && A_.ell_w == 0) autotune_lanes(A_, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>(), s);
-            if (o_.spmv_lanes_AT <= 0 && AT_.ell_w == 0) autotune_lanes(AT_, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>(), s);
-            enqueue_ax_fix(D_);
-            run_graph_loop(D_, stop, ctx, hs, stopped);
+            const DevState both[2] = {hs, hs};
+            SUPR_HIP(hipMemcpyAsync(state_.as<void>(), both, sizeof(both), hipMemcpyHostToDevice, s));
+            use_lanes(1);
+            set_products(D_, A_.f64(), AT_.f64());
+            run_graph_loop(D_, 1, stop, ctx, hs, stopped);
         }
     } else {
-        run_graph_loop(D_, stop, ctx, hs, stopped);
+        run_graph_loop(D_, 1, stop, ctx, hs, stopped);
     }
     const double t_solve = seconds_since(t1);
 
@@ -867,40 +951,35 @@ void Solver::solve_device_resident(bool mixed, supr_stop_fn stop, void* ctx, dou
     info->iterations_fp32 = fp32_attempts;
     info->checks = hs.checks;
     info->lanes_A = A_.lanes; info->lanes_AT = AT_.lanes;
-    info->seconds_setup = t_setup;
     info->seconds_solve = t_solve;
     if (o_.verbose)
-        std::fprintf(stderr, "[suprema] done: attempts=%d (fp32 %d) restarts=%d checks=%d eps=%.2e/%.2e/%.2e conv=%d solve=%.4fs setup=%.4fs lanes A=%d AT=%d ell A=%d AT=%d\n",
-                     hs.attempts, fp32_attempts, hs.restarts, hs.checks, info->eps_P, info->eps_D, info->eps_G,
-                     info->converged, t_solve, t_setup, A_.lanes, AT_.lanes, A_.ell_w, AT_.ell_w);
-}
-
-void Solver::solve(supr_stop_fn stop, void* ctx, double* x_out, double* y_out,
-                   supr_pdlp_info* info) {
-    const auto t0 = Clock::now();
-    switch (o_.mode) {
-        case SUPR_MODE_PLAIN_FP64: solve_plain(stop, ctx, x_out, y_out, info); break;
-        case SUPR_MODE_FAST_FP64: solve_device_resident(false, stop, ctx, x_out, y_out, info); break;
-        case SUPR_MODE_MIXED: solve_device_resident(true, stop, ctx, x_out, y_out, info); break;
-        default: throw BadInput("unknown mode " + std::to_string(o_.mode));
-    }
-    info->seconds_total = seconds_since(t0);
+        std::fprintf(stderr, "[suprema] done: attempts=%d (fp32 %d) accepted=%d restarts=%d checks=%d eps=%.2e/%.2e/%.2e conv=%d solve=%.4fs setup=%.4fs lanes A=%d AT=%d ell A=%d AT=%d\n",
+                     hs.attempts, fp32_attempts, hs.accepted, hs.restarts, hs.checks, info->eps_P, info->eps_D, info->eps_G,
+                     info->converged, t_solve, setup_s_, A_.lanes, AT_.lanes, A_.ell_w, AT_.ell_w);
 }
 
 // Standalone timing of the two iteration kernels at their real shapes (the
 // kernels the graph replays), for the roofline figure:
-//   out[0] ms per k_primal, out[1] ms per k_dual, out[2] ms per k_primal
-//   (FP64), out[3] ms per k_dual (FP64), out[4] lanes A, out[5] lanes A^T,
-//   out[6] ELL width of A, out[7] ELL width of A^T.
-// The loop control is opened up (no checks, no limit) so every launch does
-// a full accepted-step's work: A^T y, averages, projections, A x.
+//   out[0] ms per k_primal, out[1] ms per k_dual (FP32), out[2], out[3] the
+//   same in FP64, out[4] lanes A, out[5] lanes A^T, out[6] ELL width of A,
+//   out[7] ELL width of A^T.
+// The state is set so every launch does a full accepted step's work (A^T y,
+// average terms, projections, A x) and no decision is pending: k_primal
+// reads S[0] and writes S[1], k_dual reads S[1], so repeated launches see the
+// same state.
 void Solver::bench_kernels(int reps, double* out) {
     hipStream_t s = stream_.s;
-    setup_device_resident(true);
-    enqueue_ax_fix(F_);
+    if (o_.mode != SUPR_MODE_MIXED || !ready_) throw BadInput("bench_kernels needs a set-up mixed-mode engine");
+    use_lanes(0);
+    reset_iter(F_);
+    set_products(F_, A_.f32(), AT_.f32());
     DevState st = initial_state(true);
     st.check_every = std::numeric_limits<int32_t>::max();
     st.max_iter = std::numeric_limits<int32_t>::max();
+    st.pending = 1;
+    st.eta_acc = st.eta;
+    st.need_aty = 1;
+    st.decide_pending = 0;
     hipEvent_t e0, e1;
     SUPR_HIP(hipEventCreate(&e0));
     SUPR_HIP(hipEventCreate(&e1));
@@ -919,37 +998,23 @@ void Solver::bench_kernels(int reps, double* out) {
         const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
         const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
         const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
-        DevState* d = state_.as<DevState>();
-        SUPR_HIP(hipMemcpyAsync(d, &st, sizeof(st), hipMemcpyHostToDevice, s));
-        // need_aty and pending stay set: every k_primal does the accepted-step work
-        DevState st2 = st;
-        st2.pending = 1;
-        st2.eta_acc = st.eta;
-        SUPR_HIP(hipMemcpyAsync(d, &st2, sizeof(st2), hipMemcpyHostToDevice, s));
+        DevState* S = state_.as<DevState>();
+        const DevState both[2] = {st, st};
+        SUPR_HIP(hipMemcpyAsync(S, both, sizeof(both), hipMemcpyHostToDevice, s));
         out[slot] = time_kernel([&] {
-            SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, d, AT, B, part_n_));
+            SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, S, 0, AT, B,
+                                                     part_m_, gm, dx2_slot_, part_n_));
         });
         out[slot + 1] = time_kernel([&] {
-            SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, d, A, B, part_m_));
+            SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, S, 1, A, B,
+                                                    part_n_, gn, dx2_slot_, part_m_));
         });
         SUPR_HIP(hipGetLastError());
     };
     run(F_, float{}, 0);
-    // FP64 buffers were left uninitialised in the mixed setup: give them a point
-    {
-        std::vector<double> x0 = x0_;
-        SUPR_HIP(hipMemcpyAsync(D_.x[0].as<void>(), x0.data(), sizeof(double) * n_, hipMemcpyHostToDevice, s));
-        SUPR_HIP(hipMemcpyAsync(D_.x[1].as<void>(), x0.data(), sizeof(double) * n_, hipMemcpyHostToDevice, s));
-        SUPR_HIP(hipMemsetAsync(D_.y[0].as<void>(), 0, sizeof(double) * m_, s));
-        SUPR_HIP(hipMemsetAsync(D_.y[1].as<void>(), 0, sizeof(double) * m_, s));
-        SUPR_HIP(hipMemsetAsync(D_.ax[0].as<void>(), 0, sizeof(double) * m_, s));
-        SUPR_HIP(hipMemsetAsync(D_.ax[1].as<void>(), 0, sizeof(double) * m_, s));
-        SUPR_HIP(hipMemsetAsync(D_.xsum.as<void>(), 0, sizeof(double) * n_, s));
-        SUPR_HIP(hipMemsetAsync(D_.ysum.as<void>(), 0, sizeof(double) * m_, s));
-        SUPR_HIP(hipStreamSynchronize(s));
-    }
-    if (A_.ell_w == 0) autotune_lanes(A_, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>(), s);
-    if (AT_.ell_w == 0) autotune_lanes(AT_, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>(), s);
+    use_lanes(1);
+    reset_iter(D_);
+    set_products(D_, A_.f64(), AT_.f64());
     run(D_, double{}, 2);
     SUPR_HIP(hipStreamSynchronize(s));
     out[4] = A_.lanes; out[5] = AT_.lanes; out[6] = A_.ell_w; out[7] = AT_.ell_w;
@@ -1046,13 +1111,60 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
     if (!x_out || !y_out || !info) { set_error("null output"); return SUPR_EINVAL; }
     try {
         SUPR_HIP(hipSetDevice(opt->device));
-        std::memset(info, 0, sizeof(*info));
         Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt);
-        solver.solve(stop, stop_ctx, x_out, y_out, info);
+        solver.setup();
+        solver.run(stop, stop_ctx, x_out, y_out, info);
         return SUPR_OK;
     } catch (...) {
         return map_exception();
     }
+}
+
+struct supr_engine {
+    int device;
+    std::unique_ptr<supr::Solver> solver;
+};
+
+int supr_engine_create(const supr_lp* unscaled_form, const supr_lp* scaled_form, const double* Dr,
+                       const double* Dc, const supr_pdlp_options* opt, supr_engine** out) {
+    using namespace supr;
+    int rc = check_solve_args(unscaled_form, scaled_form, Dr, Dc, opt);
+    if (rc != SUPR_OK) return rc;
+    if (!out) { set_error("null engine output"); return SUPR_EINVAL; }
+    *out = nullptr;
+    try {
+        SUPR_HIP(hipSetDevice(opt->device));
+        auto e = std::make_unique<supr_engine>();
+        e->device = opt->device;
+        e->solver = std::make_unique<Solver>(*unscaled_form, *scaled_form, Dr, Dc, *opt);
+        e->solver->setup();
+        *out = e.release();
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int supr_engine_solve(supr_engine* engine, supr_stop_fn stop, void* stop_ctx, double* x_out,
+                      double* y_out, supr_pdlp_info* info) {
+    using namespace supr;
+    if (!engine || !engine->solver || !x_out || !y_out || !info) {
+        set_error("supr_engine_solve: null argument");
+        return SUPR_EINVAL;
+    }
+    try {
+        SUPR_HIP(hipSetDevice(engine->device));
+        engine->solver->run(stop, stop_ctx, x_out, y_out, info);
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+void supr_engine_destroy(supr_engine* engine) {
+    if (!engine) return;
+    (void)hipSetDevice(engine->device);
+    delete engine;
 }
 
 // y = A x (transposed = 0) or A^T x (transposed = 1) with one schedule.
@@ -1106,7 +1218,10 @@ int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
     if (!out8 || reps < 1) { set_error("null output or reps < 1"); return SUPR_EINVAL; }
     try {
         SUPR_HIP(hipSetDevice(opt->device));
-        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt);
+        supr_pdlp_options o = *opt;
+        o.mode = SUPR_MODE_MIXED;
+        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, o);
+        solver.setup();
         solver.bench_kernels(reps, out8);
         return SUPR_OK;
     } catch (...) {

@@ -5,7 +5,7 @@
 //  * plain kernels (k_spmv, k_plain_*): one operation per launch, the shape of
 //    the reference GPU backend (gpu/pdlp_gpu_solve.cu:42-138) -- used by the
 //    host-driven plain FP64 mode and by the SpMV autotuner;
-//  * device-resident kernels (k_primal, k_dual, k_kkt_*, k_restart_*): the
+//  * device-resident kernels (k_primal, k_dual, k_check_*, k_*_fin): the
 //    whole PDHG loop of include/pdlp_algo.hpp:103-175 with its step-size rule,
 //    ergodic averages, KKT check and restart rule evaluated on the device.
 //    Control flow lives in a DevState block in device memory; each kernel
@@ -58,7 +58,7 @@ struct DevState {
     int32_t cur;             // which half of each double buffer is current
     int32_t active;          // iterations run (0: paused for a check, or finished)
     int32_t need_aty;        // next k_primal recomputes A^T y
-    int32_t need_ax;         // k_ax_fix recomputes A x
+    int32_t decide_pending;  // the last attempt's step decision is not applied yet
     int32_t pending;         // last accepted step not yet added to the averages
     int32_t need_check;
     int32_t done;            // 1 converged, 2 iteration limit
@@ -109,6 +109,8 @@ struct KktBufs {
     double* yavg;
     double* best_x;
     double* best_y;
+    double* aty_avg;   // scaled A^T y_avg (n), A x_avg (m) from the check
+    double* ax_avg;
 };
 
 // ------------------------------------------------------------- helpers --
@@ -268,6 +270,53 @@ __device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __re
     s2 = group_sum<L, double>(s2);
 }
 
+// A x1 and A (x1 - x2) in one pass over the row (one index stream, two
+// gathers): the FP32 loop takes the extrapolation A(2 xn - x) as
+// A xn + A (xn - x) and the step statistic (yn - y)'A(xn - x) from the
+// difference itself, which keeps them accurate when xn and x are close
+// (A xn - A x would cancel to rounding noise in FP32).
+template <int L, typename T>
+__device__ __forceinline__ void row_dot_diff(const MatView<T>& M, const T* __restrict__ x1,
+                                             const T* __restrict__ x2, int row, int lane,
+                                             bool valid, T& s1, T& sd) {
+    constexpr int U = 4;
+    s1 = T(0); sd = T(0);
+    if (M.ell_w > 0) {
+        if (valid) {
+            const size_t R = (size_t)M.rows;
+            for (int k = lane; k < M.ell_w; k += L) {
+                const size_t q = (size_t)k * R + row;
+                const int c = M.idx[q];
+                const T a = M.val[q], v1 = x1[c];
+                s1 += a * v1;
+                sd += a * (v1 - x2[c]);
+            }
+        }
+    } else {
+        int b = 0, e = 0;
+        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+        int p = b + lane;
+        for (; p + (U - 1) * L < e; p += U * L) {
+            int c[U];
+            T a[U], v1[U], v2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = M.val[p + u * L]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { v1[u] = x1[c[u]]; v2[u] = x2[c[u]]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { s1 += a[u] * v1[u]; sd += a[u] * (v1[u] - v2[u]); }
+        }
+        for (; p < e; p += L) {
+            const int c = M.idx[p];
+            const T a = M.val[p], v1 = x1[c];
+            s1 += a * v1;
+            sd += a * (v1 - x2[c]);
+        }
+    }
+    s1 = group_sum<L, T>(s1);
+    sd = group_sum<L, T>(sd);
+}
+
 // Grid-stride loop over rows, L lanes per row; body(row, lane, valid) runs
 // with block-uniform trip counts so group reductions always see full groups.
 // XCD-aware mapping: the hardware deals blocks round-robin to the 8 XCDs
@@ -369,13 +418,24 @@ __global__ __launch_bounds__(kBlock) void k_plain_axpy(int len, double a,
 }
 
 // ------------------------------------------- device-resident PDHG loop --
+//
+// Launch sequence of one graph replay (G attempts, G even):
+//   k_primal(slot 0), k_dual, k_primal(slot 1), k_dual, ... , then the check
+//   group k_check_prep, k_check_kkt, k_kkt_fin, k_check_restart, k_restart_fin.
+// The step decision of attempt t is not a launch of its own: every block of
+// the next kernel (k_primal of attempt t+1, or k_check_prep) recomputes it
+// from the dual partials of attempt t and the primal sum k_dual's block 0
+// stored -- same data, same order, same bits in every block.  To let blocks
+// read the old state while block 0 publishes the new one, the state is
+// double-buffered: k_primal of graph slot k reads S[k & 1] and writes
+// S[(k + 1) & 1]; k_dual reads S[(k + 1) & 1]; the check group works on S[1]
+// and k_restart_fin copies S[1] back to S[0] for the next replay.
 
-// The step-size rule of include/pdlp_algo.hpp:113-131 on the device.
+// The step-size rule of include/pdlp_algo.hpp:113-131.
 __device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2,
                                             double inter) {
     st->attempts += 1;
     st->pending = 0;
-    st->need_ax = 0;
     const double eta = st->eta, omega = st->omega;
     const double num = omega * dx2 + dy2 / omega;
     const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : INFINITY;
@@ -399,18 +459,49 @@ __device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2
     if (st->need_check) st->active = 0;
 }
 
+// Loads the state S[p] into LDS, applies the pending step decision (if the
+// previous launch ran an attempt), and publishes the result to S[p ^ 1]
+// (block 0).  Returns the block's copy.
+__device__ __forceinline__ void load_and_decide(DevState* __restrict__ S, int p,
+                                                const double* __restrict__ part_m, int nb_m,
+                                                const double* __restrict__ dx2_slot,
+                                                DevState& Q) {
+    if (threadIdx.x == 0) Q = S[p];
+    __syncthreads();
+    if (Q.decide_pending) {
+        double sm[2];
+        sum_partials<2>(part_m, nb_m, sm);
+        if (threadIdx.x == 0) {
+            decide_step(&Q, *dx2_slot, sm[0], sm[1]);
+            Q.decide_pending = 0;
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        DevState out = Q;
+        out.decide_pending = 0;
+        S[p ^ 1] = out;
+    }
+}
+
 // Primal half of one attempt, over the columns (rows of A^T):
 //   [after an accept] aty = A^T y, xsum += eta_acc x  (pending average term)
 //   xn = clip(x - tau (c - aty), l, u);  partial sums of (xn - x)^2 in FP64.
 template <int L, typename T>
-__global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ st, MatView<T> AT,
-                                                   IterBufs<T> B, double* __restrict__ part_n) {
-    if (!st->active) return;
-    const int cur = st->cur;
-    const bool need_aty = st->need_aty != 0;
-    const bool pending = st->pending != 0;
-    const T eta_acc = (T)st->eta_acc;
-    const T tau = (T)(st->eta / st->omega);
+__global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int p,
+                                                   MatView<T> AT, IterBufs<T> B,
+                                                   const double* __restrict__ part_m, int nb_m,
+                                                   const double* __restrict__ dx2_slot,
+                                                   double* __restrict__ part_n) {
+    __shared__ DevState Q;
+    load_and_decide(S, p, part_m, nb_m, dx2_slot, Q);
+    if (!Q.active) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[p ^ 1].decide_pending = 1;   // k_dual will run
+    const int cur = Q.cur;
+    const bool need_aty = Q.need_aty != 0;
+    const bool pending = Q.pending != 0;
+    const T eta_acc = (T)Q.eta_acc;
+    const T tau = (T)(Q.eta / Q.omega);
     const T* __restrict__ x = B.x[cur];
     T* __restrict__ xn = B.x[cur ^ 1];
     const T* __restrict__ y = B.y[cur];
@@ -436,12 +527,16 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ st, Ma
 }
 
 // Dual half of one attempt, over the rows of A:
-//   axn = A xn;  [pending] ysum += eta_acc y;
-//   yn = proj(y + sigma (b - (2 axn - ax)));  partials of (yn-y)^2 and
-//   (yn-y)(axn-ax).
+//   axn = A xn, ad = A (xn - x);  [pending] ysum += eta_acc y;
+//   yn = proj(y + sigma (b - (axn + ad)));  partials of (yn-y)^2 and
+//   (yn-y)'ad.  Block 0 also sums k_primal's partials into *dx2_slot.
 template <int L, typename T>
-__global__ __launch_bounds__(kBlock) void k_dual(DevState* __restrict__ st, MatView<T> A,
-                                                 IterBufs<T> B, double* __restrict__ part_m) {
+__global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S, int p1,
+                                                 MatView<T> A, IterBufs<T> B,
+                                                 const double* __restrict__ part_n, int nb_n,
+                                                 double* __restrict__ dx2_slot,
+                                                 double* __restrict__ part_m) {
+    const DevState* st = S + p1;
     if (!st->active) return;
     const int cur = st->cur;
     const bool pending = st->pending != 0;
@@ -453,19 +548,29 @@ __global__ __launch_bounds__(kBlock) void k_dual(DevState* __restrict__ st, MatV
     const T* __restrict__ ax = B.ax[cur];
     T* __restrict__ axn = B.ax[cur ^ 1];
     double acc[2] = {0.0, 0.0};
+    // FP32: A(xn - x) from the difference (see row_dot_diff); FP64: the
+    // reference's 2 A xn - A x (pdlp_gpu_solve.cu:89) with A x carried.
+    constexpr bool kDiff = sizeof(T) == 4;
+    const T* __restrict__ x = B.x[cur];
     for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
-        const T a = row_dot<L, T>(A, xn, row, lane, valid);
+        T a, ad;
+        if constexpr (kDiff) {
+            row_dot_diff<L, T>(A, xn, x, row, lane, valid, a, ad);
+        } else {
+            a = row_dot<L, T>(A, xn, row, lane, valid);
+            ad = T(0);
+        }
         if (valid && lane == 0) {
             axn[row] = a;
             const T yi = y[row];
             if (pending) B.ysum[row] += eta_acc * yi;
-            const T axi = ax[row];
-            T v = yi + sigma * (B.b[row] - (T(2) * a - axi));
+            if constexpr (!kDiff) ad = a - ax[row];
+            T v = yi + sigma * (B.b[row] - (a + ad));
             if (B.ineq[row] && v < T(0)) v = T(0);
             yn[row] = v;
             const double d = (double)v - (double)yi;
             acc[0] += d * d;
-            acc[1] += d * ((double)a - (double)axi);
+            acc[1] += d * (double)ad;
         }
     });
     block_sum<2>(acc);
@@ -473,153 +578,146 @@ __global__ __launch_bounds__(kBlock) void k_dual(DevState* __restrict__ st, MatV
         part_m[(size_t)blockIdx.x * 2] = acc[0];
         part_m[(size_t)blockIdx.x * 2 + 1] = acc[1];
     }
-}
-
-// Step decision of one attempt (one block): sums the primal and dual
-// partials in a fixed order and applies the step-size rule.
-__global__ __launch_bounds__(kBlock) void k_decide(DevState* __restrict__ st,
-                                                   const double* __restrict__ part_n, int nb_n,
-                                                   const double* __restrict__ part_m, int nb_m) {
-    if (!st->active) return;
-    double sn[1], sm[2];
-    sum_partials<1>(part_n, nb_n, sn);
-    sum_partials<2>(part_m, nb_m, sm);
-    if (threadIdx.x == 0) decide_step(st, sn[0], sm[0], sm[1]);
-}
-
-// A x for the current iterate (after a restart to the average, at start-up
-// and after the FP32 -> FP64 switch).
-template <int L, typename T>
-__global__ __launch_bounds__(kBlock) void k_ax_fix(DevState* __restrict__ st, MatView<T> A,
-                                                   IterBufs<T> B) {
-    if (!st->need_ax) return;
-    const T* __restrict__ x = B.x[st->cur];
-    T* __restrict__ ax = B.ax[st->cur];
-    for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
-        const T a = row_dot<L, T>(A, x, row, lane, valid);
-        if (valid && lane == 0) ax[row] = a;
-    });
-}
-
-// KKT check, step 1 (columns): add the pending average term, form the
-// average and the unscaled candidates xu = Dc x, xu2 = Dc x_avg.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_kkt_prep_n(DevState* __restrict__ st, int n,
-                                                       IterBufs<T> B, KktBufs K) {
-    if (!st->need_check) return;
-    const int cur = st->cur;
-    const bool pending = st->pending != 0;
-    const T eta_acc = (T)st->eta_acc;
-    const double wsum = st->wsum;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        const T xj = B.x[cur][j];
-        T xs = B.xsum[j];
-        if (pending) { xs += eta_acc * xj; B.xsum[j] = xs; }
-        const double avg = wsum > 0 ? (double)xs / wsum : (double)xj;
-        K.xavg[j] = avg;
-        K.xu[j] = K.Dc[j] * (double)xj;
-        K.xu2[j] = K.Dc[j] * avg;
+    if (blockIdx.x == 0) {
+        double sn[1];
+        sum_partials<1>(part_n, nb_n, sn);
+        if (threadIdx.x == 0) *dx2_slot = sn[0];
     }
 }
 
+// KKT check, step 1 (blocks [0, en): columns; [en, en + em): rows): applies
+// the last attempt's decision; if a check is due, adds the pending average
+// term and forms the averages and the unscaled candidates
+// xu = Dc x, xu2 = Dc x_avg, yu = Dr y, yu2 = Dr y_avg.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_kkt_prep_m(DevState* __restrict__ st, int m,
-                                                       IterBufs<T> B, KktBufs K) {
-    if (!st->need_check) return;
-    const int cur = st->cur;
-    const bool pending = st->pending != 0;
-    const T eta_acc = (T)st->eta_acc;
-    const double wsum = st->wsum;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
-        const T yi = B.y[cur][i];
-        T ys = B.ysum[i];
-        if (pending) { ys += eta_acc * yi; B.ysum[i] = ys; }
-        const double avg = wsum > 0 ? (double)ys / wsum : (double)yi;
-        K.yavg[i] = avg;
-        K.yu[i] = K.Dr[i] * (double)yi;
-        K.yu2[i] = K.Dr[i] * avg;
+__global__ __launch_bounds__(kBlock) void k_check_prep(DevState* __restrict__ S, int n, int m,
+                                                       int en, IterBufs<T> B, KktBufs K,
+                                                       const double* __restrict__ part_m, int nb_m,
+                                                       const double* __restrict__ dx2_slot) {
+    __shared__ DevState Q;
+    load_and_decide(S, 0, part_m, nb_m, dx2_slot, Q);
+    if (!Q.need_check) return;
+    const int cur = Q.cur;
+    const bool pending = Q.pending != 0;
+    const T eta_acc = (T)Q.eta_acc;
+    const double wsum = Q.wsum;
+    if ((int)blockIdx.x < en) {
+        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += en * kBlock) {
+            const T xj = B.x[cur][j];
+            T xs = B.xsum[j];
+            if (pending) { xs += eta_acc * xj; B.xsum[j] = xs; }
+            const double avg = wsum > 0 ? (double)xs / wsum : (double)xj;
+            K.xavg[j] = avg;
+            K.xu[j] = K.Dc[j] * (double)xj;
+            K.xu2[j] = K.Dc[j] * avg;
+        }
+    } else {
+        const int b = blockIdx.x - en, em = gridDim.x - en;
+        for (int i = b * kBlock + threadIdx.x; i < m; i += em * kBlock) {
+            const T yi = B.y[cur][i];
+            T ys = B.ysum[i];
+            if (pending) { ys += eta_acc * yi; B.ysum[i] = ys; }
+            const double avg = wsum > 0 ? (double)ys / wsum : (double)yi;
+            K.yavg[i] = avg;
+            K.yu[i] = K.Dr[i] * (double)yi;
+            K.yu2[i] = K.Dr[i] * avg;
+        }
     }
 }
 
-// KKT check, step 2 (columns of the unscaled A, both candidates): bound
-// violations, dual sign violations of z = c - A^T y, the column part of the
-// dual objective, c'x.  8 partials per block (src/verify.cpp:24-94).
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_kkt_n(DevState* __restrict__ st,
-                                                  MatView<double> AuT, KktBufs K,
-                                                  double* __restrict__ part) {
-    if (!st->need_check) return;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for_rows<L>(AuT.rows, [&](int row, int lane, bool valid) {
-        double s1, s2;
-        row_dot2<L>(AuT, K.yu, K.yu2, row, lane, valid, s1, s2);
-        if (valid && lane == 0) {
-            const double cj = K.c[row], lj = K.l[row], uj = K.u[row];
-            const bool lf = isfinite(lj), uf = isfinite(uj);
+// KKT check, step 2: blocks [0, gkn) take the columns (rows of the unscaled
+// A^T), blocks [gkn, ...) the rows of the unscaled A, both candidates at
+// once (src/verify.cpp:24-94):
+//   columns: bound violations, sign violations of z = c - A^T y, the column
+//            part of the dual objective, c'x -> 8 partials per block;
+//   rows:    row violations, sign violations of y, the row part of the
+//            dual objective -> 6 partials per block.
+// It also keeps the scaled products of the average, A^T y_avg = Dc (A_u^T
+// yu2) and A x_avg = Dr (A_u xu2), so a restart to the average needs no SpMV.
+template <int LA, int LAT>
+__global__ __launch_bounds__(kBlock) void k_check_kkt(const DevState* __restrict__ S,
+                                                      MatView<double> AuT, MatView<double> Au,
+                                                      int gkn, KktBufs K,
+                                                      double* __restrict__ part_kn,
+                                                      double* __restrict__ part_km) {
+    if (!S[1].need_check) return;
+    if ((int)blockIdx.x < gkn) {
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int G = gkn;
+        const int gpb = kBlock / LAT;
+        const int g = threadIdx.x / LAT, lane = threadIdx.x % LAT;
+        for (int r0 = blockIdx.x * gpb; r0 < AuT.rows; r0 += G * gpb) {
+            const int row = r0 + g;
+            const bool valid = row < AuT.rows;
+            double s1, s2;
+            row_dot2<LAT>(AuT, K.yu, K.yu2, row, lane, valid, s1, s2);
+            if (valid && lane == 0) {
+                const double cj = K.c[row], lj = K.l[row], uj = K.u[row];
+                const bool lf = isfinite(lj), uf = isfinite(uj);
+                K.aty_avg[row] = K.Dc[row] * s2;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const double xv = k ? K.xu2[row] : K.xu[row];
-                const double z = cj - (k ? s2 : s1);
-                const double bv = smax(lj - xv, 0.0) + smax(xv - uj, 0.0);
-                double sv = 0.0;
-                if (!lf && z > 0) sv += z;
-                if (!uf && z < 0) sv += -z;
-                double zv = z, dcol = 0.0;
-                if (!lf && zv > 0) zv = 0.0;
-                if (!uf && zv < 0) zv = 0.0;
-                if (zv > 0 && lf) dcol += lj * zv;
-                if (zv < 0 && uf) dcol += -uj * -zv;
-                acc[4 * k + 0] += bv * bv;
-                acc[4 * k + 1] += sv * sv;
-                acc[4 * k + 2] += dcol;
-                acc[4 * k + 3] += cj * xv;
+                for (int k = 0; k < 2; ++k) {
+                    const double xv = k ? K.xu2[row] : K.xu[row];
+                    const double z = cj - (k ? s2 : s1);
+                    const double bv = smax(lj - xv, 0.0) + smax(xv - uj, 0.0);
+                    double sv = 0.0;
+                    if (!lf && z > 0) sv += z;
+                    if (!uf && z < 0) sv += -z;
+                    double zv = z, dcol = 0.0;
+                    if (!lf && zv > 0) zv = 0.0;
+                    if (!uf && zv < 0) zv = 0.0;
+                    if (zv > 0 && lf) dcol += lj * zv;
+                    if (zv < 0 && uf) dcol += -uj * -zv;
+                    acc[4 * k + 0] += bv * bv;
+                    acc[4 * k + 1] += sv * sv;
+                    acc[4 * k + 2] += dcol;
+                    acc[4 * k + 3] += cj * xv;
+                }
             }
         }
-    });
-    block_sum<8>(acc);
-    if (threadIdx.x == 0) {
+        block_sum<8>(acc);
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) part[(size_t)blockIdx.x * 8 + q] = acc[q];
-    }
-}
-
-// KKT check, step 3 (rows of the unscaled A, both candidates): row
-// violations, sign violations of y on infinite row bounds, the row part of
-// the dual objective.  6 partials per block.
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_kkt_m(DevState* __restrict__ st, MatView<double> Au,
-                                                  KktBufs K, double* __restrict__ part) {
-    if (!st->need_check) return;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for_rows<L>(Au.rows, [&](int row, int lane, bool valid) {
-        double a1, a2;
-        row_dot2<L>(Au, K.xu, K.xu2, row, lane, valid, a1, a2);
-        if (valid && lane == 0) {
-            const double lo = K.rL[row], hi = K.rU[row];
-            const bool lf = isfinite(lo), hf = isfinite(hi);
+            for (int q = 0; q < 8; ++q) part_kn[(size_t)blockIdx.x * 8 + q] = acc[q];
+        }
+    } else {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        const int G = gridDim.x - gkn, bb = blockIdx.x - gkn;
+        const int gpb = kBlock / LA;
+        const int g = threadIdx.x / LA, lane = threadIdx.x % LA;
+        for (int r0 = bb * gpb; r0 < Au.rows; r0 += G * gpb) {
+            const int row = r0 + g;
+            const bool valid = row < Au.rows;
+            double a1, a2;
+            row_dot2<LA>(Au, K.xu, K.xu2, row, lane, valid, a1, a2);
+            if (valid && lane == 0) {
+                const double lo = K.rL[row], hi = K.rU[row];
+                const bool lf = isfinite(lo), hf = isfinite(hi);
+                K.ax_avg[row] = K.Dr[row] * a2;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const double a = k ? a2 : a1;
-                const double yv0 = k ? K.yu2[row] : K.yu[row];
-                const double rv = smax(lo - a, 0.0) + smax(a - hi, 0.0);
-                double sv = 0.0;
-                if (!lf && yv0 > 0) sv += yv0;
-                if (!hf && yv0 < 0) sv += -yv0;
-                double yv = yv0, drow = 0.0;
-                if (!lf && yv > 0) yv = 0.0;
-                if (!hf && yv < 0) yv = 0.0;
-                if (yv > 0 && lf) drow += lo * yv;
-                if (yv < 0 && hf) drow += -hi * -yv;
-                acc[3 * k + 0] += rv * rv;
-                acc[3 * k + 1] += sv * sv;
-                acc[3 * k + 2] += drow;
+                for (int k = 0; k < 2; ++k) {
+                    const double a = k ? a2 : a1;
+                    const double yv0 = k ? K.yu2[row] : K.yu[row];
+                    const double rv = smax(lo - a, 0.0) + smax(a - hi, 0.0);
+                    double sv = 0.0;
+                    if (!lf && yv0 > 0) sv += yv0;
+                    if (!hf && yv0 < 0) sv += -yv0;
+                    double yv = yv0, drow = 0.0;
+                    if (!lf && yv > 0) yv = 0.0;
+                    if (!hf && yv < 0) yv = 0.0;
+                    if (yv > 0 && lf) drow += lo * yv;
+                    if (yv < 0 && hf) drow += -hi * -yv;
+                    acc[3 * k + 0] += rv * rv;
+                    acc[3 * k + 1] += sv * sv;
+                    acc[3 * k + 2] += drow;
+                }
             }
         }
-    });
-    block_sum<6>(acc);
-    if (threadIdx.x == 0) {
+        block_sum<6>(acc);
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) part[(size_t)blockIdx.x * 6 + q] = acc[q];
+            for (int q = 0; q < 6; ++q) part_km[(size_t)bb * 6 + q] = acc[q];
+        }
     }
 }
 
@@ -627,12 +725,13 @@ __device__ __forceinline__ double merit3(const double* e) {
     return sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
 }
 
-// KKT check, step 4 (one block): eps of both candidates, keep-best,
+// KKT check, step 3 (one block): eps of both candidates, keep-best,
 // convergence, restart decision (include/pdlp_algo.hpp:135-174), and in the
 // mixed mode the request to continue in FP64.
-__global__ __launch_bounds__(kBlock) void k_kkt_fin(DevState* __restrict__ st,
+__global__ __launch_bounds__(kBlock) void k_kkt_fin(DevState* __restrict__ S,
                                                     const double* __restrict__ part_n, int nb_n,
                                                     const double* __restrict__ part_m, int nb_m) {
+    DevState* st = S + 1;
     if (!st->need_check) return;
     double sn[8], sm[6];
     sum_partials<8>(part_n, nb_n, sn);
@@ -698,79 +797,83 @@ __global__ __launch_bounds__(kBlock) void k_kkt_fin(DevState* __restrict__ st,
     if (st->attempts >= st->max_iter) st->done = 2;
 }
 
-// KKT check, step 5 (columns): copy a new best point; on a restart measure
-// |candidate - last restart point| (scaled), move the restart point, move x
-// to the average if that is the candidate, and clear the sums.
+// KKT check, step 4 (blocks [0, en): columns; the rest: rows): copy a new
+// best point; on a restart measure |candidate - last restart point|
+// (scaled), move the restart point, and when the candidate is the average
+// move the iterate there together with its products A x_avg, A^T y_avg;
+// clear the sums.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restart_n(DevState* __restrict__ st, int n,
-                                                      IterBufs<T> B, KktBufs K,
-                                                      double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) void k_check_restart(const DevState* __restrict__ S, int n,
+                                                          int m, int en, IterBufs<T> B,
+                                                          KktBufs K, double* __restrict__ part_rn,
+                                                          double* __restrict__ part_rm) {
+    const DevState* st = S + 1;
     if (!st->need_check) return;
     const int bw = st->best_which, rs = st->restart_now, ua = st->use_avg, cur = st->cur;
     double acc[1] = {0.0};
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        if (bw) K.best_x[j] = bw == 1 ? K.xu[j] : K.xu2[j];
-        if (rs) {
-            const double cand = ua ? K.xavg[j] : (double)B.x[cur][j];
-            const double d = cand - (double)B.xlast[j];
-            acc[0] += d * d;
-            B.xlast[j] = (T)cand;
-            if (ua) B.x[cur][j] = (T)cand;
-            B.xsum[j] = T(0);
+    if ((int)blockIdx.x < en) {
+        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += en * kBlock) {
+            if (bw) K.best_x[j] = bw == 1 ? K.xu[j] : K.xu2[j];
+            if (rs) {
+                const double cand = ua ? K.xavg[j] : (double)B.x[cur][j];
+                const double d = cand - (double)B.xlast[j];
+                acc[0] += d * d;
+                B.xlast[j] = (T)cand;
+                if (ua) { B.x[cur][j] = (T)cand; B.aty[j] = (T)K.aty_avg[j]; }
+                B.xsum[j] = T(0);
+            }
         }
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) part_rn[blockIdx.x] = acc[0];
+    } else {
+        const int b = blockIdx.x - en, em = gridDim.x - en;
+        for (int i = b * kBlock + threadIdx.x; i < m; i += em * kBlock) {
+            if (bw) K.best_y[i] = bw == 1 ? K.yu[i] : K.yu2[i];
+            if (rs) {
+                const double cand = ua ? K.yavg[i] : (double)B.y[cur][i];
+                const double d = cand - (double)B.ylast[i];
+                acc[0] += d * d;
+                B.ylast[i] = (T)cand;
+                if (ua) { B.y[cur][i] = (T)cand; B.ax[cur][i] = (T)K.ax_avg[i]; }
+                B.ysum[i] = T(0);
+            }
+        }
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) part_rm[b] = acc[0];
     }
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restart_m(DevState* __restrict__ st, int m,
-                                                      IterBufs<T> B, KktBufs K,
-                                                      double* __restrict__ part) {
-    if (!st->need_check) return;
-    const int bw = st->best_which, rs = st->restart_now, ua = st->use_avg, cur = st->cur;
-    double acc[1] = {0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
-        if (bw) K.best_y[i] = bw == 1 ? K.yu[i] : K.yu2[i];
-        if (rs) {
-            const double cand = ua ? K.yavg[i] : (double)B.y[cur][i];
-            const double d = cand - (double)B.ylast[i];
-            acc[0] += d * d;
-            B.ylast[i] = (T)cand;
-            if (ua) B.y[cur][i] = (T)cand;
-            B.ysum[i] = T(0);
-        }
-    }
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
-}
-
-// KKT check, step 6 (one block): primal-weight update and restart counters;
-// resumes the iterations unless finished or switching precision.
-__global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ st,
+// KKT check, step 5 (one block): primal-weight update and restart counters;
+// resumes the iterations unless finished or switching precision; copies the
+// state back to S[0] for the next replay (also when no check ran).
+__global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ S,
                                                         const double* __restrict__ part_n, int nb_n,
                                                         const double* __restrict__ part_m, int nb_m) {
-    if (!st->need_check) return;
-    double sn[1], sm[1];
-    sum_partials<1>(part_n, nb_n, sn);
-    sum_partials<1>(part_m, nb_m, sm);
-    if (threadIdx.x != 0) return;
-    if (st->restart_now) {
-        const double ddx = sqrt(sn[0]), ddy = sqrt(sm[0]);
-        if (ddx > 1e-10 && ddy > 1e-10)
-            st->omega = exp(0.5 * log(ddy / ddx) + 0.5 * log(st->omega));
-        st->wsum = 0.0;
-        st->since_restart = 0;
-        st->restarts += 1;
-        st->merit_last_restart = st->cand_merit;
-        st->merit_prev_cand = INFINITY;
-        if (st->use_avg) { st->need_ax = 1; st->need_aty = 1; }
+    DevState* st = S + 1;
+    if (st->need_check) {
+        double sn[1], sm[1];
+        sum_partials<1>(part_n, nb_n, sn);
+        sum_partials<1>(part_m, nb_m, sm);
+        if (threadIdx.x == 0) {
+            if (st->restart_now) {
+                const double ddx = sqrt(sn[0]), ddy = sqrt(sm[0]);
+                if (ddx > 1e-10 && ddy > 1e-10)
+                    st->omega = exp(0.5 * log(ddy / ddx) + 0.5 * log(st->omega));
+                st->wsum = 0.0;
+                st->since_restart = 0;
+                st->restarts += 1;
+                st->merit_last_restart = st->cand_merit;
+                st->merit_prev_cand = INFINITY;
+                if (st->use_avg) st->need_aty = 0;   // A^T y_avg was written with y
+            }
+            st->pending = 0;
+            st->need_check = 0;
+            st->best_which = 0;
+            st->restart_now = 0;
+            if (!st->done && !st->switch_req) st->active = 1;
+        }
     }
-    st->pending = 0;
-    st->need_check = 0;
-    st->best_which = 0;
-    st->restart_now = 0;
-    if (!st->done && !st->switch_req) st->active = 1;
+    if (threadIdx.x == 0) S[0] = *st;
 }
 
 // Precision switch (mixed mode): the FP64 iterate starts at the FP32 loop's

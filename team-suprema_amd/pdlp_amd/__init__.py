@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
+    "extract_solution", "Engine", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -196,6 +196,20 @@ def _options(mode, max_iterations, check_every, tol, **kw):
     return o
 
 
+def _stop_cb(stop_flag):
+    if stop_flag is None:
+        return _lib.STOP_FN()
+    poll = stop_flag.is_set if hasattr(stop_flag, "is_set") else stop_flag
+    return _lib.STOP_FN(lambda _ctx: 1 if poll() else 0)
+
+
+def _result(info, x, y):
+    d = {k: getattr(info, k) for k, _ in _lib.SuprInfo._fields_}
+    return PdlpResult(x if info.has_point else np.zeros(0), y if info.has_point else np.zeros(0),
+                      int(info.iterations), bool(info.converged), float(info.eps_P),
+                      float(info.eps_D), float(info.eps_G), int(info.restarts), d)
+
+
 def solve_pdhg_gpu(unscaled_form, scaled_form, Dr, Dc, eta=0.99, max_iterations=200000,
                    check_every=64, tol=1e-6, stop_flag=None, mode=MODE_MIXED, **options):
     """solve_pdhg_gpu() -- include/pdlp_gpu.hpp:22-25.
@@ -213,20 +227,57 @@ def solve_pdhg_gpu(unscaled_form, scaled_form, Dr, Dc, eta=0.99, max_iterations=
     x = np.zeros(scaled_form.n)
     y = np.zeros(scaled_form.m)
     info = _lib.SuprInfo()
-    if stop_flag is None:
-        cb = _lib.STOP_FN()
-    else:
-        poll = stop_flag.is_set if hasattr(stop_flag, "is_set") else stop_flag
-        cb = _lib.STOP_FN(lambda _ctx: 1 if poll() else 0)
+    cb = _stop_cb(stop_flag)
     su, ss = unscaled_form.cstruct(), scaled_form.cstruct()
     check(lib().supr_solve_pdhg_gpu(ctypes.byref(su), ctypes.byref(ss), _dp(Dr), _dp(Dc),
                                     ctypes.byref(o), cb, None, _dp(x), _dp(y),
                                     ctypes.byref(info)))
-    d = {k: getattr(info, k) for k, _ in _lib.SuprInfo._fields_}
-    res = PdlpResult(x if info.has_point else np.zeros(0), y if info.has_point else np.zeros(0),
-                     int(info.iterations), bool(info.converged), float(info.eps_P),
-                     float(info.eps_D), float(info.eps_G), int(info.restarts), d)
-    return res
+    return _result(info, x, y)
+
+
+class Engine:
+    """A resident engine (supr_engine_*): the LP is uploaded and tuned once,
+    then solve() runs cold-start solves with the data already in HBM."""
+
+    def __init__(self, unscaled_form, scaled_form, Dr, Dc, max_iterations=200000,
+                 check_every=64, tol=1e-6, mode=MODE_MIXED, **options):
+        Dr = np.ascontiguousarray(Dr, dtype=_f64)
+        Dc = np.ascontiguousarray(Dc, dtype=_f64)
+        if Dr.shape != (scaled_form.m,) or Dc.shape != (scaled_form.n,):
+            raise ValueError("Dr must have length m and Dc length n")
+        self.m, self.n = scaled_form.m, scaled_form.n
+        o = _options(mode, max_iterations, check_every, tol, **options)
+        h = ctypes.c_void_p()
+        su, ss = unscaled_form.cstruct(), scaled_form.cstruct()
+        check(lib().supr_engine_create(ctypes.byref(su), ctypes.byref(ss), _dp(Dr), _dp(Dc),
+                                       ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def solve(self, stop_flag=None):
+        if not self._h:
+            raise ValueError("engine is closed")
+        x, y = np.zeros(self.n), np.zeros(self.m)
+        info = _lib.SuprInfo()
+        cb = _stop_cb(stop_flag)
+        check(lib().supr_engine_solve(self._h, cb, None, _dp(x), _dp(y), ctypes.byref(info)))
+        return _result(info, x, y)
+
+    def close(self):
+        if self._h:
+            lib().supr_engine_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bench_kernels(unscaled_form, scaled_form, Dr, Dc, reps=50, **options):

@@ -68,6 +68,12 @@ SIGNATURES = [
     ("supr_solve_pdhg_gpu", ctypes.c_int,
      [_LP, _LP, c_double_p, c_double_p, ctypes.POINTER(SuprOptions), STOP_FN, ctypes.c_void_p,
       c_double_p, c_double_p, ctypes.POINTER(SuprInfo)]),
+    ("supr_engine_create", ctypes.c_int,
+     [_LP, _LP, c_double_p, c_double_p, ctypes.POINTER(SuprOptions),
+      ctypes.POINTER(ctypes.c_void_p)]),
+    ("supr_engine_solve", ctypes.c_int,
+     [ctypes.c_void_p, STOP_FN, ctypes.c_void_p, c_double_p, c_double_p, ctypes.POINTER(SuprInfo)]),
+    ("supr_engine_destroy", None, [ctypes.c_void_p]),
     ("supr_bench_kernels", ctypes.c_int,
      [_LP, _LP, c_double_p, c_double_p, ctypes.POINTER(SuprOptions), ctypes.c_int32, c_double_p]),
     ("supr_spmv", ctypes.c_int,

@@ -64,3 +64,28 @@ def test_solve_rejects_mismatched_forms():
         P.solve_pdhg_gpu(lp, s, Dr, Dc, check_every=0)
     with pytest.raises(P.SupremaError):
         P.solve_pdhg_gpu(lp, s, Dr, Dc, spmv_lanes_A=3)
+
+
+REF_INCLUDE = "/root/reference/include"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_INCLUDE), reason="reference headers not present here")
+def test_adapter_compiles_against_the_reference_headers():
+    """team-suprema_amd/adapter/pdlp_gpu_hip.cpp defines the reference's
+    solve_pdhg_gpu with the reference's own RangedLP / PdlpResult types."""
+    import subprocess
+    src = os.path.join(ROOT, "team-suprema_amd", "adapter", "pdlp_gpu_hip.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-DSOVEREIGN_WITH_CUDA",
+                        "-I" + REF_INCLUDE, "-I" + os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_headers_compile_standalone():
+    import subprocess
+    for h in ("pdlp_gpu.hpp", "verify.hpp", "pdlp.hpp", "suprema_pdlp.hpp"):
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++",
+                            "-I" + os.path.join(ROOT, "include"), "-"],
+                           input=f'#include "{h}"\nint main() {{ return 0; }}\n',
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (h, r.stderr)
