@@ -40,16 +40,19 @@ def log(msg):
 
 def kernel_bytes(lp, s, ell_AT, ell_A):
     """Algorithmic bytes of one launch of each iteration kernel (every array
-    read or written once), value/vector size s.
+    read or written once), value/vector size s (4: FP32 loop, 8: FP64 loop).
       primal half (rows of A^T): A^T indices + values, row pointers unless ELL,
         y once, then per column x, c, l, u read, xsum read+write, aty and xn
-        written  -> nnz (4 + s) + [4 (n + 1)] + s m + 8 s n
-      dual half (rows of A): A indices + values, row pointers unless ELL, xn
-        once, then per row y, ax, b read, ysum read+write, axn and yn written,
-        the inequality flag -> nnz (4 + s) + [4 (m + 1)] + s n + 7 s m + m"""
+        written, and in the FP32 loop the (xn, xn - x) pair written
+        -> nnz (4 + s) + [4 (n + 1)] + s m + 8 s n  [+ 8 n]
+      dual half (rows of A): A indices + values, row pointers unless ELL, the
+        gathered vector once (xn; the (xn, xn - x) pairs in the FP32 loop),
+        then per row y, ax, b read, ysum read+write, axn and yn written, the
+        inequality flag -> nnz (4 + s) + [4 (m + 1)] + (s | 8) n + 7 s m + m"""
     n, m, nnz = lp.n, lp.m, lp.nnz
-    primal = nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + 8 * s * n
-    dual = nnz * (4 + s) + (0 if ell_A else 4 * (m + 1)) + s * n + 7 * s * m + m
+    pair = s == 4
+    primal = nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + 8 * s * n + (8 * n if pair else 0)
+    dual = nnz * (4 + s) + (0 if ell_A else 4 * (m + 1)) + (8 if pair else s) * n + 7 * s * m + m
     return primal, dual
 
 

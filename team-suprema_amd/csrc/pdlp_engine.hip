@@ -313,7 +313,7 @@ class Solver {
 
     // device-resident modes
     template <typename T> struct Iter {
-        DevBuf x[2], y[2], ax[2], aty, xsum, ysum, xlast, ylast, c, l, u, b;
+        DevBuf x[2], y[2], ax[2], aty, xsum, ysum, xlast, ylast, xd, c, l, u, b;
         IterBufs<T> view(const uint8_t* ineq) const {
             IterBufs<T> B;
             for (int k = 0; k < 2; ++k) {
@@ -326,6 +326,7 @@ class Solver {
             B.ysum = ysum.template as<T>();
             B.xlast = xlast.template as<T>();
             B.ylast = ylast.template as<T>();
+            B.xd = xd.template as<T>();
             B.c = c.template as<T>();
             B.l = l.template as<T>();
             B.u = u.template as<T>();
@@ -622,6 +623,7 @@ void Solver::alloc_iter(Iter<T>& I) {
     I.ysum = device_vec<T>(m);
     I.xlast = device_vec<T>(n);
     I.ylast = device_vec<T>(m);
+    if (sizeof(T) == 4) I.xd = device_vec<T>(2 * (size_t)n);   // (xn, xn - x) pairs
 }
 
 // Cold start: x = x0, y = 0, sums cleared, restart point = start point.

@@ -85,6 +85,7 @@ template <typename T> struct IterBufs {
     T* ysum;
     T* xlast;
     T* ylast;
+    T* xd;        // FP32 loop: interleaved pairs (xn, xn - x) for k_dual's gather
     const T* c;
     const T* l;
     const T* u;
@@ -270,26 +271,26 @@ __device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __re
     s2 = group_sum<L, double>(s2);
 }
 
-// A x1 and A (x1 - x2) in one pass over the row (one index stream, two
-// gathers): the FP32 loop takes the extrapolation A(2 xn - x) as
-// A xn + A (xn - x) and the step statistic (yn - y)'A(xn - x) from the
-// difference itself, which keeps them accurate when xn and x are close
-// (A xn - A x would cancel to rounding noise in FP32).
-template <int L, typename T>
-__device__ __forceinline__ void row_dot_diff(const MatView<T>& M, const T* __restrict__ x1,
-                                             const T* __restrict__ x2, int row, int lane,
-                                             bool valid, T& s1, T& sd) {
+// A xn and A (xn - x) in one pass over the row: one index stream and one
+// 8-byte gather of the pair (xn_j, xn_j - x_j) that k_primal stored.  The
+// FP32 loop takes the extrapolation A(2 xn - x) as A xn + A (xn - x) and the
+// step statistic (yn - y)'A(xn - x) from the difference itself, which keeps
+// them accurate when xn and x are close (A xn - A x would cancel to rounding
+// noise in FP32).
+template <int L>
+__device__ __forceinline__ void row_dot_diff(const MatView<float>& M, const float2* __restrict__ xd,
+                                             int row, int lane, bool valid, float& s1, float& sd) {
     constexpr int U = 4;
-    s1 = T(0); sd = T(0);
+    s1 = 0.f; sd = 0.f;
     if (M.ell_w > 0) {
         if (valid) {
             const size_t R = (size_t)M.rows;
             for (int k = lane; k < M.ell_w; k += L) {
                 const size_t q = (size_t)k * R + row;
-                const int c = M.idx[q];
-                const T a = M.val[q], v1 = x1[c];
-                s1 += a * v1;
-                sd += a * (v1 - x2[c]);
+                const float a = M.val[q];
+                const float2 v = xd[M.idx[q]];
+                s1 += a * v.x;
+                sd += a * v.y;
             }
         }
     } else {
@@ -298,23 +299,24 @@ __device__ __forceinline__ void row_dot_diff(const MatView<T>& M, const T* __res
         int p = b + lane;
         for (; p + (U - 1) * L < e; p += U * L) {
             int c[U];
-            T a[U], v1[U], v2[U];
+            float a[U];
+            float2 v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = M.val[p + u * L]; }
 #pragma unroll
-            for (int u = 0; u < U; ++u) { v1[u] = x1[c[u]]; v2[u] = x2[c[u]]; }
+            for (int u = 0; u < U; ++u) v[u] = xd[c[u]];
 #pragma unroll
-            for (int u = 0; u < U; ++u) { s1 += a[u] * v1[u]; sd += a[u] * (v1[u] - v2[u]); }
+            for (int u = 0; u < U; ++u) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
         }
         for (; p < e; p += L) {
-            const int c = M.idx[p];
-            const T a = M.val[p], v1 = x1[c];
-            s1 += a * v1;
-            sd += a * (v1 - x2[c]);
+            const float a = M.val[p];
+            const float2 v = xd[M.idx[p]];
+            s1 += a * v.x;
+            sd += a * v.y;
         }
     }
-    s1 = group_sum<L, T>(s1);
-    sd = group_sum<L, T>(sd);
+    s1 = group_sum<L, float>(s1);
+    sd = group_sum<L, float>(sd);
 }
 
 // Grid-stride loop over rows, L lanes per row; body(row, lane, valid) runs
@@ -518,6 +520,8 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
             if (pending) B.xsum[row] += eta_acc * xj;
             const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
             xn[row] = v;
+            if constexpr (sizeof(T) == 4)
+                reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
             const double d = (double)v - (double)xj;
             acc[0] += d * d;
         }
@@ -551,11 +555,10 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
     // FP32: A(xn - x) from the difference (see row_dot_diff); FP64: the
     // reference's 2 A xn - A x (pdlp_gpu_solve.cu:89) with A x carried.
     constexpr bool kDiff = sizeof(T) == 4;
-    const T* __restrict__ x = B.x[cur];
     for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
         T a, ad;
         if constexpr (kDiff) {
-            row_dot_diff<L, T>(A, xn, x, row, lane, valid, a, ad);
+            row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, a, ad);
         } else {
             a = row_dot<L, T>(A, xn, row, lane, valid);
             ad = T(0);
