@@ -20,9 +20,14 @@
  *   supr_extract_solution extract_solution()  include/pdlp.hpp:27-28,
  *                                             src/pdlp.cpp:84-90
  *   supr_gen_transport    tools/gen_lp.cpp:49-135 (same LP family, own PRNG)
+ *   supr_gen_blend[_sizes] no reference counterpart: seeded blending LPs (the
+ *                         refinery-blending family of tools/gen_refinery.py, LP
+ *                         only) for batched workloads
  *   supr_gen_random_lp    no reference counterpart: seeded equality-form LP
  *                         generator (6 nonzeros per column, feasible and bounded
  *                         by construction) for synthetic workloads
+ *   supr_batch_*          no reference counterpart: many independent LPs per
+ *                         launch (callers: batch drivers, request loops)
  *   supr_engine_*         no reference counterpart: a resident engine for repeated
  *                         solves of one LP (callers: benchmarks, request loops)
  *   supr_spmv             matvec() include/sparse.hpp:59-71 on the device, i.e.
@@ -212,6 +217,43 @@ int supr_gen_random_lp(int32_t m, int32_t n, int32_t col_nnz, const int32_t* col
                        uint64_t seed, double x0_hi, double u_hi, int32_t mag_decades,
                        int32_t* row_idx, int32_t* col_idx, double* val, double* c, double* rL,
                        double* rU, double* l, double* u);
+
+/* ------------------------------------------------------- batched mode -- */
+/* Many independent LPs solved together: the instances are laid out
+ * block-diagonally on the device once (supr_batch_create); every
+ * supr_batch_solve solves a subset of them (ids, or all when ids is NULL; no
+ * id twice) from the cold start.  Each launch of the device-resident loop
+ * covers every running instance, a fixed number of blocks per instance, each
+ * instance with its own step sizes, checks, restarts and (MIXED) FP32 -> FP64
+ * hand-over, so an instance's result does not depend on the other instances
+ * of the solve.  Modes: MIXED (default) or FAST_FP64.  opt->spmv_lanes_A == 0
+ * tunes the lanes-per-row rule (lanes_bias) and the per-instance block cap
+ * (max_blocks) on a short run at creation; otherwise supr_batch_configure
+ * sets them (defaults 1.0 and 32).  x_out[r] / y_out[r] (may be NULL) receive
+ * the r-th requested instance's point, infos[r] its information; *seconds
+ * (may be NULL) the solve's wall time on the device.  (No reference
+ * counterpart; callers: batch drivers and request loops.) */
+typedef struct supr_batch supr_batch;
+int supr_batch_create(int32_t count, const supr_lp* unscaled_forms, const supr_lp* scaled_forms,
+                      const double* const* Dr, const double* const* Dc,
+                      const supr_pdlp_options* opt, supr_batch** batch);
+int supr_batch_configure(supr_batch* batch, double lanes_bias, int32_t max_blocks);
+int supr_batch_config(const supr_batch* batch, double* lanes_bias, int32_t* max_blocks);
+int supr_batch_solve(supr_batch* batch, const int32_t* ids, int32_t nids, double* const* x_out,
+                     double* const* y_out, supr_pdlp_info* infos, double* seconds);
+void supr_batch_destroy(supr_batch* batch);
+
+/* Blending LP: K components blended into P products with two quality
+ * attributes.  Columns x_kp >= 0 in product-major order (col = p*K + k),
+ * n = K*P; rows: P demand equalities sum_k x_kp = d_p (d_p ~ U[50,150]),
+ * K capacity rows sum_p x_kp <= 1.1 * (use of k by a random feasible blend),
+ * 2P ranged quality rows 0.9 Q_pq d_p <= sum_k a_kq x_kp <= 1.1 Q_pq d_p with
+ * a_kq ~ U[0.5,1.5] and Q_pq the quality of that blend; costs U[10,100].
+ * m = 3P + K, nnz = 4 K P.  Feasible (the blend) and bounded (x >= 0, c > 0). */
+int supr_gen_blend(int32_t K, int32_t P, uint64_t seed, int32_t* row_idx, int32_t* col_idx,
+                   double* val, double* c, double* rL, double* rU, double* l, double* u);
+int supr_gen_blend_sizes(int32_t count, uint64_t seed, int32_t K_lo, int32_t K_hi, int32_t P_lo,
+                         int32_t P_hi, int32_t* K, int32_t* P);
 
 #ifdef __cplusplus
 }

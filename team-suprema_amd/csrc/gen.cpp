@@ -158,3 +158,75 @@ int supr_gen_random_lp(int32_t m, int32_t n, int32_t col_nnz, const int32_t* col
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Blending LP (see suprema_pdlp.h): K components, P products, two quality
+// attributes.  Columns x_{kp} in product-major order (col = p*K + k).  Rows:
+// P demand equalities, K capacity rows (<=), 2P ranged quality rows.
+// Draw order: a_{kq} (K x 2), d_p (P), blend weights w_{kp} (P x K), costs (P x K).
+int supr_gen_blend(int32_t K, int32_t P, uint64_t seed, int32_t* row_idx, int32_t* col_idx,
+                   double* val, double* c, double* rL, double* rU, double* l, double* u) {
+    if (K < 1 || P < 1 || (int64_t)K * P * 4 > INT32_MAX) {
+        supr::set_error("supr_gen_blend: need K, P >= 1 and 4*K*P < 2^31");
+        return SUPR_EINVAL;
+    }
+    Rng rng(seed);
+    const int32_t m = 3 * P + K;
+    std::vector<double> a((size_t)K * 2), d(P), w((size_t)P * K), cap(K, 0.0), qual((size_t)P * 2, 0.0);
+    for (auto& v : a) v = rng.uniform(0.5, 1.5);
+    for (auto& v : d) v = rng.uniform(50.0, 150.0);
+    for (int32_t p = 0; p < P; ++p) {
+        double s = 0.0;
+        for (int32_t k = 0; k < K; ++k) { w[(size_t)p * K + k] = rng.uniform(); s += w[(size_t)p * K + k]; }
+        for (int32_t k = 0; k < K; ++k) {
+            const double wk = w[(size_t)p * K + k] / s;          // a feasible blend of product p
+            w[(size_t)p * K + k] = wk;
+            cap[k] += wk * d[p];
+            qual[(size_t)p * 2] += a[(size_t)k * 2] * wk;
+            qual[(size_t)p * 2 + 1] += a[(size_t)k * 2 + 1] * wk;
+        }
+    }
+    const double inf = INFINITY;
+    int64_t q = 0;
+    for (int32_t p = 0; p < P; ++p) {
+        for (int32_t k = 0; k < K; ++k) {
+            const int32_t col = p * K + k;
+            c[col] = rng.uniform(10.0, 100.0);
+            l[col] = 0.0;
+            u[col] = inf;
+            row_idx[q] = p; col_idx[q] = col; val[q] = 1.0; ++q;                    // demand
+            row_idx[q] = P + k; col_idx[q] = col; val[q] = 1.0; ++q;                // capacity
+            row_idx[q] = P + K + p; col_idx[q] = col; val[q] = a[(size_t)k * 2]; ++q;          // quality 0
+            row_idx[q] = P + K + P + p; col_idx[q] = col; val[q] = a[(size_t)k * 2 + 1]; ++q;  // quality 1
+        }
+    }
+    for (int32_t p = 0; p < P; ++p) { rL[p] = rU[p] = d[p]; }
+    for (int32_t k = 0; k < K; ++k) { rL[P + k] = -inf; rU[P + k] = 1.1 * cap[k]; }
+    for (int32_t p = 0; p < P; ++p)
+        for (int32_t t = 0; t < 2; ++t) {
+            const int32_t r = P + K + t * P + p;
+            rL[r] = 0.9 * qual[(size_t)p * 2 + t] * d[p];
+            rU[r] = 1.1 * qual[(size_t)p * 2 + t] * d[p];
+        }
+    (void)m;
+    return SUPR_OK;
+}
+
+// Per-instance sizes of a blending batch: K log-uniform in [K_lo, K_hi], P
+// log-uniform in [P_lo, P_hi] (rounded), from this file's PRNG.
+int supr_gen_blend_sizes(int32_t count, uint64_t seed, int32_t K_lo, int32_t K_hi, int32_t P_lo,
+                         int32_t P_hi, int32_t* K, int32_t* P) {
+    if (count < 0 || K_lo < 1 || K_hi < K_lo || P_lo < 1 || P_hi < P_lo) {
+        supr::set_error("supr_gen_blend_sizes: bad ranges");
+        return SUPR_EINVAL;
+    }
+    Rng rng(seed);
+    for (int32_t i = 0; i < count; ++i) {
+        K[i] = (int32_t)std::lround(std::exp(rng.uniform(std::log((double)K_lo), std::log((double)K_hi))));
+        P[i] = (int32_t)std::lround(std::exp(rng.uniform(std::log((double)P_lo), std::log((double)P_hi))));
+    }
+    return SUPR_OK;
+}
+
+}  // extern "C"

@@ -888,20 +888,7 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
             hipLaunchKernelGGL(k_switch_n, dim3(grid_elems(n_)), dim3(kBlock), 0, s, state_.as<DevState>(), n_, Fv, Dv, K_);
             hipLaunchKernelGGL(k_switch_m, dim3(grid_elems(m_)), dim3(kBlock), 0, s, state_.as<DevState>(), m_, Fv, Dv, K_);
             SUPR_HIP(hipGetLastError());
-            hs.phase = 1;
-            hs.cur = 0;
-            hs.switch_req = 0;
-            hs.wsum = 0.0;
-            hs.since_restart = 0;
-            hs.restarts += 1;
-            hs.merit_last_restart = hs.cand_merit;
-            hs.merit_prev_cand = INFINITY;
-            hs.pending = 0;
-            hs.need_check = 0;
-            hs.need_aty = 0;
-            hs.decide_pending = 0;
-            hs.active = 1;
-            hs.stall = 0;
+            switch_to_fp64(hs);
             const DevState both[2] = {hs, hs};
             SUPR_HIP(hipMemcpyAsync(state_.as<void>(), both, sizeof(both), hipMemcpyHostToDevice, s));
             use_lanes(1);

@@ -1,345 +1,10 @@
-// pdlp_kernels.hpp -- CDNA4 (gfx950, wave64) kernels of the restarted-PDHG
-// engine.  Included by pdlp_engine.hip only.
-//
-// Two families:
-//  * plain kernels (k_spmv, k_plain_*): one operation per launch, the shape of
-//    the reference GPU backend (gpu/pdlp_gpu_solve.cu:42-138) -- used by the
-//    host-driven plain FP64 mode and by the SpMV autotuner;
-//  * device-resident kernels (k_primal, k_dual, k_check_*, k_*_fin): the
-//    whole PDHG loop of include/pdlp_algo.hpp:103-175 with its step-size rule,
-//    ergodic averages, KKT check and restart rule evaluated on the device.
-//    Control flow lives in a DevState block in device memory; each kernel
-//    reads it at entry and returns at once when it has nothing to do, so a
-//    fixed sequence of launches (a replayed hipGraph) can run a variable
-//    number of accepted / rejected steps and checks without a host round trip.
-//
-// SpMV schedules.  A row is handled by a group of L lanes: L = 1, 2, ..., 64
-// lanes of one wavefront (lanes stride the row's nonzeros, an xor-shuffle
-// tree combines them), or L = 256, the whole block (the LDS-staged row split
-// for long rows: four wavefront sums combined through LDS).  Matrices whose rows all have the same short
-// length (A^T of a transportation LP: 2 per row) are stored as ELL instead
-// (column-major slots, coalesced across the wavefront, no row pointers).  The
-// engine picks L per matrix by timing the candidates (autotune) or from the
-// mean row length.
-//
-// Reductions are deterministic: each block writes its partial sums and a
-// one-block kernel adds them in a fixed order.  The hot kernels carry no
-// atomics and no fences: kernel boundaries order everything.
+// pdlp_kernels.hpp -- the single-LP engine's kernels (plain and device-
+// resident); included by pdlp_engine.hip only.  Helpers, layouts and the
+// description of the SpMV schedules are in pdlp_device.hpp.
 #pragma once
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdint>
+#include "pdlp_device.hpp"
 
 namespace supr {
-
-constexpr int kBlock = 256;       // 4 wavefronts
-constexpr int kWaves = kBlock / 64;
-constexpr int kMaxBlocks = 2048;  // grid-stride cap: 8 blocks per CU on 256 CUs
-
-// std::max / std::min semantics (ties and signed zeros as libstdc++).
-template <typename T> __host__ __device__ __forceinline__ T smax(T a, T b) { return a < b ? b : a; }
-template <typename T> __host__ __device__ __forceinline__ T smin(T a, T b) { return b < a ? b : a; }
-
-// Control block of the device-resident loop (one per LP).
-struct DevState {
-    // step size, primal weight, averaging
-    double eta, omega, wsum, eta_acc;
-    // restart bookkeeping
-    double merit_last_restart, merit_prev_cand, cand_merit;
-    double best_merit, stall_ref;
-    double best_eps[3];
-    double last_eps[3];      // eps of the candidate chosen at the last check
-    double last_obj;
-    double tol, switch_tol;
-    double denom_p, denom_d, obj_offset;
-    int32_t attempts, accepted, since_restart, restarts, checks;
-    int32_t max_iter, check_every, stall_checks, stall;
-    int32_t cur;             // which half of each double buffer is current
-    int32_t active;          // iterations run (0: paused for a check, or finished)
-    int32_t need_aty;        // next k_primal recomputes A^T y
-    int32_t decide_pending;  // the last attempt's step decision is not applied yet
-    int32_t pending;         // last accepted step not yet added to the averages
-    int32_t need_check;
-    int32_t done;            // 1 converged, 2 iteration limit
-    int32_t has_best, best_which, use_avg, restart_now;
-    int32_t switch_req, phase;   // switch_req: reason code (1 merit, 2 stall near floor, 3 long stall); phase 0: FP32 iterates, 1: FP64
-    int32_t pad0, pad1;
-};
-
-template <typename V> struct MatView {
-    int32_t rows;
-    int32_t ell_w;           // > 0: ELL with this width (idx / val column-major)
-    const int32_t* ptr;      // CSR row pointers (unused for ELL)
-    const int32_t* idx;
-    const V* val;
-};
-
-// Iterate buffers of one precision.
-template <typename T> struct IterBufs {
-    T* x[2];
-    T* y[2];
-    T* ax[2];
-    T* aty;
-    T* xsum;
-    T* ysum;
-    T* xlast;
-    T* ylast;
-    T* xd;        // FP32 loop: interleaved pairs (xn, xn - x) for k_dual's gather
-    const T* c;
-    const T* l;
-    const T* u;
-    const T* b;
-    const uint8_t* ineq;
-};
-
-// Vectors of the FP64 KKT check (unscaled PDLP form).
-struct KktBufs {
-    const double* c;   // unscaled c, l, u (n) and rL, rU (m)
-    const double* l;
-    const double* u;
-    const double* rL;
-    const double* rU;
-    const double* Dr;
-    const double* Dc;
-    double* xu;        // Dc * x, Dc * x_avg
-    double* xu2;
-    double* xavg;      // scaled average
-    double* yu;
-    double* yu2;
-    double* yavg;
-    double* best_x;
-    double* best_y;
-    double* aty_avg;   // scaled A^T y_avg (n), A x_avg (m) from the check
-    double* ax_avg;
-};
-
-// ------------------------------------------------------------- helpers --
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// Sums N values over the block; result valid in thread 0.
-template <int N> __device__ __forceinline__ void block_sum(double (&v)[N]) {
-    __shared__ double lds[N][kWaves];
-#pragma unroll
-    for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int q = 0; q < N; ++q) lds[q][w] = v[q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < N; ++q) {
-            double s = lds[q][0];
-            for (int k = 1; k < kWaves; ++k) s += lds[q][k];
-            v[q] = s;
-        }
-    }
-    __syncthreads();
-}
-
-// Deterministic sum of N-wide partials of nb <= kMaxBlocks blocks by one
-// block (a separate launch, so the producers' stores are visible): thread t
-// loads blocks t, t+256, ... with independent loads, adds them in that order,
-// then a fixed tree.  Valid in thread 0.
-template <int N>
-__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb,
-                                             double (&out)[N]) {
-    constexpr int K = kMaxBlocks / kBlock;
-    double v[K][N];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int b = threadIdx.x + k * kBlock;
-#pragma unroll
-        for (int q = 0; q < N; ++q) v[k][q] = b < nb ? part[(size_t)b * N + q] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < N; ++q) {
-        out[q] = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) out[q] += v[k][q];
-    }
-    block_sum<N>(out);
-}
-
-// Sum over a group of L lanes (L <= 64: xor shuffles inside the wavefront;
-// L = 256: the whole block, wavefront sums combined through LDS).  Every lane
-// of the group gets the result.  Called with block-uniform control flow.
-template <int L, typename Acc> __device__ __forceinline__ Acc group_sum(Acc s) {
-    if constexpr (L <= 64) {
-#pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
-        return s;
-    } else {
-        static_assert(L == kBlock, "block-wide groups span the whole block");
-        __shared__ Acc red[kWaves];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        Acc t = red[0];
-#pragma unroll
-        for (int k = 1; k < kWaves; ++k) t += red[k];
-        __syncthreads();
-        return t;
-    }
-}
-
-// Row dot product with a group of L lanes; every lane of the group gets the
-// sum.  valid = false gives 0 but still takes part in the group reduction.
-// The CSR loop issues U independent index / value / gather loads before the
-// multiply-adds, so a lane has U gathers in flight instead of one.
-template <int L, typename Acc, typename V, typename X>
-__device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict__ x, int row,
-                                       int lane, bool valid) {
-    constexpr int U = 4;
-    Acc s = Acc(0);
-    if (M.ell_w > 0) {
-        if (valid) {
-            const size_t R = (size_t)M.rows;
-            for (int k = lane; k < M.ell_w; k += L) {
-                const size_t q = (size_t)k * R + row;
-                s += (Acc)M.val[q] * (Acc)x[M.idx[q]];
-            }
-        }
-    } else {
-        int b = 0, e = 0;
-        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
-        int p = b + lane;
-        for (; p + (U - 1) * L < e; p += U * L) {
-            int c[U];
-            V v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
-            X xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
-#pragma unroll
-            for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
-        }
-        for (; p < e; p += L) s += (Acc)M.val[p] * (Acc)x[M.idx[p]];
-    }
-    return group_sum<L, Acc>(s);
-}
-
-// Two right-hand sides sharing the index stream (FP64, KKT check).
-template <int L, typename V>
-__device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __restrict__ x1,
-                                         const double* __restrict__ x2, int row, int lane,
-                                         bool valid, double& s1, double& s2) {
-    constexpr int U = 4;
-    s1 = 0.0; s2 = 0.0;
-    if (M.ell_w > 0) {
-        if (valid) {
-            const size_t R = (size_t)M.rows;
-            for (int k = lane; k < M.ell_w; k += L) {
-                const size_t q = (size_t)k * R + row;
-                const int c = M.idx[q];
-                const double a = (double)M.val[q];
-                s1 += a * x1[c];
-                s2 += a * x2[c];
-            }
-        }
-    } else {
-        int b = 0, e = 0;
-        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
-        int p = b + lane;
-        for (; p + (U - 1) * L < e; p += U * L) {
-            int c[U];
-            double a[U], u1[U], u2[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = (double)M.val[p + u * L]; }
-#pragma unroll
-            for (int u = 0; u < U; ++u) { u1[u] = x1[c[u]]; u2[u] = x2[c[u]]; }
-#pragma unroll
-            for (int u = 0; u < U; ++u) { s1 += a[u] * u1[u]; s2 += a[u] * u2[u]; }
-        }
-        for (; p < e; p += L) {
-            const int c = M.idx[p];
-            const double a = (double)M.val[p];
-            s1 += a * x1[c];
-            s2 += a * x2[c];
-        }
-    }
-    s1 = group_sum<L, double>(s1);
-    s2 = group_sum<L, double>(s2);
-}
-
-// A xn and A (xn - x) in one pass over the row: one index stream and one
-// 8-byte gather of the pair (xn_j, xn_j - x_j) that k_primal stored.  The
-// FP32 loop takes the extrapolation A(2 xn - x) as A xn + A (xn - x) and the
-// step statistic (yn - y)'A(xn - x) from the difference itself, which keeps
-// them accurate when xn and x are close (A xn - A x would cancel to rounding
-// noise in FP32).
-template <int L>
-__device__ __forceinline__ void row_dot_diff(const MatView<float>& M, const float2* __restrict__ xd,
-                                             int row, int lane, bool valid, float& s1, float& sd) {
-    constexpr int U = 4;
-    s1 = 0.f; sd = 0.f;
-    if (M.ell_w > 0) {
-        if (valid) {
-            const size_t R = (size_t)M.rows;
-            for (int k = lane; k < M.ell_w; k += L) {
-                const size_t q = (size_t)k * R + row;
-                const float a = M.val[q];
-                const float2 v = xd[M.idx[q]];
-                s1 += a * v.x;
-                sd += a * v.y;
-            }
-        }
-    } else {
-        int b = 0, e = 0;
-        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
-        int p = b + lane;
-        for (; p + (U - 1) * L < e; p += U * L) {
-            int c[U];
-            float a[U];
-            float2 v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = M.val[p + u * L]; }
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = xd[c[u]];
-#pragma unroll
-            for (int u = 0; u < U; ++u) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
-        }
-        for (; p < e; p += L) {
-            const float a = M.val[p];
-            const float2 v = xd[M.idx[p]];
-            s1 += a * v.x;
-            sd += a * v.y;
-        }
-    }
-    s1 = group_sum<L, float>(s1);
-    sd = group_sum<L, float>(sd);
-}
-
-// Grid-stride loop over rows, L lanes per row; body(row, lane, valid) runs
-// with block-uniform trip counts so group reductions always see full groups.
-// XCD-aware mapping: the hardware deals blocks round-robin to the 8 XCDs
-// (block b -> XCD b % 8); when the grid is a multiple of 8, block b takes
-// the logical slot (b % 8) * (G / 8) + b / 8, so each XCD sweeps one
-// contiguous range of rows and the x entries those rows gather stay in that
-// XCD's L2.
-constexpr int kXcds = 8;
-template <int L, typename F>
-__device__ __forceinline__ void for_rows(int rows, F&& body) {
-    const int gpb = kBlock / L;
-    const int g = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
-    const int G = gridDim.x;
-    const int lb = (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
-                                    : blockIdx.x;
-    for (int r0 = lb * gpb; r0 < rows; r0 += G * gpb) {
-        const int row = r0 + g;
-        body(row, lane, row < rows);
-    }
-}
 
 // ------------------------------------------------------- plain kernels --
 
@@ -433,59 +98,6 @@ __global__ __launch_bounds__(kBlock) void k_plain_axpy(int len, double a,
 // S[(k + 1) & 1]; k_dual reads S[(k + 1) & 1]; the check group works on S[1]
 // and k_restart_fin copies S[1] back to S[0] for the next replay.
 
-// The step-size rule of include/pdlp_algo.hpp:113-131.
-__device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2,
-                                            double inter) {
-    st->attempts += 1;
-    st->pending = 0;
-    const double eta = st->eta, omega = st->omega;
-    const double num = omega * dx2 + dy2 / omega;
-    const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : INFINITY;
-    const double k1 = st->accepted + 2.0;
-    const double grow = (1.0 + pow(k1, -0.6)) * eta;
-    const double eta_next = isfinite(eta_bar) ? smin((1.0 - pow(k1, -0.3)) * eta_bar, grow) : grow;
-    if (eta <= eta_bar) {
-        st->cur ^= 1;
-        st->pending = 1;
-        st->eta_acc = eta;
-        st->wsum += eta;
-        st->accepted += 1;
-        st->since_restart += 1;
-        st->need_aty = 1;
-        if (st->accepted % st->check_every == 0) st->need_check = 1;
-    } else {
-        st->need_aty = 0;
-    }
-    st->eta = eta_next;
-    if (st->attempts >= st->max_iter) st->need_check = 1;
-    if (st->need_check) st->active = 0;
-}
-
-// Loads the state S[p] into LDS, applies the pending step decision (if the
-// previous launch ran an attempt), and publishes the result to S[p ^ 1]
-// (block 0).  Returns the block's copy.
-__device__ __forceinline__ void load_and_decide(DevState* __restrict__ S, int p,
-                                                const double* __restrict__ part_m, int nb_m,
-                                                const double* __restrict__ dx2_slot,
-                                                DevState& Q) {
-    if (threadIdx.x == 0) Q = S[p];
-    __syncthreads();
-    if (Q.decide_pending) {
-        double sm[2];
-        sum_partials<2>(part_m, nb_m, sm);
-        if (threadIdx.x == 0) {
-            decide_step(&Q, *dx2_slot, sm[0], sm[1]);
-            Q.decide_pending = 0;
-        }
-        __syncthreads();
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        DevState out = Q;
-        out.decide_pending = 0;
-        S[p ^ 1] = out;
-    }
-}
-
 // Primal half of one attempt, over the columns (rows of A^T):
 //   [after an accept] aty = A^T y, xsum += eta_acc x  (pending average term)
 //   xn = clip(x - tau (c - aty), l, u);  partial sums of (xn - x)^2 in FP64.
@@ -496,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
                                                    const double* __restrict__ dx2_slot,
                                                    double* __restrict__ part_n) {
     __shared__ DevState Q;
-    load_and_decide(S, p, part_m, nb_m, dx2_slot, Q);
+    load_and_decide(S, p, part_m, nb_m, dx2_slot, Q, blockIdx.x == 0);
     if (!Q.active) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) S[p ^ 1].decide_pending = 1;   // k_dual will run
     const int cur = Q.cur;
@@ -598,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_check_prep(DevState* __restrict__ S,
                                                        const double* __restrict__ part_m, int nb_m,
                                                        const double* __restrict__ dx2_slot) {
     __shared__ DevState Q;
-    load_and_decide(S, 0, part_m, nb_m, dx2_slot, Q);
+    load_and_decide(S, 0, part_m, nb_m, dx2_slot, Q, blockIdx.x == 0);
     if (!Q.need_check) return;
     const int cur = Q.cur;
     const bool pending = Q.pending != 0;
@@ -724,10 +336,6 @@ __global__ __launch_bounds__(kBlock) void k_check_kkt(const DevState* __restrict
     }
 }
 
-__device__ __forceinline__ double merit3(const double* e) {
-    return sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-}
-
 // KKT check, step 3 (one block): eps of both candidates, keep-best,
 // convergence, restart decision (include/pdlp_algo.hpp:135-174), and in the
 // mixed mode the request to continue in FP64.
@@ -739,65 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_kkt_fin(DevState* __restrict__ S,
     double sn[8], sm[6];
     sum_partials<8>(part_n, nb_n, sn);
     sum_partials<6>(part_m, nb_m, sm);
-    if (threadIdx.x != 0) return;
-    double eps[2][3], obj[2];
-    for (int k = 0; k < 2; ++k) {
-        const double bv2 = sn[4 * k + 0], svc = sn[4 * k + 1], dcol = sn[4 * k + 2], cx = sn[4 * k + 3];
-        const double rv2 = sm[3 * k + 0], svr = sm[3 * k + 1], drow = sm[3 * k + 2];
-        const double D = drow + dcol;
-        eps[k][0] = sqrt(rv2 + bv2) / st->denom_p;
-        eps[k][1] = sqrt(svr + svc) / st->denom_d;
-        eps[k][2] = isfinite(D) ? fabs(cx - D) / (1.0 + fabs(cx) + fabs(D)) : INFINITY;
-        obj[k] = cx + st->obj_offset;
-    }
-    st->checks += 1;
-    const double mc_cur = merit3(eps[0]), mc_avg = merit3(eps[1]);
-    int bw = 0;
-    if (mc_cur < st->best_merit) { st->best_merit = mc_cur; bw = 1; }
-    if (mc_avg < st->best_merit) { st->best_merit = mc_avg; bw = 2; }
-    if (bw) {
-        for (int q = 0; q < 3; ++q) st->best_eps[q] = eps[bw - 1][q];
-        st->has_best = 1;
-    }
-    const int use_avg = mc_avg < mc_cur;
-    const double* rep = eps[use_avg];
-    const double mc = use_avg ? mc_avg : mc_cur;
-    for (int q = 0; q < 3; ++q) st->last_eps[q] = rep[q];
-    st->last_obj = obj[use_avg];
-    st->use_avg = use_avg;
-    st->cand_merit = mc;
-    st->restart_now = 0;
-    if (smax(smax(rep[0], rep[1]), rep[2]) <= st->tol) {
-        // converged: the returned point is this candidate
-        st->done = 1;
-        st->best_which = use_avg ? 2 : 1;
-        for (int q = 0; q < 3; ++q) st->best_eps[q] = rep[q];
-        st->has_best = 1;
-        return;
-    }
-    st->best_which = bw;
-    if (st->best_merit < 0.9 * st->stall_ref) { st->stall_ref = st->best_merit; st->stall = 0; }
-    else st->stall += 1;
-    const bool sufficient = mc <= 0.2 * st->merit_last_restart;
-    const bool necessary = mc <= 0.8 * st->merit_last_restart && mc > st->merit_prev_cand;
-    const bool artificial = st->since_restart >= 0.36 * st->accepted;
-    st->merit_prev_cand = mc;
-    if (st->phase == 0 && st->attempts < st->max_iter) {
-        // FP32 -> FP64: the merit reached switch_tol, or the best merit stopped
-        // improving near the FP32 floor (within 100x switch_tol), or it has
-        // stopped improving for four times the stall budget anywhere.
-        int why = 0;
-        if (mc <= st->switch_tol) why = 1;
-        else if (st->stall_checks > 0 && st->stall >= st->stall_checks &&
-                 st->best_merit <= 100.0 * st->switch_tol) why = 2;
-        else if (st->stall_checks > 0 && st->stall >= 4 * st->stall_checks) why = 3;
-        if (why) {
-            st->switch_req = why;   // the host restarts this candidate in FP64
-            return;
-        }
-    }
-    st->restart_now = (sufficient || necessary || artificial) ? 1 : 0;
-    if (st->attempts >= st->max_iter) st->done = 2;
+    if (threadIdx.x == 0) kkt_fin_update(st, sn, sm);
 }
 
 // KKT check, step 4 (blocks [0, en): columns; the rest: rows): copy a new
@@ -857,24 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ S
         double sn[1], sm[1];
         sum_partials<1>(part_n, nb_n, sn);
         sum_partials<1>(part_m, nb_m, sm);
-        if (threadIdx.x == 0) {
-            if (st->restart_now) {
-                const double ddx = sqrt(sn[0]), ddy = sqrt(sm[0]);
-                if (ddx > 1e-10 && ddy > 1e-10)
-                    st->omega = exp(0.5 * log(ddy / ddx) + 0.5 * log(st->omega));
-                st->wsum = 0.0;
-                st->since_restart = 0;
-                st->restarts += 1;
-                st->merit_last_restart = st->cand_merit;
-                st->merit_prev_cand = INFINITY;
-                if (st->use_avg) st->need_aty = 0;   // A^T y_avg was written with y
-            }
-            st->pending = 0;
-            st->need_check = 0;
-            st->best_which = 0;
-            st->restart_now = 0;
-            if (!st->done && !st->switch_req) st->active = 1;
-        }
+        if (threadIdx.x == 0) restart_fin_update(st, sn[0], sm[0]);
     }
     if (threadIdx.x == 0) S[0] = *st;
 }

@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "Engine", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
+    "extract_solution", "Engine", "BatchEngine", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -278,6 +278,111 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+class BatchEngine:
+    """Many independent LPs solved together (supr_batch_*).
+
+    forms: list of (unscaled_form, scaled_form, Dr, Dc).  solve(ids=None)
+    solves the listed instances (all by default) from the cold start and
+    returns (list of PdlpResult, seconds).  tune=True times the lanes rule
+    and the per-instance block cap on a short run at creation; otherwise
+    lanes_bias and max_blocks are used."""
+
+    def __init__(self, forms, max_iterations=200000, check_every=64, tol=1e-6, mode=MODE_MIXED,
+                 tune=True, lanes_bias=1.0, max_blocks=32, **options):
+        self._keep = []
+        us, ss, drs, dcs = [], [], [], []
+        self.shapes = []
+        for U, S, Dr, Dc in forms:
+            Dr = np.ascontiguousarray(Dr, dtype=_f64)
+            Dc = np.ascontiguousarray(Dc, dtype=_f64)
+            if Dr.shape != (S.m,) or Dc.shape != (S.n,):
+                raise ValueError("Dr must have length m and Dc length n")
+            self._keep += [U, S, Dr, Dc]
+            us.append(U.cstruct())
+            ss.append(S.cstruct())
+            drs.append(_dp(Dr))
+            dcs.append(_dp(Dc))
+            self.shapes.append((S.m, S.n))
+        cnt = len(us)
+        if cnt == 0:
+            raise ValueError("empty batch")
+        self._U = (_lib.SuprLp * cnt)(*us)
+        self._S = (_lib.SuprLp * cnt)(*ss)
+        self._Dr = (_lib.c_double_p * cnt)(*drs)
+        self._Dc = (_lib.c_double_p * cnt)(*dcs)
+        o = _options(mode, max_iterations, check_every, tol, **options)
+        o.spmv_lanes_A = 0 if tune else 1
+        h = ctypes.c_void_p()
+        check(lib().supr_batch_create(cnt, self._U, self._S, self._Dr, self._Dc, ctypes.byref(o),
+                                      ctypes.byref(h)))
+        self._h = h
+        if not tune:
+            self.configure(lanes_bias, max_blocks)
+
+    @property
+    def config(self):
+        """(lanes_bias, max_blocks) in use."""
+        bias, cap = ctypes.c_double(), ctypes.c_int32()
+        check(lib().supr_batch_config(self._h, ctypes.byref(bias), ctypes.byref(cap)))
+        return float(bias.value), int(cap.value)
+
+    def configure(self, lanes_bias, max_blocks):
+        check(lib().supr_batch_configure(self._h, float(lanes_bias), int(max_blocks)))
+
+    def solve(self, ids=None):
+        if ids is None:
+            ids = list(range(len(self.shapes)))
+        ids_a = np.ascontiguousarray(ids, dtype=_i32)
+        k = len(ids_a)
+        if k == 0 or ids_a.min() < 0 or ids_a.max() >= len(self.shapes):
+            raise ValueError("instance ids must be in [0, count) and not empty")
+        xs = [np.zeros(self.shapes[i][1]) for i in ids_a]
+        ys = [np.zeros(self.shapes[i][0]) for i in ids_a]
+        xp = (_lib.c_double_p * k)(*[_dp(a) for a in xs])
+        yp = (_lib.c_double_p * k)(*[_dp(a) for a in ys])
+        infos = (_lib.SuprInfo * k)()
+        secs = ctypes.c_double()
+        check(lib().supr_batch_solve(self._h, _ip(ids_a), k, xp, yp, infos, ctypes.byref(secs)))
+        return [_result(infos[r], xs[r], ys[r]) for r in range(k)], float(secs.value)
+
+    def close(self):
+        if self._h:
+            lib().supr_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gen_blend(K, P, seed=1):
+    """Blending LP (K components, P products, 2 quality attributes); a ranged
+    LP: pass it through build_pdlp_form before solving.  See suprema_pdlp.h."""
+    n, m, nnz = K * P, 3 * P + K, 4 * K * P
+    ri, ci = np.zeros(nnz, dtype=_i32), np.zeros(nnz, dtype=_i32)
+    v = np.zeros(nnz)
+    c, rL, rU, l, u = np.zeros(n), np.zeros(m), np.zeros(m), np.zeros(n), np.zeros(n)
+    check(lib().supr_gen_blend(int(K), int(P), int(seed), _ip(ri), _ip(ci), _dp(v), _dp(c),
+                               _dp(rL), _dp(rU), _dp(l), _dp(u)))
+    return LP(m, n, ri, ci, v, c, rL, rU, l, u, 0.0, f"blend_{K}x{P}")
+
+
+def gen_blend_sizes(count, seed=2026, K=(20, 400), P=(10, 200)):
+    """Per-instance (K, P), each log-uniform in its range."""
+    Ks, Ps = np.zeros(count, dtype=_i32), np.zeros(count, dtype=_i32)
+    check(lib().supr_gen_blend_sizes(int(count), int(seed), int(K[0]), int(K[1]), int(P[0]),
+                                     int(P[1]), _ip(Ks), _ip(Ps)))
+    return list(zip(Ks.tolist(), Ps.tolist()))
 
 
 def bench_kernels(unscaled_form, scaled_form, Dr, Dc, reps=50, **options):

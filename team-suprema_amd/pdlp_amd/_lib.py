@@ -90,6 +90,21 @@ SIGNATURES = [
     ("supr_gen_transport", ctypes.c_int,
      [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, c_int32_p, c_int32_p]
      + [c_double_p] * 6),
+    ("supr_batch_create", ctypes.c_int,
+     [ctypes.c_int32, _LP, _LP, ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p),
+      ctypes.POINTER(SuprOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    ("supr_batch_configure", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
+    ("supr_batch_config", ctypes.c_int,
+     [ctypes.c_void_p, c_double_p, c_int32_p]),
+    ("supr_batch_solve", ctypes.c_int,
+     [ctypes.c_void_p, c_int32_p, ctypes.c_int32, ctypes.POINTER(c_double_p),
+      ctypes.POINTER(c_double_p), ctypes.POINTER(SuprInfo), c_double_p]),
+    ("supr_batch_destroy", None, [ctypes.c_void_p]),
+    ("supr_gen_blend", ctypes.c_int,
+     [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_int32_p, c_int32_p] + [c_double_p] * 6),
+    ("supr_gen_blend_sizes", ctypes.c_int,
+     [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+      ctypes.c_int32, c_int32_p, c_int32_p]),
     ("supr_gen_random_lp", ctypes.c_int,
      [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_int32_p, ctypes.c_uint64,
       ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_int32_p, c_int32_p] + [c_double_p] * 6),

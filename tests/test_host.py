@@ -110,3 +110,24 @@ def test_bad_arguments_raise():
         P.gen_transport(1, 5)
     with pytest.raises(ValueError):
         P.verify(lp, np.zeros(lp.n + 1), np.zeros(lp.m))
+
+
+def test_blend_generator_structure(oracle):
+    K, Pp = 7, 5
+    lp = P.gen_blend(K, Pp, seed=9)
+    assert (lp.m, lp.n, lp.nnz) == (3 * Pp + K, K * Pp, 4 * K * Pp)
+    # demand rows are equalities, capacity rows are <=, quality rows are ranged
+    assert np.array_equal(lp.rL[:Pp], lp.rU[:Pp])
+    assert np.all(np.isinf(lp.rL[Pp:Pp + K])) and np.all(np.isfinite(lp.rU[Pp:Pp + K]))
+    q = slice(Pp + K, None)
+    assert np.all(lp.rL[q] < lp.rU[q]) and np.allclose(lp.rU[q] / lp.rL[q], 1.1 / 0.9)
+    assert np.all((lp.c >= 10) & (lp.c <= 100))
+    # the PDLP form adds one slack column per ranged row
+    form, meta = P.build_pdlp_form(lp)
+    assert meta.n_slack == 2 * Pp and form.n == lp.n + 2 * Pp
+    # feasible and bounded: the oracle solves it
+    s = oracle.scale(form)
+    out = oracle.pdlp(form, form.with_values(*s[:6]), s[6], s[7], 200000, 64, 1e-6)
+    assert out.converged
+    sizes = P.gen_blend_sizes(512)
+    assert len(sizes) == 512 and all(20 <= k <= 400 and 10 <= p <= 200 for k, p in sizes)
