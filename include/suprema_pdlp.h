@@ -28,6 +28,9 @@
  *                         by construction) for synthetic workloads
  *   supr_batch_*          no reference counterpart: many independent LPs per
  *                         launch (callers: batch drivers, request loops)
+ *   supr_server_*         no reference counterpart: continuous batching of LP
+ *                         requests with admission by device footprint (callers:
+ *                         request loops)
  *   supr_engine_*         no reference counterpart: a resident engine for repeated
  *                         solves of one LP (callers: benchmarks, request loops)
  *   supr_spmv             matvec() include/sparse.hpp:59-71 on the device, i.e.
@@ -242,6 +245,46 @@ int supr_batch_config(const supr_batch* batch, double* lanes_bias, int32_t* max_
 int supr_batch_solve(supr_batch* batch, const int32_t* ids, int32_t nids, double* const* x_out,
                      double* const* y_out, supr_pdlp_info* infos, double* seconds);
 void supr_batch_destroy(supr_batch* batch);
+
+/* ------------------------------------------------------- serving mode -- */
+/* Continuous batching: requests are submitted at any time (thread-safe; the
+ * host-side preparation runs on the caller's thread) and wait in a FIFO
+ * queue; every supr_server_step retires finished instances, admits queued
+ * requests whose column, row and nonzero ranges fit the device arenas
+ * (capacities fixed at creation: admission by HBM footprint), starts them
+ * cold, and runs replays_per_step graph replays of the device-resident loop
+ * over all running instances.  An instance's block counts and lanes are fixed
+ * at admission, so its result is the same alone and in any mix.
+ * supr_server_result returns a finished request's point (unscaled PDLP-form
+ * coordinates) and information once (*ready = 1; then the result is
+ * released); with x_out, y_out and info all NULL it only reports readiness
+ * and the sizes m, n.  info->seconds_total is the time from submission to
+ * retirement, seconds_solve from admission to retirement.  A request larger
+ * than the capacities is refused at submission (SUPR_EINVAL).
+ * supr_server_request_bytes gives the device bytes a request occupies while
+ * resident; supr_server_device_bytes the bytes the server holds.  (No
+ * reference counterpart; callers: request loops.) */
+typedef struct supr_server_options {
+    supr_pdlp_options pdlp;    /* mode MIXED or FAST_FP64, tol, max_iterations, check_every, graph_attempts, device, verbose */
+    int64_t max_cols;          /* column arena: sum over resident requests of n + 1 */
+    int64_t max_rows;          /* row arena: sum of m + 1 */
+    int64_t max_nnz;           /* nonzero arena: sum of nnz */
+    int32_t max_instances;     /* resident requests */
+    int32_t max_blocks;        /* blocks per request and side of a launch (0: 256) */
+    double lanes_bias;         /* lanes-per-row rule (0: 1.0) */
+    int32_t replays_per_step;  /* graph replays per supr_server_step (0: 1) */
+} supr_server_options;
+typedef struct supr_server supr_server;
+void supr_server_default_options(supr_server_options* opt);
+int64_t supr_server_request_bytes(int32_t m, int32_t n, int32_t nnz, int32_t mode);
+int supr_server_create(const supr_server_options* opt, supr_server** server);
+int supr_server_device_bytes(const supr_server* server, int64_t* bytes);
+int supr_server_submit(supr_server* server, const supr_lp* unscaled_form, const supr_lp* scaled_form,
+                       const double* Dr, const double* Dc, int64_t* ticket);
+int supr_server_step(supr_server* server, int32_t* running, int32_t* queued, int32_t* finished);
+int supr_server_result(supr_server* server, int64_t ticket, int32_t* m, int32_t* n, double* x_out,
+                       double* y_out, supr_pdlp_info* info, int32_t* ready);
+void supr_server_destroy(supr_server* server);
 
 /* Blending LP: K components blended into P products with two quality
  * attributes.  Columns x_kp >= 0 in product-major order (col = p*K + k),

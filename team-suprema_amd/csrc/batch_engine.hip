@@ -1,12 +1,21 @@
-// batch_engine.hip -- many independent LPs solved together (supr_batch_*).
+// batch_engine.hip -- many independent LPs solved together: the batched
+// engine (supr_batch_*) and the serving engine (supr_server_*).
 //
-// Host side: concatenate the instances block-diagonally once; per solve,
-// lay the requested instances out over the launches (block -> instance
-// tables), cold-start them, and replay a captured graph of G attempts plus
-// the KKT-check group until every instance has converged or hit its limit.
-// When the number of running instances has halved, the finished ones are
-// dropped from the layout and the graph is captured again (compaction); an
-// instance keeps its block counts, so its arithmetic does not change.
+// Both sit on one resident pool.  The pool holds fixed-capacity device
+// arenas -- one index space for columns (every per-column vector lives at
+// the same offset in each column array), one for rows, one for nonzeros --
+// and a fixed number of state slots.  An instance occupies a column range
+// (n + 1 entries: its A^T row pointers share the offset), a row range
+// (m + 1: its A row pointers), a nonzero range and a slot, taken first-fit;
+// admission is whether those ranges fit.
+//
+// Running instances are laid out over the launches of the segmented
+// device-resident loop (batch_kernels.hpp) and a graph of G attempts plus
+// the KKT-check group is replayed.  Membership changes only at replay
+// boundaries: the batch drops finished instances when half have finished
+// (compaction); the server retires finished instances and admits queued
+// ones after every step.  An instance's block counts and lanes are fixed at
+// admission, so its arithmetic never depends on its neighbours.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,11 +23,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "batch_kernels.hpp"
@@ -27,6 +40,8 @@
 
 namespace supr {
 namespace {
+
+using Clock = std::chrono::steady_clock;
 
 struct HipError : std::runtime_error {
     explicit HipError(const std::string& m) : std::runtime_error(m) {}
@@ -55,18 +70,48 @@ class Buf {
         BATCH_HIP(hipMalloc(&p_, std::max<size_t>(count * sizeof(T), 256)));
         return static_cast<T*>(p_);
     }
-    template <typename T> T* put(const std::vector<T>& h, hipStream_t s) {
-        T* d = alloc<T>(h.size());
-        if (!h.empty()) {
-            BATCH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-            BATCH_HIP(hipStreamSynchronize(s));
-        }
-        return d;
-    }
     template <typename T> T* get() const { return static_cast<T*>(p_); }
 
    private:
     void* p_ = nullptr;
+};
+
+// First-fit interval allocator over [0, cap) with coalescing frees.
+class Arena {
+   public:
+    void reset(int64_t cap) {
+        free_.clear();
+        if (cap > 0) free_[0] = cap;
+    }
+    bool alloc(int64_t len, int64_t& off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < len) continue;
+            off = it->first;
+            const int64_t rest = it->second - len;
+            free_.erase(it);
+            if (rest) free_[off + len] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(int64_t off, int64_t len) {
+        auto it = free_.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+   private:
+    std::map<int64_t, int64_t> free_;
 };
 
 // Lanes per row from the mean row length (times the configured bias).
@@ -92,214 +137,329 @@ double inf_norm_finite(const double* v, int len) {
     return m;
 }
 
+// Device bytes per column, row and nonzero of a resident instance: the
+// FP64 iterate arrays, the KKT-check arrays, the row pointers; in the
+// mixed mode also the FP32 iterate arrays and FP32 values.
+constexpr int64_t kColBytes64 = 8 * 8 + 9 * 8 + 4, kColBytes32 = 8 * 4 + 8;
+constexpr int64_t kRowBytes64 = 7 * 8 + 1 + 8 * 8 + 4, kRowBytes32 = 7 * 4;
+constexpr int64_t kNzBytes64 = 2 * 4 + 4 * 8, kNzBytes32 = 2 * 4;
+
+int64_t footprint_bytes(int64_t m, int64_t n, int64_t nnz, bool mixed) {
+    return (n + 1) * (kColBytes64 + (mixed ? kColBytes32 : 0)) +
+           (m + 1) * (kRowBytes64 + (mixed ? kRowBytes32 : 0)) +
+           nnz * (kNzBytes64 + (mixed ? kNzBytes32 : 0));
+}
+
 }  // namespace
 
-struct Batch {
+// Host-side arrays of one instance, built on the caller's thread.
+struct Prepared {
+    int32_t m = 0, n = 0, nnz = 0;
+    std::vector<int32_t> ptrA, idxA, ptrAT, idxAT;
+    std::vector<double> vA, vAT, vuA, vuAT;
+    std::vector<float> fA, fAT;
+    std::vector<double> c, l, u, b, cu, lu, uu, rLu, rUu, Dr, Dc;
+    std::vector<float> fc, fl, fu, fb;
+    std::vector<uint8_t> ineq;
+    double meanA = 0.0, meanAT = 0.0;
+    DevState init{};
+};
+
+// include/pdlp_algo.hpp:67-77 starting values and the single-LP engine's
+// control fields (Solver::initial_state).
+DevState initial_state(const supr_lp& s, const supr_lp& un, const supr_pdlp_options& o, bool mixed) {
+    DevState st;
+    std::memset(&st, 0, sizeof(st));
+    double amax = 0.0, cn = 0.0, bn = 0.0;
+    for (int32_t p = 0; p < s.nnz; ++p) amax = smax(amax, std::fabs(s.val[p]));
+    for (int j = 0; j < s.n; ++j) cn += s.c[j] * s.c[j];
+    for (int i = 0; i < s.m; ++i) if (std::isfinite(s.rL[i])) bn += s.rL[i] * s.rL[i];
+    cn = std::sqrt(cn);
+    bn = std::sqrt(bn);
+    st.eta = amax > 0 ? 1.0 / amax : 1.0;
+    st.omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    st.merit_last_restart = INFINITY;
+    st.merit_prev_cand = INFINITY;
+    st.best_merit = INFINITY;
+    st.stall_ref = INFINITY;
+    st.tol = o.tol;
+    double sw = o.switch_tol;
+    if (sw <= 0) sw = o.tol < 1e-5 ? 1e-5 : 0.0;
+    st.switch_tol = sw;
+    st.stall_checks = o.stall_checks > 0 ? o.stall_checks : 6;
+    st.denom_p = 1.0 + std::max(inf_norm_finite(un.rL, un.m), inf_norm_finite(un.rU, un.m));
+    st.denom_d = 1.0 + inf_norm_finite(un.c, un.n);
+    st.obj_offset = un.obj_offset;
+    st.check_every = o.check_every;
+    st.need_aty = 1;
+    st.phase = mixed ? 0 : 1;
+    return st;
+}
+
+Prepared prepare(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc,
+                 const supr_pdlp_options& o, bool mixed) {
+    Prepared p;
+    p.m = S.m;
+    p.n = S.n;
+    p.nnz = S.nnz;
+    Csr A = build_csr(S, S.val, false), AT = build_csr(S, S.val, true);
+    p.vuA.resize(S.nnz);
+    p.vuAT.resize(S.nnz);
+    for (int32_t q = 0; q < S.nnz; ++q) {
+        p.vuA[q] = U.val[A.perm[q]];
+        p.vuAT[q] = U.val[AT.perm[q]];
+    }
+    p.ptrA = std::move(A.ptr);
+    p.idxA = std::move(A.idx);
+    p.vA = std::move(A.val);
+    p.ptrAT = std::move(AT.ptr);
+    p.idxAT = std::move(AT.idx);
+    p.vAT = std::move(AT.val);
+    p.c.assign(S.c, S.c + S.n);
+    p.l.assign(S.l, S.l + S.n);
+    p.u.assign(S.u, S.u + S.n);
+    p.b.assign(S.rL, S.rL + S.m);
+    p.ineq.resize(S.m);
+    for (int i = 0; i < S.m; ++i) p.ineq[i] = S.rL[i] != S.rU[i];
+    p.cu.assign(U.c, U.c + U.n);
+    p.lu.assign(U.l, U.l + U.n);
+    p.uu.assign(U.u, U.u + U.n);
+    p.rLu.assign(U.rL, U.rL + U.m);
+    p.rUu.assign(U.rU, U.rU + U.m);
+    p.Dr.assign(Dr, Dr + S.m);
+    p.Dc.assign(Dc, Dc + S.n);
+    if (mixed) {
+        p.fA.assign(p.vA.begin(), p.vA.end());
+        p.fAT.assign(p.vAT.begin(), p.vAT.end());
+        p.fc.assign(p.c.begin(), p.c.end());
+        p.fl.assign(p.l.begin(), p.l.end());
+        p.fu.assign(p.u.begin(), p.u.end());
+        p.fb.assign(p.b.begin(), p.b.end());
+    }
+    p.meanA = (double)S.nnz / S.m;
+    p.meanAT = (double)S.nnz / S.n;
+    p.init = initial_state(S, U, o, mixed);
+    return p;
+}
+
+// Checks one instance's pair of forms (what supr_batch_create and
+// supr_server_submit accept).
+bool check_instance(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc,
+                    const std::string& what) {
+    if (!check_lp(&U, "unscaled_form") || !check_lp(&S, "scaled_form")) return false;
+    if (U.m != S.m || U.n != S.n || U.nnz != S.nnz || S.m < 1 || S.n < 1 || !Dr || !Dc) {
+        set_error(what + " has mismatched or empty forms");
+        return false;
+    }
+    for (int32_t p = 0; p < S.nnz; ++p)
+        if (U.row_idx[p] != S.row_idx[p] || U.col_idx[p] != S.col_idx[p]) {
+            set_error(what + ": the forms differ in pattern");
+            return false;
+        }
+    return true;
+}
+
+struct Pool {
     int device = 0;
     hipStream_t stream = nullptr;
-    int count = 0;
     bool mixed = true;
     supr_pdlp_options opt{};
+    int64_t Ncap = 0, Mcap = 0, Zcap = 0;
+    int Scap = 0;
+    Arena an, am, az;
+    std::vector<int32_t> free_slots;
     std::vector<BatchInst> inst;
+    std::vector<int64_t> nnz_slot;
+    std::vector<DevState> init;
     std::vector<double> meanA, meanAT;
-    std::vector<DevState> init;   // per-instance initial state (max_iter set per solve)
-    std::vector<double> c_unscaled, obj_offset;   // for the reported objective
-    int64_t M = 0, N = 0, Z = 0;
+    std::vector<std::vector<double>> c_unscaled;
+    std::vector<double> obj_offset;
     double bias = 1.0;
     int cap = 32;
-    int64_t BNall = 0, BMall = 0;  // block totals of the whole batch at this configuration
     // device storage
     Buf ptrA, idxA, ptrAT, idxAT, uA, uAT, dA, dAT, fA, fAT;
     Buf fx[2], fy[2], fax[2], faty, fxsum, fysum, fxlast, fylast, fxd, fc, fl, fu, fb;
     Buf dx[2], dy[2], dax[2], daty, dxsum, dysum, dxlast, dylast, dc, dl, du, db;
     Buf ineq, cu, lu, uu, rLu, rUu, Dr, Dc, xu, xu2, xavg, yu, yu2, yavg, atyavg, axavg, bestx, besty;
-    Buf dstate, ddx2, dparts, dstatus, dinst, dtabn, dtabm, dids;
+    Buf dstate, ddx2, dparts, dstatus, dinst, dtabn, dtabm, dids, dstage, dslots;
     BatchVals V{};
     IterBufs<float> F{};
     IterBufs<double> D{};
     KktBufs K{};
     double *part_n = nullptr, *part_m = nullptr, *part_kn = nullptr, *part_km = nullptr;
     double *part_rn = nullptr, *part_rm = nullptr;
+    int64_t device_bytes = 0;
 
-    ~Batch() { if (stream) (void)hipStreamDestroy(stream); }
+    ~Pool() { if (stream) (void)hipStreamDestroy(stream); }
 
-    void build(int32_t cnt, const supr_lp* U, const supr_lp* S, const double* const* Drs,
-               const double* const* Dcs, const supr_pdlp_options& o) {
-        count = cnt;
+    template <typename T> T* arr(Buf& b, int64_t len) {
+        device_bytes += (int64_t)sizeof(T) * std::max<int64_t>(len, 1);
+        return b.alloc<T>((size_t)std::max<int64_t>(len, 1));
+    }
+
+    void create(int64_t ncap, int64_t mcap, int64_t zcap, int scap, const supr_pdlp_options& o,
+                double b, int c) {
         opt = o;
         mixed = o.mode != SUPR_MODE_FAST_FP64;
+        Ncap = ncap; Mcap = mcap; Zcap = zcap; Scap = scap;
+        an.reset(Ncap); am.reset(Mcap); az.reset(Zcap);
+        free_slots.resize(Scap);
+        for (int k = 0; k < Scap; ++k) free_slots[k] = Scap - 1 - k;   // slot 0 is taken first
+        inst.assign(Scap, BatchInst{});
+        nnz_slot.assign(Scap, 0);
+        init.resize(Scap);
+        meanA.assign(Scap, 0.0);
+        meanAT.assign(Scap, 0.0);
+        c_unscaled.resize(Scap);
+        obj_offset.assign(Scap, 0.0);
         BATCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        inst.assign(cnt, BatchInst{});
-        meanA.resize(cnt);
-        meanAT.resize(cnt);
-        init.resize(cnt);
-        obj_offset.resize(cnt);
-        for (int k = 0; k < cnt; ++k) { M += S[k].m; N += S[k].n; Z += S[k].nnz; }
-        std::vector<int32_t> hptrA, hptrAT, hidxA(Z), hidxAT(Z);
-        std::vector<double> hvA(Z), hvAT(Z), hvuA(Z), hvuAT(Z);
-        hptrA.reserve(M + cnt);
-        hptrAT.reserve(N + cnt);
-        std::vector<double> c(N), l(N), u(N), b(M), hlu(N), huu(N), hrL(M), hrU(M), hDr(M), hDc(N);
-        c_unscaled.assign(N, 0.0);
-        std::vector<uint8_t> hineq(M);
-        int64_t ro = 0, co = 0, za = 0;
-        for (int k = 0; k < cnt; ++k) {
-            const supr_lp& s = S[k];
-            const supr_lp& un = U[k];
-            BatchInst& I = inst[k];
-            I.m = s.m;
-            I.n = s.n;
-            I.ro = ro;
-            I.co = co;
-            I.pa = (int64_t)hptrA.size();
-            I.pat = (int64_t)hptrAT.size();
-            I.za = za;
-            const Csr A = build_csr(s, s.val, false), AT = build_csr(s, s.val, true);
-            hptrA.insert(hptrA.end(), A.ptr.begin(), A.ptr.end());
-            hptrAT.insert(hptrAT.end(), AT.ptr.begin(), AT.ptr.end());
-            for (int32_t p = 0; p < s.nnz; ++p) {
-                hidxA[za + p] = A.idx[p];
-                hvA[za + p] = A.val[p];
-                hvuA[za + p] = un.val[A.perm[p]];
-                hidxAT[za + p] = AT.idx[p];
-                hvAT[za + p] = AT.val[p];
-                hvuAT[za + p] = un.val[AT.perm[p]];
-            }
-            meanA[k] = (double)s.nnz / s.m;
-            meanAT[k] = (double)s.nnz / s.n;
-            for (int j = 0; j < s.n; ++j) {
-                c[co + j] = s.c[j]; l[co + j] = s.l[j]; u[co + j] = s.u[j];
-                c_unscaled[co + j] = un.c[j]; hlu[co + j] = un.l[j]; huu[co + j] = un.u[j];
-                hDc[co + j] = Dcs[k][j];
-            }
-            for (int i = 0; i < s.m; ++i) {
-                b[ro + i] = s.rL[i];
-                hineq[ro + i] = s.rL[i] != s.rU[i];
-                hrL[ro + i] = un.rL[i]; hrU[ro + i] = un.rU[i];
-                hDr[ro + i] = Drs[k][i];
-            }
-            init[k] = initial_state(s, un);
-            obj_offset[k] = un.obj_offset;
-            ro += s.m;
-            co += s.n;
-            za += s.nnz;
-        }
-        hipStream_t st = stream;
-        ptrA.put(hptrA, st);
-        idxA.put(hidxA, st);
-        ptrAT.put(hptrAT, st);
-        idxAT.put(hidxAT, st);
-        V.uA = uA.put(hvuA, st);
-        V.uAT = uAT.put(hvuAT, st);
-        V.dA = dA.put(hvA, st);
-        V.dAT = dAT.put(hvAT, st);
+        arr<int32_t>(ptrA, Mcap); arr<int32_t>(idxA, Zcap);
+        arr<int32_t>(ptrAT, Ncap); arr<int32_t>(idxAT, Zcap);
+        V.uA = arr<double>(uA, Zcap); V.uAT = arr<double>(uAT, Zcap);
+        V.dA = arr<double>(dA, Zcap); V.dAT = arr<double>(dAT, Zcap);
         auto iter = [&](auto& B, auto tag, Buf* x, Buf* y, Buf* ax, Buf& aty, Buf& xsum, Buf& ysum,
                         Buf& xlast, Buf& ylast, Buf& cb, Buf& lb, Buf& ub, Buf& bb) {
             using T = decltype(tag);
             for (int q = 0; q < 2; ++q) {
-                B.x[q] = x[q].alloc<T>(N);
-                B.y[q] = y[q].alloc<T>(M);
-                B.ax[q] = ax[q].alloc<T>(M);
+                B.x[q] = arr<T>(x[q], Ncap);
+                B.y[q] = arr<T>(y[q], Mcap);
+                B.ax[q] = arr<T>(ax[q], Mcap);
             }
-            B.aty = aty.alloc<T>(N);
-            B.xsum = xsum.alloc<T>(N);
-            B.ysum = ysum.alloc<T>(M);
-            B.xlast = xlast.alloc<T>(N);
-            B.ylast = ylast.alloc<T>(M);
-            B.c = cb.put(std::vector<T>(c.begin(), c.end()), st);
-            B.l = lb.put(std::vector<T>(l.begin(), l.end()), st);
-            B.u = ub.put(std::vector<T>(u.begin(), u.end()), st);
-            B.b = bb.put(std::vector<T>(b.begin(), b.end()), st);
+            B.aty = arr<T>(aty, Ncap);
+            B.xsum = arr<T>(xsum, Ncap);
+            B.ysum = arr<T>(ysum, Mcap);
+            B.xlast = arr<T>(xlast, Ncap);
+            B.ylast = arr<T>(ylast, Mcap);
+            B.c = arr<T>(cb, Ncap);
+            B.l = arr<T>(lb, Ncap);
+            B.u = arr<T>(ub, Ncap);
+            B.b = arr<T>(bb, Mcap);
+            B.xd = nullptr;
         };
-        const uint8_t* dineq = ineq.put(hineq, st);
+        const uint8_t* dineq = arr<uint8_t>(ineq, Mcap);
         iter(D, double{}, dx, dy, dax, daty, dxsum, dysum, dxlast, dylast, dc, dl, du, db);
-        D.xd = nullptr;
         D.ineq = dineq;
         if (mixed) {
-            V.fA = fA.put(std::vector<float>(hvA.begin(), hvA.end()), st);
-            V.fAT = fAT.put(std::vector<float>(hvAT.begin(), hvAT.end()), st);
+            V.fA = arr<float>(fA, Zcap);
+            V.fAT = arr<float>(fAT, Zcap);
             iter(F, float{}, fx, fy, fax, faty, fxsum, fysum, fxlast, fylast, fc, fl, fu, fb);
-            F.xd = fxd.alloc<float>(2 * N);
+            F.xd = arr<float>(fxd, 2 * Ncap);
             F.ineq = dineq;
-        } else {
-            F = IterBufs<float>{};
-            F.xd = nullptr;
         }
-        K.c = cu.put(c_unscaled, st); K.l = lu.put(hlu, st); K.u = uu.put(huu, st);
-        K.rL = rLu.put(hrL, st); K.rU = rUu.put(hrU, st);
-        K.Dr = Dr.put(hDr, st); K.Dc = Dc.put(hDc, st);
-        K.xu = xu.alloc<double>(N); K.xu2 = xu2.alloc<double>(N); K.xavg = xavg.alloc<double>(N);
-        K.aty_avg = atyavg.alloc<double>(N); K.best_x = bestx.alloc<double>(N);
-        K.yu = yu.alloc<double>(M); K.yu2 = yu2.alloc<double>(M); K.yavg = yavg.alloc<double>(M);
-        K.ax_avg = axavg.alloc<double>(M); K.best_y = besty.alloc<double>(M);
-        dstate.alloc<DevState>(2 * (size_t)cnt);
-        ddx2.alloc<double>(cnt);
-        dstatus.alloc<double>(2);
-        dinst.alloc<BatchInst>(cnt);
-        dids.alloc<int32_t>(cnt);
-        set_config(1.0, 32);
+        K.c = arr<double>(cu, Ncap); K.l = arr<double>(lu, Ncap); K.u = arr<double>(uu, Ncap);
+        K.rL = arr<double>(rLu, Mcap); K.rU = arr<double>(rUu, Mcap);
+        K.Dr = arr<double>(Dr, Mcap); K.Dc = arr<double>(Dc, Ncap);
+        K.xu = arr<double>(xu, Ncap); K.xu2 = arr<double>(xu2, Ncap); K.xavg = arr<double>(xavg, Ncap);
+        K.aty_avg = arr<double>(atyavg, Ncap); K.best_x = arr<double>(bestx, Ncap);
+        K.yu = arr<double>(yu, Mcap); K.yu2 = arr<double>(yu2, Mcap); K.yavg = arr<double>(yavg, Mcap);
+        K.ax_avg = arr<double>(axavg, Mcap); K.best_y = arr<double>(besty, Mcap);
+        arr<DevState>(dstate, 2 * (int64_t)Scap);
+        arr<double>(ddx2, Scap);
+        arr<double>(dstatus, 2);
+        arr<BatchInst>(dinst, Scap);
+        arr<int32_t>(dids, Scap);
+        arr<DevState>(dstage, Scap);
+        arr<int32_t>(dslots, Scap);
+        set_config(b, c);
     }
 
-    // include/pdlp_algo.hpp:67-77 starting values and the single-LP
-    // engine's control fields (Solver::initial_state).
-    DevState initial_state(const supr_lp& s, const supr_lp& un) const {
-        DevState st;
-        std::memset(&st, 0, sizeof(st));
-        double amax = 0.0, cn = 0.0, bn = 0.0;
-        for (int32_t p = 0; p < s.nnz; ++p) amax = smax(amax, std::fabs(s.val[p]));
-        for (int j = 0; j < s.n; ++j) cn += s.c[j] * s.c[j];
-        for (int i = 0; i < s.m; ++i) if (std::isfinite(s.rL[i])) bn += s.rL[i] * s.rL[i];
-        cn = std::sqrt(cn);
-        bn = std::sqrt(bn);
-        st.eta = amax > 0 ? 1.0 / amax : 1.0;
-        st.omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
-        st.merit_last_restart = INFINITY;
-        st.merit_prev_cand = INFINITY;
-        st.best_merit = INFINITY;
-        st.stall_ref = INFINITY;
-        st.tol = opt.tol;
-        double sw = opt.switch_tol;
-        if (sw <= 0) sw = opt.tol < 1e-5 ? 1e-5 : 0.0;
-        st.switch_tol = sw;
-        st.stall_checks = opt.stall_checks > 0 ? opt.stall_checks : 6;
-        st.denom_p = 1.0 + std::max(inf_norm_finite(un.rL, un.m), inf_norm_finite(un.rU, un.m));
-        st.denom_d = 1.0 + inf_norm_finite(un.c, un.n);
-        st.obj_offset = un.obj_offset;
-        st.check_every = opt.check_every;
-        st.need_aty = 1;
-        st.phase = mixed ? 0 : 1;
-        return st;
-    }
-
-    // Lanes rule and per-instance block cap; partial-sum storage for the
-    // whole batch at this configuration.
+    // Lanes rule and per-instance block cap (only while no instance runs);
+    // partial-sum and table storage for Scap instances at this cap.
     void set_config(double b, int c) {
-        int64_t bn_all = 0, bm_all = 0;
-        std::vector<BatchInst> next(inst);
-        for (int k = 0; k < count; ++k) {
-            BatchInst& I = next[k];
-            I.LA = lanes_for(meanA[k], b);
-            I.LAT = lanes_for(meanAT[k], b);
-            I.bn = blocks_for(I.n, I.LAT, c);
-            I.bm = blocks_for(I.m, I.LA, c);
-            bn_all += I.bn;
-            bm_all += I.bm;
-        }
-        if (bn_all + bm_all > (int64_t)INT32_MAX / 16) throw ArgError("batch too large for one launch");
-        double* p = dparts.alloc<double>((size_t)10 * bn_all + 9 * bm_all);
-        part_n = p; p += bn_all;
-        part_kn = p; p += 8 * bn_all;
-        part_rn = p; p += bn_all;
-        part_m = p; p += 2 * bm_all;
-        part_km = p; p += 6 * bm_all;
+        const int64_t per_side = (int64_t)Scap * c;
+        if (per_side > (int64_t)INT32_MAX / 16) throw ArgError("too many blocks for one launch");
+        double* p = dparts.alloc<double>((size_t)19 * per_side);
+        part_n = p; p += per_side;
+        part_kn = p; p += 8 * per_side;
+        part_rn = p; p += per_side;
+        part_m = p; p += 2 * per_side;
+        part_km = p; p += 6 * per_side;
         part_rm = p;
-        dtabn.alloc<int32_t>(bn_all);
-        dtabm.alloc<int32_t>(bm_all);
-        inst.swap(next);
-        BNall = bn_all;
-        BMall = bm_all;
+        dtabn.alloc<int32_t>(per_side);
+        dtabm.alloc<int32_t>(per_side);
         bias = b;
         cap = c;
+        for (int k = 0; k < Scap; ++k) if (inst[k].n > 0) shape_rule(k);
+    }
+
+    void shape_rule(int k) {
+        BatchInst& I = inst[k];
+        I.LA = lanes_for(meanA[k], bias);
+        I.LAT = lanes_for(meanAT[k], bias);
+        I.bn = blocks_for(I.n, I.LAT, cap);
+        I.bm = blocks_for(I.m, I.LA, cap);
+    }
+
+    bool fits(const Prepared& p) const {
+        return p.n + 1 <= Ncap && p.m + 1 <= Mcap && p.nnz <= Zcap;
+    }
+
+    // Takes ranges and a slot for p and uploads it; -1 when it does not fit now.
+    int add(const Prepared& p) {
+        if (free_slots.empty()) return -1;
+        int64_t co = 0, ro = 0, za = 0;
+        if (!an.alloc(p.n + 1, co)) return -1;
+        if (!am.alloc(p.m + 1, ro)) { an.release(co, p.n + 1); return -1; }
+        if (p.nnz > 0 && !az.alloc(p.nnz, za)) { an.release(co, p.n + 1); am.release(ro, p.m + 1); return -1; }
+        const int k = free_slots.back();
+        free_slots.pop_back();
+        BatchInst& I = inst[k];
+        I = BatchInst{};
+        I.m = p.m; I.n = p.n;
+        I.ro = ro; I.co = co; I.pa = ro; I.pat = co; I.za = za;
+        nnz_slot[k] = p.nnz;
+        meanA[k] = p.meanA;
+        meanAT[k] = p.meanAT;
+        shape_rule(k);
+        init[k] = p.init;
+        c_unscaled[k] = p.cu;
+        obj_offset[k] = p.init.obj_offset;
+        hipStream_t st = stream;
+        auto put = [&](auto* dst, const auto& v) {
+            if (!v.empty())
+                BATCH_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st));
+        };
+        put(ptrA.get<int32_t>() + ro, p.ptrA);
+        put(ptrAT.get<int32_t>() + co, p.ptrAT);
+        put(idxA.get<int32_t>() + za, p.idxA);
+        put(idxAT.get<int32_t>() + za, p.idxAT);
+        put(dA.get<double>() + za, p.vA);
+        put(dAT.get<double>() + za, p.vAT);
+        put(uA.get<double>() + za, p.vuA);
+        put(uAT.get<double>() + za, p.vuAT);
+        put(dc.get<double>() + co, p.c);
+        put(dl.get<double>() + co, p.l);
+        put(du.get<double>() + co, p.u);
+        put(db.get<double>() + ro, p.b);
+        put(ineq.get<uint8_t>() + ro, p.ineq);
+        put(cu.get<double>() + co, p.cu);
+        put(lu.get<double>() + co, p.lu);
+        put(uu.get<double>() + co, p.uu);
+        put(Dc.get<double>() + co, p.Dc);
+        put(rLu.get<double>() + ro, p.rLu);
+        put(rUu.get<double>() + ro, p.rUu);
+        put(Dr.get<double>() + ro, p.Dr);
+        if (mixed) {
+            put(fA.get<float>() + za, p.fA);
+            put(fAT.get<float>() + za, p.fAT);
+            put(fc.get<float>() + co, p.fc);
+            put(fl.get<float>() + co, p.fl);
+            put(fu.get<float>() + co, p.fu);
+            put(fb.get<float>() + ro, p.fb);
+        }
+        BATCH_HIP(hipStreamSynchronize(st));   // p's vectors may go away after this
+        return k;
+    }
+
+    void remove(int k) {
+        BatchInst& I = inst[k];
+        an.release(I.co, I.n + 1);
+        am.release(I.ro, I.m + 1);
+        if (nnz_slot[k] > 0) az.release(I.za, nnz_slot[k]);
+        nnz_slot[k] = 0;
+        I = BatchInst{};
+        c_unscaled[k].clear();
+        free_slots.push_back(k);
     }
 
     // Lays the instances `live` out over the launches; returns the geometry.
@@ -316,7 +476,7 @@ struct Batch {
             om += I.bm;
         }
         hipStream_t st = stream;
-        BATCH_HIP(hipMemcpyAsync(dinst.get<BatchInst>(), inst.data(), sizeof(BatchInst) * count,
+        BATCH_HIP(hipMemcpyAsync(dinst.get<BatchInst>(), inst.data(), sizeof(BatchInst) * Scap,
                                  hipMemcpyHostToDevice, st));
         BATCH_HIP(hipMemcpyAsync(dtabn.get<int32_t>(), tn.data(), sizeof(int32_t) * tn.size(),
                                  hipMemcpyHostToDevice, st));
@@ -337,6 +497,38 @@ struct Batch {
         g.ptrAT = ptrAT.get<int32_t>();
         g.idxAT = idxAT.get<int32_t>();
         return g;
+    }
+
+    // Cold start of `fresh` (initial states with this iteration limit,
+    // starting iterates).  Lays out only `fresh`: the caller lays out the
+    // running set afterwards.
+    void cold_start(const std::vector<int32_t>& fresh, int max_iter) {
+        if (fresh.empty()) return;
+        std::vector<DevState> st0(fresh.size());
+        for (size_t r = 0; r < fresh.size(); ++r) {
+            DevState s = init[fresh[r]];
+            s.max_iter = max_iter;
+            s.active = max_iter > 0 ? 1 : 0;
+            s.need_check = max_iter > 0 ? 0 : 1;
+            st0[r] = s;
+        }
+        hipStream_t st = stream;
+        BATCH_HIP(hipMemcpyAsync(dstage.get<DevState>(), st0.data(), sizeof(DevState) * st0.size(),
+                                 hipMemcpyHostToDevice, st));
+        BATCH_HIP(hipMemcpyAsync(dslots.get<int32_t>(), fresh.data(), sizeof(int32_t) * fresh.size(),
+                                 hipMemcpyHostToDevice, st));
+        const int cnt = (int)fresh.size();
+        hipLaunchKernelGGL(kb_put_states, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                           dstate.get<DevState>(), dstage.get<DevState>(), dslots.get<int32_t>(), cnt);
+        const BatchGeo g = layout(fresh);
+        if (mixed) {
+            hipLaunchKernelGGL(kb_cold<float>, dim3(g.BN + g.BM), dim3(kBlock), 0, st, g, F);
+        } else {
+            hipLaunchKernelGGL(kb_cold<double>, dim3(g.BN + g.BM), dim3(kBlock), 0, st, g, D);
+            hipLaunchKernelGGL(kb_cold_ax<double>, dim3(g.BM), dim3(kBlock), 0, st, g, V, D);
+        }
+        BATCH_HIP(hipGetLastError());
+        BATCH_HIP(hipStreamSynchronize(st));
     }
 
     void enqueue_replay(const BatchGeo& g, int live, int G) {
@@ -377,42 +569,87 @@ struct Batch {
         return exec;
     }
 
-    // Solves the instances `ids` from the cold start; fills states[k] (S[2k]
-    // at the end) for k in ids.  Returns seconds.
+    int graph_attempts() const {
+        int G = opt.graph_attempts > 0 ? opt.graph_attempts : 16;
+        return G + (G & 1);   // the state double buffer needs an even number per replay
+    }
+
+    void read_states(std::vector<DevState>& h) {
+        h.resize(2 * (size_t)Scap);
+        BATCH_HIP(hipMemcpy(h.data(), dstate.get<DevState>(), sizeof(DevState) * h.size(),
+                            hipMemcpyDeviceToHost));
+    }
+
+    // The point and information of a finished instance.
+    void result(int k, const DevState& s, double* x_out, double* y_out, supr_pdlp_info& inf) {
+        const BatchInst& I = inst[k];
+        std::memset(&inf, 0, sizeof(inf));
+        inf.iterations = s.attempts;
+        inf.restarts = s.restarts;
+        inf.converged = s.done == 1 ? 1 : 0;
+        inf.has_point = s.has_best ? 1 : 0;
+        inf.eps_P = s.best_eps[0]; inf.eps_D = s.best_eps[1]; inf.eps_G = s.best_eps[2];
+        inf.iterations_fp32 = mixed ? (s.phase == 0 ? s.attempts : s.attempts_fp32) : 0;
+        inf.checks = s.checks;
+        inf.lanes_A = I.LA; inf.lanes_AT = I.LAT;
+        double obj = obj_offset[k];
+        if (s.has_best) {
+            // the returned point and its objective c'x + offset (unscaled form)
+            std::vector<double> xh(I.n);
+            BATCH_HIP(hipMemcpy(xh.data(), K.best_x + I.co, sizeof(double) * I.n, hipMemcpyDeviceToHost));
+            double cx = 0.0;
+            for (int j = 0; j < I.n; ++j) cx += c_unscaled[k][j] * xh[j];
+            obj = cx + obj_offset[k];
+            if (x_out) std::copy(xh.begin(), xh.end(), x_out);
+            if (y_out) BATCH_HIP(hipMemcpy(y_out, K.best_y + I.ro, sizeof(double) * I.m, hipMemcpyDeviceToHost));
+        }
+        inf.objective = obj;
+    }
+};
+
+// ------------------------------------------------------------ the batch --
+
+struct Batch {
+    Pool pool;
+    int count = 0;
+    std::vector<int32_t> slot;   // instance -> pool slot
+
+    void build(int32_t cnt, const supr_lp* U, const supr_lp* S, const double* const* Drs,
+               const double* const* Dcs, const supr_pdlp_options& o) {
+        count = cnt;
+        const bool mixed = o.mode != SUPR_MODE_FAST_FP64;
+        int64_t N = 0, M = 0, Z = 0;
+        for (int k = 0; k < cnt; ++k) { N += S[k].n + 1; M += S[k].m + 1; Z += S[k].nnz; }
+        pool.create(N, M, Z, cnt, o, 1.0, 32);
+        slot.resize(cnt);
+        for (int k = 0; k < cnt; ++k) {
+            slot[k] = pool.add(prepare(U[k], S[k], Drs[k], Dcs[k], o, mixed));
+            if (slot[k] < 0) throw std::runtime_error("batch layout: an instance does not fit its own pool");
+        }
+    }
+
+    // Solves `ids` from the cold start; states[r] is the final state of ids[r].
     double run(const std::vector<int32_t>& ids, int max_iter, std::vector<DevState>& states) {
-        hipStream_t st = stream;
-        std::vector<DevState> h(2 * (size_t)count);
+        std::vector<int32_t> live;
         {
             std::vector<char> seen(count, 0);
             for (int32_t k : ids) {
                 if (k < 0 || k >= count) throw ArgError("instance id out of range");
                 if (seen[k]) throw ArgError("instance id listed twice");
                 seen[k] = 1;
-                DevState s0 = init[k];
-                s0.max_iter = max_iter;
-                s0.active = max_iter > 0 ? 1 : 0;
-                s0.need_check = max_iter > 0 ? 0 : 1;
-                h[2 * k] = h[2 * k + 1] = s0;
+                live.push_back(slot[k]);
             }
         }
-        const auto t0 = std::chrono::steady_clock::now();
-        BATCH_HIP(hipMemcpyAsync(dstate.get<DevState>(), h.data(), sizeof(DevState) * h.size(),
-                                 hipMemcpyHostToDevice, st));
-        std::vector<int32_t> live(ids);
-        BatchGeo g = layout(live);
-        if (mixed) {
-            hipLaunchKernelGGL(kb_cold<float>, dim3(g.BN + g.BM), dim3(kBlock), 0, st, g, F);
-        } else {
-            hipLaunchKernelGGL(kb_cold<double>, dim3(g.BN + g.BM), dim3(kBlock), 0, st, g, D);
-            hipLaunchKernelGGL(kb_cold_ax<double>, dim3(g.BM), dim3(kBlock), 0, st, g, V, D);
-        }
-        BATCH_HIP(hipGetLastError());
-        int G = opt.graph_attempts > 0 ? opt.graph_attempts : 16;
-        G += G & 1;   // the state double buffer needs an even number of attempts per replay
+        hipStream_t st = pool.stream;
+        const auto t0 = Clock::now();
+        pool.cold_start(live, max_iter);
+        BatchGeo g = pool.layout(live);
+        const int G = pool.graph_attempts();
         hipGraphExec_t exec = nullptr;
         constexpr int R = 4;
         double* ring = nullptr;
         hipEvent_t ev[R] = {};
+        std::vector<DevState> h;
         auto cleanup = [&]() {
             (void)hipStreamSynchronize(st);
             for (auto& e : ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
@@ -422,16 +659,16 @@ struct Batch {
         try {
             BATCH_HIP(hipHostMalloc((void**)&ring, sizeof(double) * 2 * R, hipHostMallocDefault));
             for (auto& e : ev) BATCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            exec = capture(g, (int)live.size(), G);
+            exec = pool.capture(g, (int)live.size(), G);
             double last_progress = -1.0;
             int idle = 0, replays = 0, inflight = 0;
             size_t live_at_capture = live.size();
             while (true) {
                 if (inflight == R) {
-                    const int slot = (replays - inflight) % R;
-                    BATCH_HIP(hipEventSynchronize(ev[slot]));
+                    const int s = (replays - inflight) % R;
+                    BATCH_HIP(hipEventSynchronize(ev[s]));
                     --inflight;
-                    const double running = ring[2 * slot], progress = ring[2 * slot + 1];
+                    const double running = ring[2 * s], progress = ring[2 * s + 1];
                     if (running == 0.0) break;
                     // A replay always advances attempts or checks of a running
                     // instance; a loop that stops moving is a logic error.
@@ -445,27 +682,26 @@ struct Batch {
                         // compaction: drain, drop finished instances, capture again
                         BATCH_HIP(hipStreamSynchronize(st));
                         inflight = 0;
-                        BATCH_HIP(hipMemcpy(h.data(), dstate.get<DevState>(), sizeof(DevState) * h.size(),
-                                            hipMemcpyDeviceToHost));
+                        pool.read_states(h);
                         std::vector<int32_t> still;
                         for (int32_t k : live) if (!h[2 * k].done) still.push_back(k);
                         if (still.empty()) break;
-                        if (opt.verbose)
+                        if (pool.opt.verbose)
                             std::fprintf(stderr, "[suprema] batch: replay %d, %zu of %zu instances running, layout compacted\n",
                                          replays, still.size(), live.size());
                         live.swap(still);
                         live_at_capture = live.size();
                         (void)hipGraphExecDestroy(exec);
                         exec = nullptr;
-                        g = layout(live);
-                        exec = capture(g, (int)live.size(), G);
+                        g = pool.layout(live);
+                        exec = pool.capture(g, (int)live.size(), G);
                     }
                 }
-                const int slot = replays % R;
+                const int s = replays % R;
                 BATCH_HIP(hipGraphLaunch(exec, st));
-                BATCH_HIP(hipMemcpyAsync(&ring[2 * slot], dstatus.get<void>(), sizeof(double) * 2,
+                BATCH_HIP(hipMemcpyAsync(&ring[2 * s], pool.dstatus.get<void>(), sizeof(double) * 2,
                                          hipMemcpyDeviceToHost, st));
-                BATCH_HIP(hipEventRecord(ev[slot], st));
+                BATCH_HIP(hipEventRecord(ev[s], st));
                 ++replays;
                 ++inflight;
             }
@@ -475,10 +711,10 @@ struct Batch {
             throw;
         }
         cleanup();
-        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        BATCH_HIP(hipMemcpy(h.data(), dstate.get<DevState>(), sizeof(DevState) * h.size(), hipMemcpyDeviceToHost));
-        states.resize(count);
-        for (int32_t k : ids) states[k] = h[2 * k];
+        const double secs = std::chrono::duration<double>(Clock::now() - t0).count();
+        pool.read_states(h);
+        states.resize(ids.size());
+        for (size_t r = 0; r < ids.size(); ++r) states[r] = h[2 * slot[ids[r]]];
         return secs;
     }
 
@@ -492,13 +728,146 @@ struct Batch {
         int best_cap = 32;
         for (double b : {0.5, 1.0, 2.0}) {
             for (int c : {8, 32, 128}) {
-                set_config(b, c);
+                pool.set_config(b, c);
                 run(all, probe_iters, scratch_states);            // warm
                 const double t = run(all, probe_iters, scratch_states);
                 if (t < best) { best = t; best_bias = b; best_cap = c; }
             }
         }
-        set_config(best_bias, best_cap);
+        pool.set_config(best_bias, best_cap);
+    }
+};
+
+// ----------------------------------------------------------- the server --
+
+struct Request {
+    int64_t ticket = 0;
+    Prepared prep;
+    Clock::time_point submitted;
+};
+
+struct Finished {
+    int32_t m = 0, n = 0;
+    std::vector<double> x, y;
+    supr_pdlp_info info{};
+};
+
+struct Server {
+    Pool pool;
+    supr_server_options so{};
+    std::mutex mu;                      // guards queue, next_ticket, done
+    std::deque<std::unique_ptr<Request>> queue;
+    int64_t next_ticket = 1;
+    std::unordered_map<int64_t, Finished> done;
+    // owned by the stepping thread
+    std::vector<int32_t> live;          // running slots
+    std::unordered_map<int32_t, std::unique_ptr<Request>> running;   // slot -> request
+    std::unordered_map<int32_t, Clock::time_point> admitted;
+    hipGraphExec_t exec = nullptr;
+    int32_t finished_total = 0;
+    std::vector<DevState> h;
+
+    ~Server() { if (exec) (void)hipGraphExecDestroy(exec); }
+
+    void create(const supr_server_options& o) {
+        so = o;
+        const int cap = o.max_blocks > 0 ? o.max_blocks : 256;
+        const double bias = o.lanes_bias > 0 ? o.lanes_bias : 1.0;
+        pool.create(o.max_cols, o.max_rows, o.max_nnz, o.max_instances, o.pdlp, bias, cap);
+    }
+
+    int64_t submit(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc) {
+        auto r = std::make_unique<Request>();
+        r->prep = prepare(U, S, Dr, Dc, pool.opt, pool.mixed);   // caller's thread, no lock held
+        r->submitted = Clock::now();
+        if (!pool.fits(r->prep)) throw ArgError("request exceeds the server's column, row or nonzero capacity");
+        std::lock_guard<std::mutex> lock(mu);
+        r->ticket = next_ticket++;
+        const int64_t t = r->ticket;
+        queue.push_back(std::move(r));
+        return t;
+    }
+
+    // Retires finished instances, admits queued requests that fit (first
+    // come, first served: a request that does not fit waits at the head),
+    // then runs the configured number of graph replays.
+    void step(int32_t* n_running, int32_t* n_queued, int32_t* n_finished) {
+        bool changed = false;
+        if (!live.empty()) {
+            std::vector<int32_t> still;
+            for (int32_t k : live) {
+                const DevState& s = h[2 * k];
+                if (!s.done) { still.push_back(k); continue; }
+                auto it = running.find(k);
+                Finished f;
+                f.m = pool.inst[k].m;
+                f.n = pool.inst[k].n;
+                f.x.resize(f.n);
+                f.y.resize(f.m);
+                pool.result(k, s, f.x.data(), f.y.data(), f.info);
+                const auto now = Clock::now();
+                f.info.seconds_solve = std::chrono::duration<double>(now - admitted[k]).count();
+                f.info.seconds_total = std::chrono::duration<double>(now - it->second->submitted).count();
+                f.info.seconds_setup = f.info.seconds_total - f.info.seconds_solve;
+                const int64_t t = it->second->ticket;
+                running.erase(it);
+                admitted.erase(k);
+                pool.remove(k);
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    done.emplace(t, std::move(f));
+                }
+                ++finished_total;
+                changed = true;
+            }
+            live.swap(still);
+        }
+        std::vector<int32_t> fresh;
+        while (true) {
+            std::unique_ptr<Request> r;
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                if (queue.empty()) break;
+                r = std::move(queue.front());
+                queue.pop_front();
+            }
+            const int k = pool.add(r->prep);
+            if (k < 0) {
+                std::lock_guard<std::mutex> lock(mu);
+                queue.push_front(std::move(r));
+                break;
+            }
+            r->prep = Prepared{};   // uploaded; free the host copy
+            admitted[k] = Clock::now();
+            running[k] = std::move(r);
+            fresh.push_back(k);
+        }
+        if (!fresh.empty()) {
+            pool.cold_start(fresh, pool.opt.max_iterations);
+            live.insert(live.end(), fresh.begin(), fresh.end());
+            changed = true;
+        }
+        if (changed) {
+            if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+            if (!live.empty()) {
+                const BatchGeo g = pool.layout(live);
+                exec = pool.capture(g, (int)live.size(), pool.graph_attempts());
+                if (pool.opt.verbose)
+                    std::fprintf(stderr, "[suprema] server: %zu running, graph captured\n", live.size());
+            }
+        }
+        if (!live.empty()) {
+            const int reps = so.replays_per_step > 0 ? so.replays_per_step : 1;
+            for (int q = 0; q < reps; ++q) BATCH_HIP(hipGraphLaunch(exec, pool.stream));
+            BATCH_HIP(hipStreamSynchronize(pool.stream));
+            pool.read_states(h);
+        }
+        *n_running = (int32_t)live.size();
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            *n_queued = (int32_t)queue.size();
+        }
+        *n_finished = finished_total;
     }
 };
 
@@ -506,6 +875,10 @@ struct Batch {
 
 struct supr_batch {
     supr::Batch b;
+};
+
+struct supr_server {
+    supr::Server s;
 };
 
 extern "C" {
@@ -529,18 +902,10 @@ int supr_batch_create(int32_t count, const supr_lp* unscaled_forms, const supr_l
     }
     int64_t zsum = 0;
     for (int k = 0; k < count; ++k) {
-        const supr_lp &U = unscaled_forms[k], &S = scaled_forms[k];
-        if (!check_lp(&U, "unscaled_form") || !check_lp(&S, "scaled_form")) return SUPR_EINVAL;
-        if (U.m != S.m || U.n != S.n || U.nnz != S.nnz || S.m < 1 || S.n < 1 || !Dr[k] || !Dc[k]) {
-            set_error("supr_batch_create: instance " + std::to_string(k) + " has mismatched or empty forms");
+        if (!check_instance(unscaled_forms[k], scaled_forms[k], Dr[k], Dc[k],
+                            "supr_batch_create: instance " + std::to_string(k)))
             return SUPR_EINVAL;
-        }
-        for (int32_t p = 0; p < S.nnz; ++p)
-            if (U.row_idx[p] != S.row_idx[p] || U.col_idx[p] != S.col_idx[p]) {
-                set_error("supr_batch_create: instance " + std::to_string(k) + " forms differ in pattern");
-                return SUPR_EINVAL;
-            }
-        zsum += S.nnz;
+        zsum += scaled_forms[k].nnz;
     }
     if (zsum > INT32_MAX) { set_error("supr_batch_create: total nnz exceeds int32"); return SUPR_EINVAL; }
     int devs = 0;
@@ -548,7 +913,7 @@ int supr_batch_create(int32_t count, const supr_lp* unscaled_forms, const supr_l
     try {
         BATCH_HIP(hipSetDevice(opt->device));
         auto h = std::make_unique<supr_batch>();
-        h->b.device = opt->device;
+        h->b.pool.device = opt->device;
         h->b.build(count, unscaled_forms, scaled_forms, Dr, Dc, *opt);
         if (opt->spmv_lanes_A == 0) h->b.tune(128);
         *out = h.release();
@@ -575,8 +940,8 @@ int supr_batch_configure(supr_batch* batch, double lanes_bias, int32_t max_block
         return SUPR_EINVAL;
     }
     try {
-        BATCH_HIP(hipSetDevice(batch->b.device));
-        batch->b.set_config(lanes_bias, max_blocks);
+        BATCH_HIP(hipSetDevice(batch->b.pool.device));
+        batch->b.pool.set_config(lanes_bias, max_blocks);
         return SUPR_OK;
     } catch (const ArgError& e) {
         set_error(e.what());
@@ -589,8 +954,8 @@ int supr_batch_configure(supr_batch* batch, double lanes_bias, int32_t max_block
 
 int supr_batch_config(const supr_batch* batch, double* lanes_bias, int32_t* max_blocks) {
     if (!batch || !lanes_bias || !max_blocks) { supr::set_error("null argument"); return SUPR_EINVAL; }
-    *lanes_bias = batch->b.bias;
-    *max_blocks = batch->b.cap;
+    *lanes_bias = batch->b.pool.bias;
+    *max_blocks = batch->b.pool.cap;
     return SUPR_OK;
 }
 
@@ -603,42 +968,17 @@ int supr_batch_solve(supr_batch* batch, const int32_t* ids, int32_t nids, double
     }
     try {
         Batch& b = batch->b;
-        BATCH_HIP(hipSetDevice(b.device));
+        BATCH_HIP(hipSetDevice(b.pool.device));
         std::vector<int32_t> list;
         if (ids) list.assign(ids, ids + nids);
         else { list.resize(b.count); std::iota(list.begin(), list.end(), 0); }
         std::vector<DevState> st;
-        const double secs = b.run(list, b.opt.max_iterations, st);
+        const double secs = b.run(list, b.pool.opt.max_iterations, st);
         if (seconds) *seconds = secs;
-        std::vector<double> xh;
         for (size_t r = 0; r < list.size(); ++r) {
-            const int32_t id = list[r];
-            const BatchInst& I = b.inst[id];
-            const DevState& s = st[id];
-            supr_pdlp_info& inf = infos[r];
-            std::memset(&inf, 0, sizeof(inf));
-            inf.iterations = s.attempts;
-            inf.restarts = s.restarts;
-            inf.converged = s.done == 1 ? 1 : 0;
-            inf.has_point = s.has_best ? 1 : 0;
-            inf.eps_P = s.best_eps[0]; inf.eps_D = s.best_eps[1]; inf.eps_G = s.best_eps[2];
-            inf.iterations_fp32 = b.mixed ? (s.phase == 0 ? s.attempts : s.attempts_fp32) : 0;
-            inf.checks = s.checks;
-            inf.lanes_A = I.LA; inf.lanes_AT = I.LAT;
-            inf.seconds_solve = secs;
-            double obj = b.obj_offset[id];
-            if (s.has_best) {
-                // the returned point and its objective c'x + offset (unscaled form)
-                xh.resize(I.n);
-                BATCH_HIP(hipMemcpy(xh.data(), b.K.best_x + I.co, sizeof(double) * I.n, hipMemcpyDeviceToHost));
-                double cx = 0.0;
-                for (int j = 0; j < I.n; ++j) cx += b.c_unscaled[I.co + j] * xh[j];
-                obj = cx + b.obj_offset[id];
-                if (x_out && x_out[r]) std::copy(xh.begin(), xh.end(), x_out[r]);
-                if (y_out && y_out[r])
-                    BATCH_HIP(hipMemcpy(y_out[r], b.K.best_y + I.ro, sizeof(double) * I.m, hipMemcpyDeviceToHost));
-            }
-            inf.objective = obj;
+            b.pool.result(b.slot[list[r]], st[r], x_out ? x_out[r] : nullptr, y_out ? y_out[r] : nullptr,
+                          infos[r]);
+            infos[r].seconds_solve = secs;
         }
         return SUPR_OK;
     } catch (const ArgError& e) {
@@ -655,8 +995,130 @@ int supr_batch_solve(supr_batch* batch, const int32_t* ids, int32_t nids, double
 
 void supr_batch_destroy(supr_batch* batch) {
     if (!batch) return;
-    (void)hipSetDevice(batch->b.device);
+    (void)hipSetDevice(batch->b.pool.device);
     delete batch;
+}
+
+void supr_server_default_options(supr_server_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    supr_pdlp_default_options(&o->pdlp);
+    o->max_cols = (int64_t)1 << 24;
+    o->max_rows = (int64_t)1 << 23;
+    o->max_nnz = (int64_t)1 << 27;
+    o->max_instances = 256;
+    o->max_blocks = 256;
+    o->lanes_bias = 1.0;
+    o->replays_per_step = 1;
+}
+
+int64_t supr_server_request_bytes(int32_t m, int32_t n, int32_t nnz, int32_t mode) {
+    return supr::footprint_bytes(m, n, nnz, mode != SUPR_MODE_FAST_FP64);
+}
+
+int supr_server_create(const supr_server_options* opt, supr_server** out) {
+    using namespace supr;
+    if (!opt || !out) { set_error("supr_server_create: null argument"); return SUPR_EINVAL; }
+    *out = nullptr;
+    const supr_pdlp_options& p = opt->pdlp;
+    if (p.mode == SUPR_MODE_PLAIN_FP64 || p.max_iterations < 0 || p.check_every < 1 || !(p.tol >= 0) ||
+        opt->max_cols < 2 || opt->max_rows < 2 || opt->max_nnz < 1 || opt->max_instances < 1 ||
+        opt->max_blocks < 0 || opt->max_blocks > kMaxBlocks) {
+        set_error("supr_server_create: bad options (device-resident mode, positive capacities, max_blocks <= 2048)");
+        return SUPR_EINVAL;
+    }
+    int devs = 0;
+    if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) { set_error("no HIP device visible"); return SUPR_ENODEV; }
+    try {
+        BATCH_HIP(hipSetDevice(p.device));
+        auto h = std::make_unique<supr_server>();
+        h->s.pool.device = p.device;
+        h->s.create(*opt);
+        *out = h.release();
+        return SUPR_OK;
+    } catch (const ArgError& e) {
+        set_error(e.what());
+        return SUPR_EINVAL;
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return SUPR_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return SUPR_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SUPR_EINTERNAL;
+    }
+}
+
+int supr_server_device_bytes(const supr_server* server, int64_t* bytes) {
+    if (!server || !bytes) { supr::set_error("null argument"); return SUPR_EINVAL; }
+    *bytes = server->s.pool.device_bytes;
+    return SUPR_OK;
+}
+
+int supr_server_submit(supr_server* server, const supr_lp* unscaled_form, const supr_lp* scaled_form,
+                       const double* Dr, const double* Dc, int64_t* ticket) {
+    using namespace supr;
+    if (!server || !unscaled_form || !scaled_form || !ticket) {
+        set_error("supr_server_submit: null argument");
+        return SUPR_EINVAL;
+    }
+    if (!check_instance(*unscaled_form, *scaled_form, Dr, Dc, "supr_server_submit")) return SUPR_EINVAL;
+    try {
+        *ticket = server->s.submit(*unscaled_form, *scaled_form, Dr, Dc);
+        return SUPR_OK;
+    } catch (const ArgError& e) {
+        set_error(e.what());
+        return SUPR_EINVAL;
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return SUPR_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SUPR_EINTERNAL;
+    }
+}
+
+int supr_server_step(supr_server* server, int32_t* running, int32_t* queued, int32_t* finished) {
+    using namespace supr;
+    if (!server || !running || !queued || !finished) { set_error("supr_server_step: null argument"); return SUPR_EINVAL; }
+    try {
+        BATCH_HIP(hipSetDevice(server->s.pool.device));
+        server->s.step(running, queued, finished);
+        return SUPR_OK;
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return SUPR_EDEVICE;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SUPR_EINTERNAL;
+    }
+}
+
+int supr_server_result(supr_server* server, int64_t ticket, int32_t* m, int32_t* n, double* x_out,
+                       double* y_out, supr_pdlp_info* info, int32_t* ready) {
+    using namespace supr;
+    if (!server || !ready) { set_error("supr_server_result: null argument"); return SUPR_EINVAL; }
+    std::lock_guard<std::mutex> lock(server->s.mu);
+    auto it = server->s.done.find(ticket);
+    *ready = it != server->s.done.end() ? 1 : 0;
+    if (!*ready) return SUPR_OK;
+    const Finished& f = it->second;
+    if (m) *m = f.m;
+    if (n) *n = f.n;
+    if (!x_out && !y_out && !info) return SUPR_OK;   // size query: the result stays
+    if (x_out) std::copy(f.x.begin(), f.x.end(), x_out);
+    if (y_out) std::copy(f.y.begin(), f.y.end(), y_out);
+    if (info) *info = f.info;
+    server->s.done.erase(it);
+    return SUPR_OK;
+}
+
+void supr_server_destroy(supr_server* server) {
+    if (!server) return;
+    (void)hipSetDevice(server->s.pool.device);
+    delete server;
 }
 
 }  // extern "C"

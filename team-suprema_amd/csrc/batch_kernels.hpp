@@ -623,4 +623,17 @@ __global__ __launch_bounds__(kBlock) void kb_cold_ax(BatchGeo g, BatchVals V, It
     });
 }
 
+// Initial states of newly started instances: S[2k] = S[2k+1] = src[r] for
+// k = slots[r].
+__global__ __launch_bounds__(kBlock) void kb_put_states(DevState* __restrict__ S,
+                                                        const DevState* __restrict__ src,
+                                                        const int32_t* __restrict__ slots,
+                                                        int count) {
+    for (int r = blockIdx.x * kBlock + threadIdx.x; r < count; r += gridDim.x * kBlock) {
+        const DevState s = src[r];
+        S[2 * slots[r]] = s;
+        S[2 * slots[r] + 1] = s;
+    }
+}
+
 }  // namespace supr

@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "Engine", "BatchEngine", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
+    "extract_solution", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -350,6 +350,84 @@ class BatchEngine:
     def close(self):
         if self._h:
             lib().supr_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching LP server (supr_server_*).
+
+    submit(unscaled_form, scaled_form, Dr, Dc) -> ticket (any thread);
+    step() -> (running, queued, finished_total) retires, admits and runs
+    replays_per_step graph replays; result(ticket) -> PdlpResult or None.
+    Capacities (max_cols, max_rows, max_nnz, max_instances) bound what is
+    resident at once; a request that does not fit waits in the queue."""
+
+    def __init__(self, mode=MODE_MIXED, tol=1e-6, max_iterations=200000, check_every=64,
+                 max_cols=1 << 24, max_rows=1 << 23, max_nnz=1 << 27, max_instances=256,
+                 max_blocks=256, lanes_bias=1.0, replays_per_step=1, **options):
+        so = _lib.SuprServerOptions()
+        lib().supr_server_default_options(ctypes.byref(so))
+        so.pdlp = _options(mode, max_iterations, check_every, tol, **options)
+        so.max_cols, so.max_rows, so.max_nnz = int(max_cols), int(max_rows), int(max_nnz)
+        so.max_instances, so.max_blocks = int(max_instances), int(max_blocks)
+        so.lanes_bias, so.replays_per_step = float(lanes_bias), int(replays_per_step)
+        h = ctypes.c_void_p()
+        check(lib().supr_server_create(ctypes.byref(so), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def device_bytes(self):
+        b = ctypes.c_int64()
+        check(lib().supr_server_device_bytes(self._h, ctypes.byref(b)))
+        return int(b.value)
+
+    @staticmethod
+    def request_bytes(m, n, nnz, mode=MODE_MIXED):
+        return int(lib().supr_server_request_bytes(int(m), int(n), int(nnz), int(mode)))
+
+    def submit(self, unscaled_form, scaled_form, Dr, Dc):
+        Dr = np.ascontiguousarray(Dr, dtype=_f64)
+        Dc = np.ascontiguousarray(Dc, dtype=_f64)
+        if Dr.shape != (scaled_form.m,) or Dc.shape != (scaled_form.n,):
+            raise ValueError("Dr must have length m and Dc length n")
+        U, S = unscaled_form.cstruct(), scaled_form.cstruct()
+        t = ctypes.c_int64()
+        check(lib().supr_server_submit(self._h, ctypes.byref(U), ctypes.byref(S), _dp(Dr), _dp(Dc),
+                                       ctypes.byref(t)))
+        return int(t.value)
+
+    def step(self):
+        r, q, f = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().supr_server_step(self._h, ctypes.byref(r), ctypes.byref(q), ctypes.byref(f)))
+        return int(r.value), int(q.value), int(f.value)
+
+    def result(self, ticket):
+        ready, m, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().supr_server_result(self._h, int(ticket), ctypes.byref(m), ctypes.byref(n), None,
+                                       None, None, ctypes.byref(ready)))
+        if not ready.value:
+            return None
+        x, y = np.zeros(n.value), np.zeros(m.value)
+        info = _lib.SuprInfo()
+        check(lib().supr_server_result(self._h, int(ticket), None, None, _dp(x), _dp(y),
+                                       ctypes.byref(info), ctypes.byref(ready)))
+        return _result(info, x, y)
+
+    def close(self):
+        if self._h:
+            lib().supr_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __enter__(self):

@@ -59,7 +59,17 @@ class SuprInfo(ctypes.Structure):
 STOP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
 
 # (name, restype, argtypes) of every entry point declared in suprema_pdlp.h
+class SuprServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("pdlp", SuprOptions), ("max_cols", ctypes.c_int64), ("max_rows", ctypes.c_int64),
+        ("max_nnz", ctypes.c_int64), ("max_instances", ctypes.c_int32),
+        ("max_blocks", ctypes.c_int32), ("lanes_bias", ctypes.c_double),
+        ("replays_per_step", ctypes.c_int32),
+    ]
+
+
 _LP = ctypes.POINTER(SuprLp)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
 SIGNATURES = [
     ("supr_last_error", ctypes.c_char_p, []),
     ("supr_version", ctypes.c_char_p, []),
@@ -100,6 +110,19 @@ SIGNATURES = [
      [ctypes.c_void_p, c_int32_p, ctypes.c_int32, ctypes.POINTER(c_double_p),
       ctypes.POINTER(c_double_p), ctypes.POINTER(SuprInfo), c_double_p]),
     ("supr_batch_destroy", None, [ctypes.c_void_p]),
+    ("supr_server_default_options", None, [ctypes.POINTER(SuprServerOptions)]),
+    ("supr_server_request_bytes", ctypes.c_int64,
+     [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("supr_server_create", ctypes.c_int,
+     [ctypes.POINTER(SuprServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    ("supr_server_device_bytes", ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
+    ("supr_server_submit", ctypes.c_int,
+     [ctypes.c_void_p, _LP, _LP, c_double_p, c_double_p, c_int64_p]),
+    ("supr_server_step", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p]),
+    ("supr_server_result", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, c_int32_p, c_int32_p, c_double_p, c_double_p,
+      ctypes.POINTER(SuprInfo), c_int32_p]),
+    ("supr_server_destroy", None, [ctypes.c_void_p]),
     ("supr_gen_blend", ctypes.c_int,
      [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_int32_p, c_int32_p] + [c_double_p] * 6),
     ("supr_gen_blend_sizes", ctypes.c_int,
