@@ -14,7 +14,18 @@ Besides the headline (mixed FP32-iterate / FP64-residual engine) it reports:
   * the two iteration kernels' duration (HIP events, standalone launches at
     this LP's shapes) and achieved bandwidth from algorithmic bytes;
   * the reference's own CPU PDLP (oracle/_ref, 1 thread) on a bounded prefix of
-    the same solve, extrapolated with the reference's full iteration count.
+    the same solve, extrapolated with the reference's full iteration count;
+  * "batched": 512 blending LPs (20-400 components x 10-200 products,
+    log-uniform sizes, n 230 to 73,117) solved together in the mixed mode
+    with the tuned layout, every result verified; solves/s against
+    sequential plain-FP64 and sequential resident-mixed solves of a
+    stratified sample of the same instances;
+  * "serving": an open-loop Poisson stream of random LPs (6 nonzeros per
+    column, m/n in [0.2, 0.8], nnz log-uniform) through the continuous-
+    batching server, every result verified; p50/p99 latency and solves/s,
+    against one-at-a-time plain FP64 solves of the same requests (their
+    latency from a FIFO queue over the same arrivals);
+  * with --large: the power-law LP (m = 500,000, n = 2,000,000, 35.1M nnz).
 """
 import argparse
 import json
@@ -90,6 +101,170 @@ def cpu_baseline(lp, scaled, Dr, Dc, sample_attempts):
     }
 
 
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a, dtype=float), q))
+
+
+def bench_batched(count=512, sample=8):
+    """Batched blending LPs (mixed mode, tuned layout) against sequential
+    single-LP solves of a stratified sample of the same instances."""
+    sizes = P.gen_blend_sizes(count, seed=2026)
+    lps, forms = [], []
+    for i, (K, Pp) in enumerate(sizes):
+        lp = P.gen_blend(K, Pp, seed=10000 + i)
+        form, meta = P.build_pdlp_form(lp)
+        s, Dr, Dc = P.scale(form)
+        lps.append((lp, meta))
+        forms.append((form, s, Dr, Dc))
+    t0 = time.perf_counter()
+    eng = P.BatchEngine(forms, tol=TOL, mode=P.MODE_MIXED, tune=True)
+    setup = time.perf_counter() - t0
+    res, secs = eng.solve()
+    cfg = eng.config
+    eng.close()
+    worst = 0.0
+    for (lp, meta), r in zip(lps, res):
+        x, y = P.extract_solution(meta, r.x, r.y)
+        rep = P.verify(lp, x, y)
+        worst = max(worst, rep.eps_P, rep.eps_D, rep.eps_G)
+    conv = sum(r.converged for r in res)
+    if conv != count or worst > TOL:
+        raise SystemExit(f"batched: {conv}/{count} converged, worst verified eps {worst:.3e}")
+    order = np.argsort([f[0].nnz for f in forms])
+    pick = [int(order[i]) for i in range(count // (2 * sample), count, count // sample)]
+    seq = {}
+    for label, mode in (("plain_fp64", P.MODE_PLAIN_FP64), ("resident_mixed", P.MODE_MIXED)):
+        ts = []
+        for i in pick:
+            f, s, Dr, Dc = forms[i]
+            t0 = time.perf_counter()
+            r = P.solve_pdhg_gpu(f, s, Dr, Dc, tol=TOL, mode=mode)
+            ts.append(time.perf_counter() - t0)
+            if not r.converged:
+                raise SystemExit(f"batched baseline {label}: instance {i} did not converge")
+        seq[label] = dict(mean_seconds=float(np.mean(ts)), solves_per_s=1.0 / float(np.mean(ts)))
+    sps = count / secs
+    log(f"batched: {count} LPs in {secs:.3f} s = {sps:.1f} solves/s; sequential plain "
+        f"{seq['plain_fp64']['solves_per_s']:.2f}/s, resident mixed {seq['resident_mixed']['solves_per_s']:.2f}/s")
+    return {
+        "workload": f"{count} blending LPs (K 20-400 x P 10-200 log-uniform, seed 2026), mixed mode, tol 1e-6, "
+                    "every result verified on the original LP",
+        "instances": count, "total_nnz": int(sum(f[0].nnz for f in forms)),
+        "seconds": secs, "solves_per_s": sps, "setup_seconds": setup,
+        "layout": {"lanes_bias": cfg[0], "max_blocks": cfg[1]},
+        "attempts_max": int(max(r.iterations for r in res)),
+        "attempts_median": int(np.median([r.iterations for r in res])),
+        "worst_verified_eps": worst,
+        "sequential_sample": {"instances": pick, **seq},
+        "speedup_vs_sequential_plain": sps / seq["plain_fp64"]["solves_per_s"],
+        "speedup_vs_sequential_resident_mixed": sps / seq["resident_mixed"]["solves_per_s"],
+    }
+
+
+def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7):
+    """Open-loop Poisson stream through the server against one-at-a-time
+    plain FP64 solves of the same requests."""
+    import threading
+    rng = np.random.default_rng(seed)
+    nnzs = np.exp(rng.uniform(np.log(1e4), np.log(nnz_hi), count))
+    ratios = rng.uniform(0.2, 0.8, count)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
+    reqs = []
+    for i in range(count):
+        n = max(20, int(nnzs[i] / 6))
+        m = max(6, int(ratios[i] * n))
+        lp = P.gen_random_lp(m, n, 6, seed=1000 + i)
+        s, Dr, Dc = P.scale(lp)
+        reqs.append((lp, s, Dr, Dc))
+    caps = dict(max_cols=1 << 24, max_rows=1 << 24, max_nnz=1 << 27, max_instances=64, max_blocks=1024)
+
+    def serve(open_loop):
+        with P.Server(mode=P.MODE_MIXED, tol=TOL, **caps) as srv:
+            tickets, done, res = [None] * count, [None] * count, [None] * count
+            lock = threading.Lock()
+            t0 = time.perf_counter()
+
+            def submitter():
+                for i in range(count):
+                    if open_loop:
+                        d = t0 + arrivals[i] - time.perf_counter()
+                        if d > 0:
+                            time.sleep(d)
+                    tk = srv.submit(*reqs[i])
+                    with lock:
+                        tickets[i] = tk
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            nd = 0
+            while nd < count:
+                srv.step()
+                with lock:
+                    tk = list(tickets)
+                for i in range(count):
+                    if tk[i] is not None and done[i] is None:
+                        r = srv.result(tk[i])
+                        if r is not None:
+                            done[i] = time.perf_counter() - t0
+                            res[i] = r
+                            nd += 1
+            th.join()
+            dev = srv.device_bytes
+        worst = 0.0
+        for i in range(count):
+            rep = P.verify(reqs[i][0], res[i].x, res[i].y)
+            worst = max(worst, rep.eps_P, rep.eps_D, rep.eps_G)
+        if not all(r.converged for r in res) or worst > TOL:
+            raise SystemExit(f"serving: not all requests verified (worst eps {worst:.3e})")
+        done = np.array(done)
+        lat = done - (arrivals if open_loop else 0.0)
+        return dict(makespan=float(done.max()), solves_per_s=count / float(done.max()),
+                    latency_p50=_pct(lat, 50), latency_p99=_pct(lat, 99),
+                    worst_verified_eps=worst, device_bytes=dev)
+
+    sat = serve(False)
+    ol = serve(True)
+    svc = []
+    for i in range(count):
+        t0 = time.perf_counter()
+        r = P.solve_pdhg_gpu(*reqs[i], tol=TOL, mode=P.MODE_PLAIN_FP64)
+        svc.append(time.perf_counter() - t0)
+        if not r.converged:
+            raise SystemExit(f"serving baseline: request {i} did not converge")
+    end, lat = 0.0, []
+    for i in range(count):
+        end = max(arrivals[i], end) + svc[i]
+        lat.append(end - arrivals[i])
+    plain = dict(total_seconds=float(np.sum(svc)), solves_per_s=count / float(np.sum(svc)),
+                 latency_p50=_pct(lat, 50), latency_p99=_pct(lat, 99))
+    log(f"serving: saturated {sat['solves_per_s']:.2f} solves/s; open loop {qps} qps p50 "
+        f"{ol['latency_p50']:.2f} s p99 {ol['latency_p99']:.2f} s; plain one-at-a-time "
+        f"{plain['solves_per_s']:.3f} solves/s, p50 {plain['latency_p50']:.1f} s p99 {plain['latency_p99']:.1f} s")
+    return {
+        "workload": f"{count} random LPs (6 nnz/column, m/n U[0.2,0.8], nnz log-uniform [1e4, {nnz_hi:.0e}], "
+                    f"seed {seed}), Poisson arrivals at {qps} qps, mixed mode, tol 1e-6, every result verified",
+        "total_nnz": int(sum(r[0].nnz for r in reqs)),
+        "saturated": sat, "open_loop": ol, "plain_fp64_one_at_a_time": plain,
+        "speedup_solves_per_s_vs_plain": sat["solves_per_s"] / plain["solves_per_s"],
+    }
+
+
+def bench_large():
+    """The power-law LP: mixed mode to 1e-6 (verified), kernel timing."""
+    lp = P.gen_power_law_lp(500000, 2000000, seed=2026)
+    s, Dr, Dc = P.scale(lp)
+    kb = P.bench_kernels(lp, s, Dr, Dc, reps=50)
+    eng = P.Engine(lp, s, Dr, Dc, tol=TOL, mode=P.MODE_MIXED, max_iterations=400000, verbose=2)
+    r = eng.solve()
+    eng.close()
+    rep = P.verify(lp, r.x, r.y)
+    return {"workload": f"power-law LP m=500000 n=2000000 nnz={lp.nnz} (Zipf 1.5 on [2,200] column degrees, "
+                        "magnitudes log-uniform [1e-3,1e3]), mixed mode, tol 1e-6",
+            "converged": r.converged, "attempts": r.iterations, "fp32_attempts": r.info["iterations_fp32"],
+            "seconds_solve": r.info["seconds_solve"], "verified_eps": [rep.eps_P, rep.eps_D, rep.eps_G],
+            "objective": rep.objective, "kernels_ms": kb}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +273,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256, help="attempts of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-full", action="store_true",
+                    help="serving stream over the full nnz range [1e4, 2e7] (long)")
+    ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone iteration-kernel timing (for profiling it)")
     args = ap.parse_args()
@@ -178,6 +358,13 @@ def main():
         log(f"CPU baseline: reference CPU PDLP, {args.cpu_sample} attempts")
         cpu = cpu_baseline(lp, scaled, Dr, Dc, args.cpu_sample)
 
+    batched = None if args.no_batched else bench_batched()
+    serving = None
+    if not args.no_serving:
+        serving = (bench_serving(count=16, qps=0.5, nnz_hi=2e7) if args.serving_full
+                   else bench_serving())
+    large = bench_large() if args.large else None
+
     head = results["mixed"]
     best_name = min(results, key=lambda k: results[k]["seconds"])
     out = {
@@ -220,6 +407,9 @@ def main():
         },
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
+        "large": large,
     }
     print(json.dumps(out))
 

@@ -111,7 +111,7 @@ typedef struct supr_pdlp_options {
     int32_t spmv_lanes_A;      /* lanes per row for A (0: autotune; 1..64 powers of two, or 256 = one block per row) */
     int32_t spmv_lanes_AT;     /* lanes per row for A^T (0: autotune) */
     int32_t device;            /* HIP device ordinal */
-    int32_t verbose;           /* 0 quiet, 1 phase summary on stderr */
+    int32_t verbose;           /* 0 quiet, 1 phase summary on stderr, 2 also a progress line every 5 s */
 } supr_pdlp_options;
 
 /* Fills the reference's defaults: mode MIXED, max_iterations 200000,

@@ -823,6 +823,7 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
         SUPR_HIP(hipHostMalloc((void**)&ring, sizeof(DevState) * R, hipHostMallocDefault));
         for (auto& e : ev) SUPR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         int last_attempts = -1, last_checks = -1, idle = 0;
+        auto last_log = Clock::now();
         while (true) {
             if (replays >= R) {
                 const int slot = replays % R;
@@ -839,6 +840,12 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
                 }
                 last_attempts = hs.attempts;
                 last_checks = hs.checks;
+                if (o_.verbose >= 2 && seconds_since(last_log) >= 5.0) {
+                    std::fprintf(stderr, "[suprema] %s loop: attempts=%d accepted=%d checks=%d restarts=%d merit=%.3e best=%.3e\n",
+                                 prec ? "fp64" : "fp32", hs.attempts, hs.accepted, hs.checks, hs.restarts,
+                                 hs.cand_merit, hs.best_merit);
+                    last_log = Clock::now();
+                }
             }
             const int slot = replays % R;
             SUPR_HIP(hipGraphLaunch(exec, s));

@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "bench_kernels", "device_count",
+    "extract_solution", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -508,3 +508,24 @@ def gen_random_lp(m, n, col_nnz=6, seed=42, x0_hi=5.0, u_hi=10.0, mag_decades=0,
                                    _ip(ri), _ip(ci), _dp(v), _dp(c), _dp(rL), _dp(rU), _dp(l),
                                    _dp(u)))
     return LP(m, n, ri, ci, v, c, rL, rU, l, u, 0.0, f"random_{m}x{n}")
+
+
+def zipf_degrees(n, s=1.5, k_min=2, k_max=200, seed=42):
+    """Column degrees with P(k) proportional to k**-s on k_min..k_max (the
+    power law truncated to the range), drawn by inverse CDF from a seeded
+    numpy PCG64 stream.  For s = 1.5 on [2, 200] the mean is about 17.6."""
+    k = np.arange(k_min, k_max + 1, dtype=np.float64)
+    cdf = np.cumsum(k ** -s)
+    cdf /= cdf[-1]
+    u = np.random.default_rng(seed).random(n)
+    return (np.searchsorted(cdf, u, side="right") + k_min).astype(_i32)
+
+
+def gen_power_law_lp(m=500000, n=2000000, seed=42):
+    """The large random LP: power-law column degrees (zipf_degrees), values
+    of log-uniform magnitude in [1e-3, 1e3] with random sign, box bounds
+    [0, U[1, 100]], feasible and bounded as gen_random_lp."""
+    deg = np.minimum(zipf_degrees(n, seed=seed), m)
+    lp = gen_random_lp(m, n, 0, seed=seed, u_hi=-100.0, mag_decades=3, col_deg=deg)
+    lp.name = f"power_law_{m}x{n}"
+    return lp

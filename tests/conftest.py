@@ -54,6 +54,10 @@ def golden_lp(case):
     g = case["gen"]
     if g["kind"] == "random":
         return P.gen_random_lp(g["m"], g["n"], g["col_nnz"], seed=g["seed"])
+    if g["kind"] == "power_law":
+        return P.gen_power_law_lp(g["m"], g["n"], seed=g["seed"])
+    if g["kind"] == "blend":
+        return P.build_pdlp_form(P.gen_blend(g["K"], g["P"], seed=g["seed"]))[0]
     return P.gen_transport(g["S"], g["D"], seed=g["seed"])
 
 

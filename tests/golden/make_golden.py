@@ -97,10 +97,16 @@ def main():
         ("random_2000x5000_s42", dict(kind="random", m=2000, n=5000, col_nnz=6, seed=42)),
         ("transport_60x60_s42", dict(kind="transport", S=60, D=60, seed=42)),
         ("transport_30x40_s7", dict(kind="transport", S=30, D=40, seed=7)),
+        ("power_law_1000x4000_s7", dict(kind="power_law", m=1000, n=4000, seed=7)),
+        ("blend_25x40_s103", dict(kind="blend", K=25, P=40, seed=103)),
     ]
     for name, g in synth:
         if g["kind"] == "random":
             lp = P.gen_random_lp(g["m"], g["n"], g["col_nnz"], seed=g["seed"])
+        elif g["kind"] == "power_law":
+            lp = P.gen_power_law_lp(g["m"], g["n"], seed=g["seed"])
+        elif g["kind"] == "blend":
+            lp, _ = P.build_pdlp_form(P.gen_blend(g["K"], g["P"], seed=g["seed"]))
         else:
             lp = P.gen_transport(g["S"], g["D"], seed=g["seed"])
         case = {"name": name, "kind": g["kind"], "gen": g, "m": lp.m, "n": lp.n, "nnz": lp.nnz,

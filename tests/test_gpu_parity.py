@@ -88,9 +88,10 @@ def _check_against_reference(oracle, case, form, res):
 
 FAST_CASES = ["afiro", "adlittle", "blend", "sc50a", "sc50b", "sc105", "kb2", "scagr7", "israel",
               "share1b", "beaconfd", "tiny_range", "med", "illcond", "random_200x500_s1",
-              "random_2000x5000_s42", "transport_60x60_s42", "transport_30x40_s7"]
+              "random_2000x5000_s42", "transport_60x60_s42", "transport_30x40_s7",
+              "power_law_1000x4000_s7", "blend_25x40_s103"]
 PLAIN_CASES = ["afiro", "sc50a", "blend", "tiny_range", "transport_30x40_s7", "random_200x500_s1",
-               "med"]
+               "med", "power_law_1000x4000_s7"]
 
 
 @pytest.mark.parametrize("name", FAST_CASES)

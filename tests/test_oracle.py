@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import pdlp_amd as P
-from conftest import load_fixture
+from conftest import golden_lp, load_fixture
 
 
 def sha(*arrays):
@@ -40,16 +40,14 @@ def case_lp(oracle, case):
         assert [int(v) for v in dims] == case["form_dims"]
         assert fsha == case["form_sha"], "PDLP form differs from the reference's build_pdlp_form"
         return form
-    g = case["gen"]
-    if g["kind"] == "random":
-        return P.gen_random_lp(g["m"], g["n"], g["col_nnz"], seed=g["seed"])
-    return P.gen_transport(g["S"], g["D"], seed=g["seed"])
+    return golden_lp(case)
 
 
 # Small and medium golden cases run in the default CPU suite; all are quick.
 GOLDEN_CASES = ["afiro", "adlittle", "blend", "sc50a", "sc50b", "sc105", "kb2", "scagr7", "israel",
                 "share1b", "beaconfd", "tiny_range", "med", "illcond", "random_200x500_s1",
-                "random_2000x5000_s42", "transport_60x60_s42", "transport_30x40_s7"]
+                "random_2000x5000_s42", "transport_60x60_s42", "transport_30x40_s7",
+                "power_law_1000x4000_s7", "blend_25x40_s103"]
 
 
 @pytest.mark.parametrize("name", GOLDEN_CASES)
@@ -72,9 +70,7 @@ def test_generators_match_golden(golden):
     for case in golden.values():
         if case["kind"] == "mps":
             continue
-        g = case["gen"]
-        lp = (P.gen_random_lp(g["m"], g["n"], g["col_nnz"], seed=g["seed"]) if g["kind"] == "random"
-              else P.gen_transport(g["S"], g["D"], seed=g["seed"]))
+        lp = golden_lp(case)
         assert sha(lp.row, lp.col, lp.val, lp.c, lp.rL, lp.rU, lp.l, lp.u) == case["lp_sha"]
 
 
