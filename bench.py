@@ -101,6 +101,19 @@ def cpu_baseline(lp, scaled, Dr, Dc, sample_attempts):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_transport_1000x1000.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        k = json.loads(block)["kernels"][kernel]
+        return k["read_bytes"] + k["write_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def _pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=float), q))
 
@@ -278,6 +291,10 @@ def main():
     ap.add_argument("--serving-full", action="store_true",
                     help="serving stream over the full nnz range [1e4, 2e7] (long)")
     ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
+    ap.add_argument("--only", choices=["batched", "serving", "large"],
+                    help="run only this section and print its JSON")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a plain device copy of known size (reference for HBM counters)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone iteration-kernel timing (for profiling it)")
     args = ap.parse_args()
@@ -289,6 +306,18 @@ def main():
     if P.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        ms = P.bench_copy(nbytes, reps=10)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 11,
+                          "ms_per_copy": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
+
+    if args.only:
+        fn = {"batched": bench_batched, "serving": bench_serving, "large": bench_large}[args.only]
+        print(json.dumps({args.only: fn()}))
+        return
 
     log("generating transport LP 1000x1000 and scaling (host, untimed)")
     lp = P.gen_transport(1000, 1000, seed=42)
@@ -401,7 +430,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": dom["frac"],
-            "traffic": None,
+            "traffic": measured_traffic(dom_name),
+            "traffic_source": "profiles/hbm_traffic_transport_1000x1000.md (rocprofv3 FETCH_SIZE + "
+                              "WRITE_SIZE per dispatch, copy-calibrated)",
             "algorithmic_bytes_per_launch": dom["bytes"],
             "ms_per_launch": dom["ms"],
         },

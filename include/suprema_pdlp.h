@@ -39,6 +39,8 @@
  *   supr_bench_kernels    no reference counterpart: times the engine's two
  *                         iteration kernels at a problem's real shapes (callers:
  *                         benchmarks reporting achieved HBM bandwidth)
+ *   supr_bench_copy       no reference counterpart: a plain device copy (callers:
+ *                         benchmarks calibrating HBM counters and bandwidth)
  *   supr_last_error       the what() text of the reference's std::runtime_error
  *                         (SOVEREIGN_CUDA_CHECK, gpu/pdlp_gpu_solve.cu:32-38)
  *
@@ -177,6 +179,11 @@ int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32
 int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                        const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                        int32_t reps, double* out8);
+
+/* Times a plain device-to-device copy of `bytes` (a positive multiple of
+ * 16) with a 16-byte-vector stream kernel, reps launches after one warm-up;
+ * *ms_per_copy = ms per launch (reads `bytes`, writes `bytes`). */
+int supr_bench_copy(int64_t bytes, int32_t reps, int32_t device, double* ms_per_copy);
 
 /* ------------------------------------------------------- host-side form -- */
 /* verify(): out7 = eps_P, eps_D, eps_G, objective, dual_objective,

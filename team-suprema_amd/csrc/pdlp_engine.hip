@@ -1225,4 +1225,45 @@ int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
     }
 }
 
+// Times a plain device copy of `bytes` (multiple of 16) reps times.
+int supr_bench_copy(int64_t bytes, int32_t reps, int32_t device, double* ms_per_copy) {
+    using namespace supr;
+    if (bytes < 16 || bytes % 16 || reps < 1 || !ms_per_copy) {
+        set_error("supr_bench_copy: bytes must be a positive multiple of 16, reps >= 1");
+        return SUPR_EINVAL;
+    }
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        SUPR_HIP(hipSetDevice(device));
+        SUPR_HIP(hipMalloc(&src, bytes));
+        SUPR_HIP(hipMalloc(&dst, bytes));
+        SUPR_HIP(hipMemset(src, 0, bytes));
+        SUPR_HIP(hipEventCreate(&e0));
+        SUPR_HIP(hipEventCreate(&e1));
+        const int64_t count = bytes / 16;
+        const int grid = (int)std::min<int64_t>((count + kBlock - 1) / kBlock, 8192);
+        hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kBlock), 0, nullptr, (const float4*)src, (float4*)dst, count);
+        SUPR_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kBlock), 0, nullptr, (const float4*)src, (float4*)dst, count);
+        SUPR_HIP(hipEventRecord(e1, nullptr));
+        SUPR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SUPR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(src);
+        (void)hipFree(dst);
+        return SUPR_OK;
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (src) (void)hipFree(src);
+        if (dst) (void)hipFree(dst);
+        return map_exception();
+    }
+}
+
 }  // extern "C"

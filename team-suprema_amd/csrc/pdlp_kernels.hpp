@@ -84,6 +84,15 @@ __global__ __launch_bounds__(kBlock) void k_plain_axpy(int len, double a,
     if (j < len) y[j] += a * x[j];
 }
 
+// Plain stream copy (16-byte vectors, grid-stride): the reference point for
+// achievable bandwidth and for calibrating HBM traffic counters.
+__global__ __launch_bounds__(kBlock) void k_copy4(const float4* __restrict__ src,
+                                                  float4* __restrict__ dst, int64_t count) {
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
+         k += (int64_t)gridDim.x * kBlock)
+        dst[k] = src[k];
+}
+
 // ------------------------------------------- device-resident PDHG loop --
 //
 // Launch sequence of one graph replay (G attempts, G even):

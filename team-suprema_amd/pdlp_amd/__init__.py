@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "device_count",
+    "extract_solution", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -476,6 +476,13 @@ def bench_kernels(unscaled_form, scaled_form, Dr, Dc, reps=50, **options):
     keys = ["primal_ms_fp32", "dual_ms_fp32", "primal_ms_fp64", "dual_ms_fp64", "lanes_A",
             "lanes_AT", "ell_A", "ell_AT"]
     return dict(zip(keys, (float(v) for v in out)))
+
+
+def bench_copy(nbytes=1 << 30, reps=20, device=0):
+    """ms per launch of a plain device copy of nbytes (supr_bench_copy)."""
+    ms = ctypes.c_double()
+    check(lib().supr_bench_copy(int(nbytes), int(reps), int(device), ctypes.byref(ms)))
+    return float(ms.value)
 
 
 def gen_transport(S, D, seed=42, cost_max=100):
