@@ -343,6 +343,63 @@ __device__ __forceinline__ void for_rows(int rows, F&& body) {
     }
 }
 
+// The block's slot after the XCD-aware remap (see for_rows).
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x;
+    return (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
+                            : blockIdx.x;
+}
+
+// LDS-staged row schedule.  With L lanes per row, for_rows leaves a row's
+// epilogue (the projection, averages and stores after the row product) to
+// one lane of each group: on a matrix with many rows most of every
+// epilogue instruction's lanes are idle and its accesses are scattered.
+// Here a block takes tiles of kBlock consecutive rows: it forms the tile's
+// row products with L lanes per row (kBlock / L rows per pass, L passes),
+// stages the NA values per row in LDS, then thread t runs the epilogue of
+// the tile's row t with coalesced accesses.  Used when a matrix has at
+// least kStageRows rows (enough tiles to fill the GPU) and L >= 2.
+constexpr int kStageRows = 1 << 17;
+template <int L> __host__ __device__ __forceinline__ bool staged_rows(int rows) {
+    return L >= 2 && rows >= kStageRows;
+}
+
+template <int L, int NA, typename Acc, typename Dot, typename Epi>
+__device__ __forceinline__ void for_row_tiles(int rows, Dot&& dot, Epi&& epi) {
+    __shared__ Acc stage[NA][kBlock];
+    constexpr int gpb = L >= kBlock ? 1 : kBlock / L;   // rows per pass
+    constexpr int passes = kBlock / gpb;
+    const int g = threadIdx.x / (kBlock / gpb);
+    const int lane = threadIdx.x % (kBlock / gpb);
+    for (int t0 = xcd_block() * kBlock; t0 < rows; t0 += gridDim.x * kBlock) {
+        for (int pass = 0; pass < passes; ++pass) {
+            const int r = pass * gpb + g;
+            const int row = t0 + r;
+            Acc v[NA];
+            dot(row, lane, row < rows, v);
+            if (lane == 0) {
+#pragma unroll
+                for (int q = 0; q < NA; ++q) stage[q][r] = v[q];
+            }
+        }
+        __syncthreads();
+        const int row = t0 + threadIdx.x;
+        if (row < rows) {
+            Acc v[NA];
+#pragma unroll
+            for (int q = 0; q < NA; ++q) v[q] = stage[q][threadIdx.x];
+            epi(row, v);
+        }
+        __syncthreads();
+    }
+}
+
+// One row per thread (epilogue-only passes), XCD-remapped, grid-stride.
+template <typename F> __device__ __forceinline__ void for_rows_flat(int rows, F&& body) {
+    for (int row = xcd_block() * kBlock + threadIdx.x; row < rows; row += gridDim.x * kBlock)
+        body(row);
+}
+
 __device__ __forceinline__ double merit3(const double* e) {
     return sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
 }

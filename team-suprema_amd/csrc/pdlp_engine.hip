@@ -105,7 +105,8 @@ DevBuf device_vec(size_t len) { return DevBuf(len * sizeof(T)); }
 // Row-loop grids are rounded up to a multiple of the XCD count (the kernels'
 // XCD-aware block mapping needs it; extra blocks find no rows).
 int grid_rows(int rows, int L) {
-    const int per = L >= kBlock ? 1 : kBlock / L;
+    // staged schedules (pdlp_device.hpp) work in tiles of kBlock rows
+    const int per = (L >= 2 && rows >= kStageRows) ? kBlock : (L >= kBlock ? 1 : kBlock / L);
     long g = ((long)rows + per - 1) / per;
     g = std::max(1L, std::min<long>(g, kMaxBlocks));
     if (g >= kXcds) g = (g + kXcds - 1) / kXcds * kXcds;

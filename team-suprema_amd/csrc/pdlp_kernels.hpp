@@ -12,6 +12,13 @@ namespace supr {
 template <int L, typename V, typename X, typename Y>
 __global__ __launch_bounds__(kBlock) void k_spmv(MatView<V> M, const X* __restrict__ x,
                                                  Y* __restrict__ y) {
+    if (staged_rows<L>(M.rows)) {
+        for_row_tiles<L, 1, V>(
+            M.rows,
+            [&](int row, int lane, bool valid, V (&v)[1]) { v[0] = row_dot<L, V>(M, x, row, lane, valid); },
+            [&](int row, const V (&v)[1]) { y[row] = (Y)v[0]; });
+        return;
+    }
     for_rows<L>(M.rows, [&](int row, int lane, bool valid) {
         const V s = row_dot<L, V>(M, x, row, lane, valid);
         if (valid && lane == 0) y[row] = (Y)s;
@@ -129,24 +136,41 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
     T* __restrict__ xn = B.x[cur ^ 1];
     const T* __restrict__ y = B.y[cur];
     double acc[1] = {0.0};
-    for_rows<L>(AT.rows, [&](int row, int lane, bool valid) {
-        T a = T(0);
+    auto epilogue = [&](int row, T a) {
+        const T xj = x[row];
+        if (pending) B.xsum[row] += eta_acc * xj;
+        const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
+        xn[row] = v;
+        if constexpr (sizeof(T) == 4)
+            reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
+        const double d = (double)v - (double)xj;
+        acc[0] += d * d;
+    };
+    if (staged_rows<L>(AT.rows)) {
         if (need_aty) {
-            a = row_dot<L, T>(AT, y, row, lane, valid);
-            if (valid && lane == 0) B.aty[row] = a;
+            for_row_tiles<L, 1, T>(
+                AT.rows,
+                [&](int row, int lane, bool valid, T (&v)[1]) { v[0] = row_dot<L, T>(AT, y, row, lane, valid); },
+                [&](int row, const T (&v)[1]) {
+                    B.aty[row] = v[0];
+                    epilogue(row, v[0]);
+                });
+        } else {
+            for_rows_flat(AT.rows, [&](int row) { epilogue(row, B.aty[row]); });
         }
-        if (valid && lane == 0) {
-            if (!need_aty) a = B.aty[row];
-            const T xj = x[row];
-            if (pending) B.xsum[row] += eta_acc * xj;
-            const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
-            xn[row] = v;
-            if constexpr (sizeof(T) == 4)
-                reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
-            const double d = (double)v - (double)xj;
-            acc[0] += d * d;
-        }
-    });
+    } else {
+        for_rows<L>(AT.rows, [&](int row, int lane, bool valid) {
+            T a = T(0);
+            if (need_aty) {
+                a = row_dot<L, T>(AT, y, row, lane, valid);
+                if (valid && lane == 0) B.aty[row] = a;
+            }
+            if (valid && lane == 0) {
+                if (!need_aty) a = B.aty[row];
+                epilogue(row, a);
+            }
+        });
+    }
     block_sum<1>(acc);
     if (threadIdx.x == 0) part_n[blockIdx.x] = acc[0];
 }
@@ -176,27 +200,35 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
     // FP32: A(xn - x) from the difference (see row_dot_diff); FP64: the
     // reference's 2 A xn - A x (pdlp_gpu_solve.cu:89) with A x carried.
     constexpr bool kDiff = sizeof(T) == 4;
-    for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
-        T a, ad;
+    auto dot = [&](int row, int lane, bool valid, T (&v)[2]) {
         if constexpr (kDiff) {
-            row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, a, ad);
+            row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, v[0], v[1]);
         } else {
-            a = row_dot<L, T>(A, xn, row, lane, valid);
-            ad = T(0);
+            v[0] = row_dot<L, T>(A, xn, row, lane, valid);
+            v[1] = T(0);
         }
-        if (valid && lane == 0) {
-            axn[row] = a;
-            const T yi = y[row];
-            if (pending) B.ysum[row] += eta_acc * yi;
-            if constexpr (!kDiff) ad = a - ax[row];
-            T v = yi + sigma * (B.b[row] - (a + ad));
-            if (B.ineq[row] && v < T(0)) v = T(0);
-            yn[row] = v;
-            const double d = (double)v - (double)yi;
-            acc[0] += d * d;
-            acc[1] += d * (double)ad;
-        }
-    });
+    };
+    auto epilogue = [&](int row, T a, T ad) {
+        axn[row] = a;
+        const T yi = y[row];
+        if (pending) B.ysum[row] += eta_acc * yi;
+        if constexpr (!kDiff) ad = a - ax[row];
+        T v = yi + sigma * (B.b[row] - (a + ad));
+        if (B.ineq[row] && v < T(0)) v = T(0);
+        yn[row] = v;
+        const double d = (double)v - (double)yi;
+        acc[0] += d * d;
+        acc[1] += d * (double)ad;
+    };
+    if (staged_rows<L>(A.rows)) {
+        for_row_tiles<L, 2, T>(A.rows, dot, [&](int row, const T (&v)[2]) { epilogue(row, v[0], v[1]); });
+    } else {
+        for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
+            T v[2];
+            dot(row, lane, valid, v);
+            if (valid && lane == 0) epilogue(row, v[0], v[1]);
+        });
+    }
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         part_m[(size_t)blockIdx.x * 2] = acc[0];

@@ -40,6 +40,15 @@ def _spmv_cases():
     rows, cols = np.array(rows), np.array(cols)
     cases["ragged_empty"] = P.LP(m, n, rows, cols, rng.standard_normal(len(rows)), np.zeros(n),
                                  np.zeros(m), np.zeros(m), np.zeros(n), np.zeros(n))
+    # tall enough (>= 131072 rows both ways) for the LDS-staged schedules
+    mt = nt = 140000
+    lens = rng.integers(0, 40, mt)
+    lens[::7] = 0
+    trow = np.repeat(np.arange(mt), lens)
+    tcol = rng.integers(0, nt, trow.size)
+    cases["tall_ragged_staged"] = P.LP(mt, nt, trow, tcol, rng.standard_normal(trow.size),
+                                       np.zeros(nt), np.zeros(mt), np.zeros(mt), np.zeros(nt),
+                                       np.zeros(nt))
     return cases
 
 
@@ -221,3 +230,26 @@ def test_cpp_drop_in_api():
     assert r.returncode == 0 and out["failures"] == 0, (r.stdout, r.stderr)
     assert out["converged"] == 1 and max(out["eps"]) <= 1e-6
     assert out["cancelled_converged"] == 0 and out["threw"] == 1
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_staged_schedules_solve_a_tall_lp(oracle, mode):
+    """A (140,000 rows) and A^T (200,000 rows) both run the LDS-staged
+    schedules (lanes forced to 4 and 2): the solve must reach a point the
+    oracle's verifier accepts, with the same objective in both modes."""
+    lp = P.gen_random_lp(140000, 200000, 6, seed=5)
+    s, Dr, Dc = P.scale(lp)
+    res = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, max_iterations=500000, mode=mode,
+                           spmv_lanes_A=4, spmv_lanes_AT=2)
+    assert res.converged and (res.info["lanes_A"], res.info["lanes_AT"]) == (4, 2)
+    rep = oracle.verify(lp, res.x, res.y)
+    assert max(rep[0], rep[1], rep[2]) <= TOL, rep[:3]
+    np.testing.assert_allclose([res.eps_P, res.eps_D, res.eps_G], rep[:3], rtol=1e-6, atol=1e-12)
+    STAGED_OBJ.setdefault(mode, rep[3])
+    if len(STAGED_OBJ) == 2:
+        a, b = STAGED_OBJ.values()
+        assert abs(a - b) <= OBJ_RTOL * (1 + abs(b))
+
+
+STAGED_OBJ = {}
+
