@@ -113,7 +113,7 @@ typedef struct supr_pdlp_options {
     int32_t spmv_lanes_A;      /* lanes per row for A (0: autotune; 1..64 powers of two, or 256 = one block per row) */
     int32_t spmv_lanes_AT;     /* lanes per row for A^T (0: autotune) */
     int32_t device;            /* HIP device ordinal */
-    int32_t verbose;           /* 0 quiet, 1 phase summary on stderr, 2 also a progress line every 5 s */
+    int32_t verbose;           /* 0 quiet, 1 phase summary on stderr, 2 also a progress line every 5 s, 3 also per-kernel timing (batched) */
 } supr_pdlp_options;
 
 /* Fills the reference's defaults: mode MIXED, max_iterations 200000,
@@ -237,9 +237,9 @@ int supr_gen_random_lp(int32_t m, int32_t n, int32_t col_nnz, const int32_t* col
  * instance with its own step sizes, checks, restarts and (MIXED) FP32 -> FP64
  * hand-over, so an instance's result does not depend on the other instances
  * of the solve.  Modes: MIXED (default) or FAST_FP64.  opt->spmv_lanes_A == 0
- * tunes the lanes-per-row rule (lanes_bias) and the per-instance block cap
- * (max_blocks) on a short run at creation; otherwise supr_batch_configure
- * sets them (defaults 1.0 and 32).  x_out[r] / y_out[r] (may be NULL) receive
+ * tunes the lanes-per-row rule (lanes_bias) on a short run at creation;
+ * supr_batch_configure sets it and the per-instance block cap (max_blocks)
+ * explicitly (defaults 1.0 and 512).  x_out[r] / y_out[r] (may be NULL) receive
  * the r-th requested instance's point, infos[r] its information; *seconds
  * (may be NULL) the solve's wall time on the device.  (No reference
  * counterpart; callers: batch drivers and request loops.) */
@@ -277,7 +277,7 @@ typedef struct supr_server_options {
     int64_t max_rows;          /* row arena: sum of m + 1 */
     int64_t max_nnz;           /* nonzero arena: sum of nnz */
     int32_t max_instances;     /* resident requests */
-    int32_t max_blocks;        /* blocks per request and side of a launch (0: 256) */
+    int32_t max_blocks;        /* blocks per request and side of a launch (0: 1024) */
     double lanes_bias;         /* lanes-per-row rule (0: 1.0) */
     int32_t replays_per_step;  /* graph replays per supr_server_step (0: 1) */
 } supr_server_options;

@@ -363,23 +363,32 @@ struct Pool {
         set_config(b, c);
     }
 
-    // Lanes rule and per-instance block cap (only while no instance runs);
-    // partial-sum and table storage for Scap instances at this cap.
+    // Lanes rule and per-instance block cap (only while no instance runs).
+    // Partial-sum and table storage: the resident instances' block counts
+    // plus a full cap for every free slot.
     void set_config(double b, int c) {
-        const int64_t per_side = (int64_t)Scap * c;
-        if (per_side > (int64_t)INT32_MAX / 16) throw ArgError("too many blocks for one launch");
-        double* p = dparts.alloc<double>((size_t)19 * per_side);
-        part_n = p; p += per_side;
-        part_kn = p; p += 8 * per_side;
-        part_rn = p; p += per_side;
-        part_m = p; p += 2 * per_side;
-        part_km = p; p += 6 * per_side;
-        part_rm = p;
-        dtabn.alloc<int32_t>(per_side);
-        dtabm.alloc<int32_t>(per_side);
         bias = b;
         cap = c;
-        for (int k = 0; k < Scap; ++k) if (inst[k].n > 0) shape_rule(k);
+        int64_t need_n = 0, need_m = 0, used = 0;
+        for (int k = 0; k < Scap; ++k) {
+            if (inst[k].n == 0) continue;
+            shape_rule(k);
+            need_n += inst[k].bn;
+            need_m += inst[k].bm;
+            ++used;
+        }
+        need_n += (Scap - used) * (int64_t)c;
+        need_m += (Scap - used) * (int64_t)c;
+        if (need_n + need_m > (int64_t)INT32_MAX / 16) throw ArgError("too many blocks for one launch");
+        double* p = dparts.alloc<double>((size_t)(10 * need_n + 9 * need_m));
+        part_n = p; p += need_n;
+        part_kn = p; p += 8 * need_n;
+        part_rn = p; p += need_n;
+        part_m = p; p += 2 * need_m;
+        part_km = p; p += 6 * need_m;
+        part_rm = p;
+        dtabn.alloc<int32_t>(need_n);
+        dtabm.alloc<int32_t>(need_m);
     }
 
     void shape_rule(int k) {
@@ -531,15 +540,20 @@ struct Pool {
         BATCH_HIP(hipStreamSynchronize(st));
     }
 
-    void enqueue_replay(const BatchGeo& g, int live, int G) {
+    // marks (diagnostics only, never while capturing): 2G + 2 events, one
+    // before each attempt launch, one after the last, one after the checks.
+    void enqueue_replay(const BatchGeo& g, int live, int G, hipEvent_t* marks = nullptr) {
         hipStream_t st = stream;
         DevState* S = dstate.get<DevState>();
         double* dx2 = ddx2.get<double>();
         for (int a = 0; a < G; ++a) {
             const int p = a & 1;
+            if (marks) BATCH_HIP(hipEventRecord(marks[2 * a], st));
             hipLaunchKernelGGL(kb_primal, dim3(g.BN), dim3(kBlock), 0, st, S, p, g, V, F, D, part_m, dx2, part_n);
+            if (marks) BATCH_HIP(hipEventRecord(marks[2 * a + 1], st));
             hipLaunchKernelGGL(kb_dual, dim3(g.BM), dim3(kBlock), 0, st, S, p ^ 1, g, V, F, D, part_n, dx2, part_m);
         }
+        if (marks) BATCH_HIP(hipEventRecord(marks[2 * G], st));
         hipLaunchKernelGGL(kb_check_prep, dim3(g.BN + g.BM), dim3(kBlock), 0, st, S, g, F, D, K, part_m, dx2);
         hipLaunchKernelGGL(kb_check_kkt, dim3(g.BN + g.BM), dim3(kBlock), 0, st, S, g, V, K, part_kn, part_km);
         hipLaunchKernelGGL(kb_kkt_fin, dim3(live), dim3(kBlock), 0, st, S, g, part_kn, part_km);
@@ -548,7 +562,29 @@ struct Pool {
             hipLaunchKernelGGL(kb_switch_products, dim3(g.BN + g.BM), dim3(kBlock), 0, st, S, g, V, D);
         hipLaunchKernelGGL(kb_restart_fin, dim3(live), dim3(kBlock), 0, st, S, g, part_rn, part_rm);
         hipLaunchKernelGGL(kb_status, dim3(1), dim3(kBlock), 0, st, S, g, live, dstatus.get<double>());
+        if (marks) BATCH_HIP(hipEventRecord(marks[2 * G + 1], st));
         BATCH_HIP(hipGetLastError());
+    }
+
+    // One replay launched kernel by kernel with events between the launches
+    // (verbose >= 3): prints where a replay's time goes.  It is a real
+    // replay: the instances advance exactly as under the graph.
+    void profile_replay(const BatchGeo& g, int live, int G) {
+        std::vector<hipEvent_t> ev(2 * G + 2, nullptr);
+        for (auto& e : ev) BATCH_HIP(hipEventCreate(&e));
+        enqueue_replay(g, live, G, ev.data());
+        BATCH_HIP(hipStreamSynchronize(stream));
+        float prim = 0.f, dual = 0.f, check = 0.f, t = 0.f;
+        for (int a = 0; a < G; ++a) {
+            BATCH_HIP(hipEventElapsedTime(&t, ev[2 * a], ev[2 * a + 1]));
+            prim += t;
+            BATCH_HIP(hipEventElapsedTime(&t, ev[2 * a + 1], ev[2 * a + 2]));
+            dual += t;
+        }
+        BATCH_HIP(hipEventElapsedTime(&check, ev[2 * G], ev[2 * G + 1]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        std::fprintf(stderr, "[suprema] batch profile: %d running, blocks %d + %d: primal %.1f us, dual %.1f us per attempt; check group %.1f us\n",
+                     live, g.BN, g.BM, 1e3 * prim / G, 1e3 * dual / G, 1e3 * check);
     }
 
     hipGraphExec_t capture(const BatchGeo& g, int live, int G) {
@@ -609,6 +645,10 @@ struct Pool {
 
 // ------------------------------------------------------------ the batch --
 
+// Per-instance block cap of the batched engine: large enough that the
+// largest instances, which finish last, are not starved of blocks.
+constexpr int kBatchCap = 512;
+
 struct Batch {
     Pool pool;
     int count = 0;
@@ -620,12 +660,13 @@ struct Batch {
         const bool mixed = o.mode != SUPR_MODE_FAST_FP64;
         int64_t N = 0, M = 0, Z = 0;
         for (int k = 0; k < cnt; ++k) { N += S[k].n + 1; M += S[k].m + 1; Z += S[k].nnz; }
-        pool.create(N, M, Z, cnt, o, 1.0, 32);
+        pool.create(N, M, Z, cnt, o, 1.0, kBatchCap);
         slot.resize(cnt);
         for (int k = 0; k < cnt; ++k) {
             slot[k] = pool.add(prepare(U[k], S[k], Drs[k], Dcs[k], o, mixed));
             if (slot[k] < 0) throw std::runtime_error("batch layout: an instance does not fit its own pool");
         }
+        pool.set_config(pool.bias, pool.cap);   // size the partial sums exactly
     }
 
     // Solves `ids` from the cold start; states[r] is the final state of ids[r].
@@ -659,6 +700,7 @@ struct Batch {
         try {
             BATCH_HIP(hipHostMalloc((void**)&ring, sizeof(double) * 2 * R, hipHostMallocDefault));
             for (auto& e : ev) BATCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            if (pool.opt.verbose >= 3) pool.profile_replay(g, (int)live.size(), G);
             exec = pool.capture(g, (int)live.size(), G);
             double last_progress = -1.0;
             int idle = 0, replays = 0, inflight = 0;
@@ -694,6 +736,7 @@ struct Batch {
                         (void)hipGraphExecDestroy(exec);
                         exec = nullptr;
                         g = pool.layout(live);
+                        if (pool.opt.verbose >= 3) pool.profile_replay(g, (int)live.size(), G);
                         exec = pool.capture(g, (int)live.size(), G);
                     }
                 }
@@ -718,23 +761,23 @@ struct Batch {
         return secs;
     }
 
-    // Times every (lanes bias, block cap) candidate on a short run of the
-    // whole batch and keeps the fastest.
+    // Times every lanes-rule candidate on a short run of the whole batch
+    // and keeps the fastest.  The block cap stays at its setting: it only
+    // limits the largest instances, which decide the tail of a solve that a
+    // short probe does not reach.
     void tune(int probe_iters) {
         std::vector<int32_t> all(count);
         std::iota(all.begin(), all.end(), 0);
         std::vector<DevState> scratch_states;
         double best = std::numeric_limits<double>::infinity(), best_bias = 1.0;
-        int best_cap = 32;
+        const int c = pool.cap;
         for (double b : {0.5, 1.0, 2.0}) {
-            for (int c : {8, 32, 128}) {
-                pool.set_config(b, c);
-                run(all, probe_iters, scratch_states);            // warm
-                const double t = run(all, probe_iters, scratch_states);
-                if (t < best) { best = t; best_bias = b; best_cap = c; }
-            }
+            pool.set_config(b, c);
+            run(all, probe_iters, scratch_states);            // warm
+            const double t = run(all, probe_iters, scratch_states);
+            if (t < best) { best = t; best_bias = b; }
         }
-        pool.set_config(best_bias, best_cap);
+        pool.set_config(best_bias, c);
     }
 };
 
@@ -771,7 +814,7 @@ struct Server {
 
     void create(const supr_server_options& o) {
         so = o;
-        const int cap = o.max_blocks > 0 ? o.max_blocks : 256;
+        const int cap = o.max_blocks > 0 ? o.max_blocks : 1024;
         const double bias = o.lanes_bias > 0 ? o.lanes_bias : 1.0;
         pool.create(o.max_cols, o.max_rows, o.max_nnz, o.max_instances, o.pdlp, bias, cap);
     }
@@ -1007,7 +1050,7 @@ void supr_server_default_options(supr_server_options* o) {
     o->max_rows = (int64_t)1 << 23;
     o->max_nnz = (int64_t)1 << 27;
     o->max_instances = 256;
-    o->max_blocks = 256;
+    o->max_blocks = 1024;
     o->lanes_bias = 1.0;
     o->replays_per_step = 1;
 }

@@ -285,12 +285,12 @@ class BatchEngine:
 
     forms: list of (unscaled_form, scaled_form, Dr, Dc).  solve(ids=None)
     solves the listed instances (all by default) from the cold start and
-    returns (list of PdlpResult, seconds).  tune=True times the lanes rule
-    and the per-instance block cap on a short run at creation; otherwise
-    lanes_bias and max_blocks are used."""
+    returns (list of PdlpResult, seconds).  tune=True times the lanes-per-row
+    rule on a short run at creation (block cap 512); otherwise lanes_bias and
+    max_blocks are used."""
 
     def __init__(self, forms, max_iterations=200000, check_every=64, tol=1e-6, mode=MODE_MIXED,
-                 tune=True, lanes_bias=1.0, max_blocks=32, **options):
+                 tune=True, lanes_bias=1.0, max_blocks=512, **options):
         self._keep = []
         us, ss, drs, dcs = [], [], [], []
         self.shapes = []
@@ -376,7 +376,7 @@ class Server:
 
     def __init__(self, mode=MODE_MIXED, tol=1e-6, max_iterations=200000, check_every=64,
                  max_cols=1 << 24, max_rows=1 << 23, max_nnz=1 << 27, max_instances=256,
-                 max_blocks=256, lanes_bias=1.0, replays_per_step=1, **options):
+                 max_blocks=1024, lanes_bias=1.0, replays_per_step=1, **options):
         so = _lib.SuprServerOptions()
         lib().supr_server_default_options(ctypes.byref(so))
         so.pdlp = _options(mode, max_iterations, check_every, tol, **options)
