@@ -130,6 +130,17 @@ int blocks_for(int rows, int L, int cap) {
     return (int)std::max(1L, std::min<long>(need, cap));
 }
 
+// A side of an instance runs the LDS-staged tiles (256 row products staged,
+// coalesced epilogues) when it has L >= 2 and at least this many rows.
+constexpr int kBatchStageRows = 1024;
+int staged_side(int rows, int L) { return (L >= 2 && rows >= kBatchStageRows) ? 1 : 0; }
+
+// Blocks of a staged side: one 256-row tile each, at most `cap`.
+int tiles_for(int rows, int cap) {
+    const long need = ((long)rows + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min<long>(need, cap));
+}
+
 double inf_norm_finite(const double* v, int len) {
     double m = 0.0;
     for (int k = 0; k < len; ++k)
@@ -391,12 +402,16 @@ struct Pool {
         dtabm.alloc<int32_t>(need_m);
     }
 
+    // Lanes per row, staged or not, and block counts of instance k: functions
+    // of its shape and the configuration only.
     void shape_rule(int k) {
         BatchInst& I = inst[k];
         I.LA = lanes_for(meanA[k], bias);
         I.LAT = lanes_for(meanAT[k], bias);
-        I.bn = blocks_for(I.n, I.LAT, cap);
-        I.bm = blocks_for(I.m, I.LA, cap);
+        I.sn = staged_side(I.n, I.LAT);
+        I.sm = staged_side(I.m, I.LA);
+        I.bn = I.sn ? tiles_for(I.n, cap) : blocks_for(I.n, I.LAT, cap);
+        I.bm = I.sm ? tiles_for(I.m, cap) : blocks_for(I.m, I.LA, cap);
     }
 
     bool fits(const Prepared& p) const {

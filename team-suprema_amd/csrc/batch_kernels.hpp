@@ -28,6 +28,7 @@ struct BatchInst {
     int32_t LA, LAT;     // lanes per row of A and of A^T (1, 4, 16, 64)
     int32_t bn, bm;      // blocks of this instance in column-side / row-side launches
     int32_t on, om;      // its first block in each (also its first partial slot)
+    int32_t sn, sm;      // column / row side uses the LDS-staged tiles (for_row_tiles)
     int64_t ro, co;      // offsets into the concatenated row / column vectors
     int64_t pa, pat;     // offsets of its row-pointer arrays (m + 1 and n + 1 entries)
     int64_t za;          // offset into the nonzero arrays (A and A^T hold nnz each)
@@ -168,8 +169,32 @@ __device__ __forceinline__ double primal_rows(const BatchGeo& g, const BatchInst
     T* __restrict__ xn = sel(B.x, cur ^ 1);
     const T* __restrict__ y = sel(B.y, cur);
     double acc = 0.0;
+    auto epilogue = [&](int row, T a) {
+        const T xj = x[row];
+        if (pending) B.xsum[row] += eta_acc * xj;
+        const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
+        xn[row] = v;
+        if constexpr (sizeof(T) == 4)
+            reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
+        const double d = (double)v - (double)xj;
+        acc += d * d;
+    };
     with_lanes(I.LAT, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
+        if (I.sn) {
+            if (need_aty) {
+                for_row_tiles<L, 1, T>(
+                    I.n, lb, I.bn,
+                    [&](int row, int lane, bool valid, T (&v)[1]) { v[0] = row_dot<L, T>(AT, y, row, lane, valid); },
+                    [&](int row, const T (&v)[1]) {
+                        B.aty[row] = v[0];
+                        epilogue(row, v[0]);
+                    });
+            } else {
+                for_rows_flat(I.n, lb, I.bn, [&](int row) { epilogue(row, B.aty[row]); });
+            }
+            return;
+        }
         inst_rows<L>(I.n, lb, I.bn, [&](int row, int lane, bool valid) {
             T a = T(0);
             if (need_aty) {
@@ -178,14 +203,7 @@ __device__ __forceinline__ double primal_rows(const BatchGeo& g, const BatchInst
             }
             if (valid && lane == 0) {
                 if (!need_aty) a = B.aty[row];
-                const T xj = x[row];
-                if (pending) B.xsum[row] += eta_acc * xj;
-                const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
-                xn[row] = v;
-                if constexpr (sizeof(T) == 4)
-                    reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
-                const double d = (double)v - (double)xj;
-                acc += d * d;
+                epilogue(row, a);
             }
         });
     });
@@ -233,28 +251,37 @@ __device__ __forceinline__ void dual_rows(const BatchGeo& g, const BatchInst& I,
     const T* __restrict__ ax = sel(B.ax, cur);
     T* __restrict__ axn = sel(B.ax, cur ^ 1);
     constexpr bool kDiff = sizeof(T) == 4;
+    auto epilogue = [&](int row, T a, T ad) {
+        axn[row] = a;
+        const T yi = y[row];
+        if (pending) B.ysum[row] += eta_acc * yi;
+        if constexpr (!kDiff) ad = a - ax[row];
+        T v = yi + sigma * (B.b[row] - (a + ad));
+        if (B.ineq[row] && v < T(0)) v = T(0);
+        yn[row] = v;
+        const double d = (double)v - (double)yi;
+        acc[0] += d * d;
+        acc[1] += d * (double)ad;
+    };
     with_lanes(I.LA, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
-        inst_rows<L>(I.m, lb, I.bm, [&](int row, int lane, bool valid) {
-            T a, ad;
+        auto dot = [&](int row, int lane, bool valid, T (&v)[2]) {
             if constexpr (kDiff) {
-                row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, a, ad);
+                row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, v[0], v[1]);
             } else {
-                a = row_dot<L, T>(A, xn, row, lane, valid);
-                ad = T(0);
+                v[0] = row_dot<L, T>(A, xn, row, lane, valid);
+                v[1] = T(0);
             }
-            if (valid && lane == 0) {
-                axn[row] = a;
-                const T yi = y[row];
-                if (pending) B.ysum[row] += eta_acc * yi;
-                if constexpr (!kDiff) ad = a - ax[row];
-                T v = yi + sigma * (B.b[row] - (a + ad));
-                if (B.ineq[row] && v < T(0)) v = T(0);
-                yn[row] = v;
-                const double d = (double)v - (double)yi;
-                acc[0] += d * d;
-                acc[1] += d * (double)ad;
-            }
+        };
+        if (I.sm) {
+            for_row_tiles<L, 2, T>(I.m, lb, I.bm, dot,
+                                   [&](int row, const T (&v)[2]) { epilogue(row, v[0], v[1]); });
+            return;
+        }
+        inst_rows<L>(I.m, lb, I.bm, [&](int row, int lane, bool valid) {
+            T v[2];
+            dot(row, lane, valid, v);
+            if (valid && lane == 0) epilogue(row, v[0], v[1]);
         });
     });
 }

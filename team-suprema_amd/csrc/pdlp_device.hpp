@@ -364,14 +364,23 @@ template <int L> __host__ __device__ __forceinline__ bool staged_rows(int rows) 
     return L >= 2 && rows >= kStageRows;
 }
 
+// LDS staging buffer, one per (type, width) whatever the lanes per row, so
+// a kernel that dispatches L at run time holds one copy.
+template <typename Acc, int NA> __device__ __forceinline__ Acc (&stage_buffer())[NA][kBlock] {
+    __shared__ Acc buf[NA][kBlock];
+    return buf;
+}
+
+// Tiles b0, b0 + nb, b0 + 2 nb, ... of kBlock rows (the single-LP kernels
+// pass the remapped block and the grid; a batched instance its own blocks).
 template <int L, int NA, typename Acc, typename Dot, typename Epi>
-__device__ __forceinline__ void for_row_tiles(int rows, Dot&& dot, Epi&& epi) {
-    __shared__ Acc stage[NA][kBlock];
+__device__ __forceinline__ void for_row_tiles(int rows, int b0, int nb, Dot&& dot, Epi&& epi) {
+    Acc (&stage)[NA][kBlock] = stage_buffer<Acc, NA>();
     constexpr int gpb = L >= kBlock ? 1 : kBlock / L;   // rows per pass
     constexpr int passes = kBlock / gpb;
     const int g = threadIdx.x / (kBlock / gpb);
     const int lane = threadIdx.x % (kBlock / gpb);
-    for (int t0 = xcd_block() * kBlock; t0 < rows; t0 += gridDim.x * kBlock) {
+    for (int t0 = b0 * kBlock; t0 < rows; t0 += nb * kBlock) {
         for (int pass = 0; pass < passes; ++pass) {
             const int r = pass * gpb + g;
             const int row = t0 + r;
@@ -394,10 +403,9 @@ __device__ __forceinline__ void for_row_tiles(int rows, Dot&& dot, Epi&& epi) {
     }
 }
 
-// One row per thread (epilogue-only passes), XCD-remapped, grid-stride.
-template <typename F> __device__ __forceinline__ void for_rows_flat(int rows, F&& body) {
-    for (int row = xcd_block() * kBlock + threadIdx.x; row < rows; row += gridDim.x * kBlock)
-        body(row);
+// One row per thread (epilogue-only passes) over blocks b0, b0 + nb, ...
+template <typename F> __device__ __forceinline__ void for_rows_flat(int rows, int b0, int nb, F&& body) {
+    for (int row = b0 * kBlock + threadIdx.x; row < rows; row += nb * kBlock) body(row);
 }
 
 __device__ __forceinline__ double merit3(const double* e) {

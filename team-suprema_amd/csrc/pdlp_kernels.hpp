@@ -14,7 +14,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(MatView<V> M, const X* __restri
                                                  Y* __restrict__ y) {
     if (staged_rows<L>(M.rows)) {
         for_row_tiles<L, 1, V>(
-            M.rows,
+            M.rows, xcd_block(), gridDim.x,
             [&](int row, int lane, bool valid, V (&v)[1]) { v[0] = row_dot<L, V>(M, x, row, lane, valid); },
             [&](int row, const V (&v)[1]) { y[row] = (Y)v[0]; });
         return;
@@ -149,14 +149,14 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
     if (staged_rows<L>(AT.rows)) {
         if (need_aty) {
             for_row_tiles<L, 1, T>(
-                AT.rows,
+                AT.rows, xcd_block(), gridDim.x,
                 [&](int row, int lane, bool valid, T (&v)[1]) { v[0] = row_dot<L, T>(AT, y, row, lane, valid); },
                 [&](int row, const T (&v)[1]) {
                     B.aty[row] = v[0];
                     epilogue(row, v[0]);
                 });
         } else {
-            for_rows_flat(AT.rows, [&](int row) { epilogue(row, B.aty[row]); });
+            for_rows_flat(AT.rows, xcd_block(), gridDim.x, [&](int row) { epilogue(row, B.aty[row]); });
         }
     } else {
         for_rows<L>(AT.rows, [&](int row, int lane, bool valid) {
@@ -221,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
         acc[1] += d * (double)ad;
     };
     if (staged_rows<L>(A.rows)) {
-        for_row_tiles<L, 2, T>(A.rows, dot, [&](int row, const T (&v)[2]) { epilogue(row, v[0], v[1]); });
+        for_row_tiles<L, 2, T>(A.rows, xcd_block(), gridDim.x, dot,
+                               [&](int row, const T (&v)[2]) { epilogue(row, v[0], v[1]); });
     } else {
         for_rows<L>(A.rows, [&](int row, int lane, bool valid) {
             T v[2];

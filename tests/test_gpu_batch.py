@@ -2,7 +2,7 @@
 batch against the FP64 oracle (same tolerances as test_gpu_parity.py), and
 an instance's result is the same bits alone, in the whole batch and in
 another mix.  Instances: netlib fixtures, a generated transportation LP and
-generated blending LPs, 9 in all, so the layout compaction (more than 8
+generated blending LPs, 10 in all, so the layout compaction (more than 8
 instances, half of them finished) runs as well."""
 import sys
 
@@ -26,7 +26,9 @@ def say(*a):
 @pytest.fixture(scope="module")
 def batch_cases(oracle, golden):
     cases = []
-    for name in ["afiro", "tiny_range", "sc50a", "sc105", "blend", "israel", "transport_30x40_s7"]:
+    # random_2000x5000_s42 runs both sides on the LDS-staged tiles (>= 1024 rows, L >= 2)
+    for name in ["afiro", "tiny_range", "sc50a", "sc105", "blend", "israel", "transport_30x40_s7",
+                 "random_2000x5000_s42"]:
         case = golden[name]
         cases.append((name, golden_lp(case), case["ref"]["objective_form"]))
     for k, (K, Pp) in enumerate([(20, 10), (35, 12)]):
