@@ -262,6 +262,23 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7):
     }
 
 
+def bench_large_kernels(reps=50):
+    """The power-law LP's iteration kernels alone (HIP events), with their
+    algorithmic bytes -- the run to profile them."""
+    lp = P.gen_power_law_lp(500000, 2000000, seed=2026)
+    s, Dr, Dc = P.scale(lp)
+    kb = P.bench_kernels(lp, s, Dr, Dc, reps=reps)
+    out = {"workload": f"power-law LP m=500000 n=2000000 nnz={lp.nnz}", "lanes_A": kb["lanes_A"],
+           "lanes_AT": kb["lanes_AT"], "kernels": {}}
+    for name, key, sz in (("k_primal_f32", "primal_ms_fp32", 4), ("k_dual_f32", "dual_ms_fp32", 4),
+                          ("k_primal_f64", "primal_ms_fp64", 8), ("k_dual_f64", "dual_ms_fp64", 8)):
+        prim, dual = kernel_bytes(lp, sz, False, False)
+        b = prim if "primal" in name else dual
+        out["kernels"][name] = dict(ms=kb[key], bytes=b, GBps=b / (kb[key] * 1e-3) / 1e9,
+                                    frac=b / (kb[key] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    return out
+
+
 def bench_large():
     """The power-law LP: mixed mode to 1e-6 (verified), kernel timing."""
     lp = P.gen_power_law_lp(500000, 2000000, seed=2026)
@@ -291,7 +308,7 @@ def main():
     ap.add_argument("--serving-full", action="store_true",
                     help="serving stream over the full nnz range [1e4, 2e7] (long)")
     ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
-    ap.add_argument("--only", choices=["batched", "serving", "large"],
+    ap.add_argument("--only", choices=["batched", "serving", "large", "large-kernels"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain device copy of known size (reference for HBM counters)")
@@ -315,7 +332,8 @@ def main():
         return
 
     if args.only:
-        fn = {"batched": bench_batched, "serving": bench_serving, "large": bench_large}[args.only]
+        fn = {"batched": bench_batched, "serving": bench_serving, "large": bench_large,
+              "large-kernels": bench_large_kernels}[args.only]
         print(json.dumps({args.only: fn()}))
         return
 
