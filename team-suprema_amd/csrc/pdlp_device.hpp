@@ -196,7 +196,10 @@ template <int L, typename Acc> __device__ __forceinline__ Acc group_sum(Acc s) {
 // Row dot product with a group of L lanes; every lane of the group gets the
 // sum.  valid = false gives 0 but still takes part in the group reduction.
 // The CSR loop issues U independent index / value / gather loads before the
-// multiply-adds, so a lane has U gathers in flight instead of one.
+// multiply-adds, so a lane has U gathers in flight instead of one; the last
+// group of fewer than U is issued the same way under a predicate instead of
+// one load pair at a time.  The terms are added in the order p, p + L,
+// p + 2L, ... either way.
 template <int L, typename Acc, typename V, typename X>
 __device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict__ x, int row,
                                        int lane, bool valid) {
@@ -225,7 +228,23 @@ __device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict_
 #pragma unroll
             for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
         }
-        for (; p < e; p += L) s += (Acc)M.val[p] * (Acc)x[M.idx[p]];
+        if (p < e) {
+            bool ok[U];
+            int c[U];
+            V v[U];
+            X xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ok[u] = p + u * L < e;
+                c[u] = ok[u] ? M.idx[p + u * L] : 0;
+                v[u] = ok[u] ? M.val[p + u * L] : V(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = ok[u] ? x[c[u]] : X(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u]) s += (Acc)v[u] * (Acc)xv[u];
+        }
     }
     return group_sum<L, Acc>(s);
 }
@@ -262,11 +281,24 @@ __device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __re
 #pragma unroll
             for (int u = 0; u < U; ++u) { s1 += a[u] * u1[u]; s2 += a[u] * u2[u]; }
         }
-        for (; p < e; p += L) {
-            const int c = M.idx[p];
-            const double a = (double)M.val[p];
-            s1 += a * x1[c];
-            s2 += a * x2[c];
+        if (p < e) {
+            bool ok[U];
+            int c[U];
+            double a[U], u1[U], u2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ok[u] = p + u * L < e;
+                c[u] = ok[u] ? M.idx[p + u * L] : 0;
+                a[u] = ok[u] ? (double)M.val[p + u * L] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                u1[u] = ok[u] ? x1[c[u]] : 0.0;
+                u2[u] = ok[u] ? x2[c[u]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u]) { s1 += a[u] * u1[u]; s2 += a[u] * u2[u]; }
         }
     }
     s1 = group_sum<L, double>(s1);
@@ -310,11 +342,22 @@ __device__ __forceinline__ void row_dot_diff(const MatView<float>& M, const floa
 #pragma unroll
             for (int u = 0; u < U; ++u) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
         }
-        for (; p < e; p += L) {
-            const float a = M.val[p];
-            const float2 v = xd[M.idx[p]];
-            s1 += a * v.x;
-            sd += a * v.y;
+        if (p < e) {
+            bool ok[U];
+            int c[U];
+            float a[U];
+            float2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ok[u] = p + u * L < e;
+                c[u] = ok[u] ? M.idx[p + u * L] : 0;
+                a[u] = ok[u] ? M.val[p + u * L] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = ok[u] ? xd[c[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u]) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
         }
     }
     s1 = group_sum<L, float>(s1);
