@@ -335,7 +335,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(instances shard across GPUs without communication; see DESIGN.md)",
+              f"(independent instances would shard across GPUs without communication)",
               file=sys.stderr)
         sys.exit(2)
     if P.device_count() < 1:

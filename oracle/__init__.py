@@ -5,7 +5,7 @@
 built here when the reference tree is present).  Both expose the same
 functions over the product's `pdlp_amd.LP` objects (any object with the same
 fields works).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
-baseline leg may use this package, and only as the checker.
+baseline leg may use this package, and only to check results against.
 """
 import ctypes
 import os

@@ -2,8 +2,8 @@
  * pdlp_oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C, FP64, single-threaded restatement of the reference's CPU PDLP
- * path.  It is the checker the parity tests, the smoke test and the bench's
- * CPU-baseline leg compare the HIP engine against; nothing in the product
+ * path.  The parity tests, the smoke test and the bench's CPU-baseline leg
+ * compare the HIP engine against it; nothing in the product
  * (team-suprema_amd/) links, loads or calls it.
  *
  * Every function follows one reference function and cites it (paths are

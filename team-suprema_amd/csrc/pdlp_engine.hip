@@ -913,7 +913,7 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
         SUPR_HIP(hipMemcpy(x_out, bestx_.as<void>(), sizeof(double) * n_, hipMemcpyDeviceToHost));
         SUPR_HIP(hipMemcpy(y_out, besty_.as<void>(), sizeof(double) * m_, hipMemcpyDeviceToHost));
     } else {
-        // No check ran yet (stopped early): judge the current point on the host.
+        // No check ran yet (stopped early): assess the current point on the host.
         std::vector<double> xh(n_), yh(m_);
         if (hs.phase == 0) {
             std::vector<float> xf(n_), yf(m_);

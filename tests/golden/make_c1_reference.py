@@ -4,7 +4,7 @@ sources) to completion on the 1,000,000-variable transportation LP
 writes tests/golden/c1_reference.json: its iteration / restart counts, eps,
 objective and single-thread wall time on the machine that ran it.  The
 benchmark's CPU-baseline leg times a bounded prefix of the same run on the
-GPU box's host and extrapolates with these counts."""
+benchmark machine's host cores and extrapolates with these counts."""
 import json
 import os
 import sys

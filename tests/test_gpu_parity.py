@@ -147,7 +147,7 @@ def transport_1m():
 
 def test_transport_1m_verified_and_modes_agree(oracle, transport_1m):
     """1,000,000-variable transportation LP: every device-resident mode reaches
-    1e-6 as judged by the oracle's verifier; the objectives agree; a second
+    1e-6 as measured by the oracle's verifier; the objectives agree; a second
     run with the same SpMV schedule is bit-identical (deterministic
     reductions)."""
     lp, scaled, Dr, Dc = transport_1m

@@ -106,7 +106,7 @@ NETLIB = ["afiro", "adlittle", "blend", "sc50a", "sc50b", "sc105", "kb2", "scagr
 @pytest.mark.parametrize("name", NETLIB)
 def test_oracle_reaches_published_netlib_optimum(oracle, golden, name):
     """audit_lp's rule: |obj - published| / (1 + |published|) <= 1e-4
-    (tools/audit_lp.cpp:270-277), judged on the original LP after
+    (tools/audit_lp.cpp:270-277), evaluated on the original LP after
     extract_solution, with the sign convention of report_objective."""
     case = golden[name]
     lp, maximize = load_fixture(name)
