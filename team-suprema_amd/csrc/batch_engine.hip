@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -300,8 +301,16 @@ struct Pool {
     double *part_n = nullptr, *part_m = nullptr, *part_kn = nullptr, *part_km = nullptr;
     double *part_rn = nullptr, *part_rm = nullptr;
     int64_t device_bytes = 0;
+    double* status_h = nullptr;   // host-coherent [running, progress] written by kb_status
+    double* status_d = nullptr;   // the same memory, device address
+    int32_t* done_h = nullptr;    // host-coherent done flag per slot, written by kb_restart_fin
+    int32_t* done_d = nullptr;
 
-    ~Pool() { if (stream) (void)hipStreamDestroy(stream); }
+    ~Pool() {
+        if (status_h) (void)hipHostFree(status_h);
+        if (done_h) (void)hipHostFree(done_h);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 
     template <typename T> T* arr(Buf& b, int64_t len) {
         device_bytes += (int64_t)sizeof(T) * std::max<int64_t>(len, 1);
@@ -366,7 +375,12 @@ struct Pool {
         K.ax_avg = arr<double>(axavg, Mcap); K.best_y = arr<double>(besty, Mcap);
         arr<DevState>(dstate, 2 * (int64_t)Scap);
         arr<double>(ddx2, Scap);
-        arr<double>(dstatus, 2);
+        BATCH_HIP(hipHostMalloc((void**)&status_h, 2 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped));
+        status_h[0] = status_h[1] = 0.0;
+        BATCH_HIP(hipHostGetDevicePointer((void**)&status_d, status_h, 0));
+        BATCH_HIP(hipHostMalloc((void**)&done_h, sizeof(int32_t) * Scap, hipHostMallocCoherent | hipHostMallocMapped));
+        std::memset(done_h, 0, sizeof(int32_t) * Scap);
+        BATCH_HIP(hipHostGetDevicePointer((void**)&done_d, done_h, 0));
         arr<BatchInst>(dinst, Scap);
         arr<int32_t>(dids, Scap);
         arr<DevState>(dstage, Scap);
@@ -529,6 +543,7 @@ struct Pool {
     void cold_start(const std::vector<int32_t>& fresh, int max_iter) {
         if (fresh.empty()) return;
         std::vector<DevState> st0(fresh.size());
+        for (int32_t k : fresh) done_h[k] = 0;
         for (size_t r = 0; r < fresh.size(); ++r) {
             DevState s = init[fresh[r]];
             s.max_iter = max_iter;
@@ -575,8 +590,8 @@ struct Pool {
         hipLaunchKernelGGL(kb_check_restart, dim3(g.BN + g.BM), dim3(kBlock), 0, st, S, g, F, D, K, part_rn, part_rm);
         if (mixed)
             hipLaunchKernelGGL(kb_switch_products, dim3(g.BN + g.BM), dim3(kBlock), 0, st, S, g, V, D);
-        hipLaunchKernelGGL(kb_restart_fin, dim3(live), dim3(kBlock), 0, st, S, g, part_rn, part_rm);
-        hipLaunchKernelGGL(kb_status, dim3(1), dim3(kBlock), 0, st, S, g, live, dstatus.get<double>());
+        hipLaunchKernelGGL(kb_restart_fin, dim3(live), dim3(kBlock), 0, st, S, g, part_rn, part_rm, done_d);
+        hipLaunchKernelGGL(kb_status, dim3(1), dim3(kBlock), 0, st, S, g, live, status_d);
         if (marks) BATCH_HIP(hipEventRecord(marks[2 * G + 1], st));
         BATCH_HIP(hipGetLastError());
     }
@@ -703,17 +718,14 @@ struct Batch {
         const int G = pool.graph_attempts();
         hipGraphExec_t exec = nullptr;
         constexpr int R = 4;
-        double* ring = nullptr;
         hipEvent_t ev[R] = {};
         std::vector<DevState> h;
         auto cleanup = [&]() {
             (void)hipStreamSynchronize(st);
             for (auto& e : ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
-            if (ring) { (void)hipHostFree(ring); ring = nullptr; }
             if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
         };
         try {
-            BATCH_HIP(hipHostMalloc((void**)&ring, sizeof(double) * 2 * R, hipHostMallocDefault));
             for (auto& e : ev) BATCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             if (pool.opt.verbose >= 3) pool.profile_replay(g, (int)live.size(), G);
             exec = pool.capture(g, (int)live.size(), G);
@@ -725,7 +737,11 @@ struct Batch {
                     const int s = (replays - inflight) % R;
                     BATCH_HIP(hipEventSynchronize(ev[s]));
                     --inflight;
-                    const double running = ring[2 * s], progress = ring[2 * s + 1];
+                    // the status the replays' last kernel wrote (this replay's or a
+                    // newer one's: both only move forward)
+                    std::atomic_thread_fence(std::memory_order_acquire);
+                    const double running = ((volatile double*)pool.status_h)[0];
+                    const double progress = ((volatile double*)pool.status_h)[1];
                     if (running == 0.0) break;
                     // A replay always advances attempts or checks of a running
                     // instance; a loop that stops moving is a logic error.
@@ -757,8 +773,6 @@ struct Batch {
                 }
                 const int s = replays % R;
                 BATCH_HIP(hipGraphLaunch(exec, st));
-                BATCH_HIP(hipMemcpyAsync(&ring[2 * s], pool.dstatus.get<void>(), sizeof(double) * 2,
-                                         hipMemcpyDeviceToHost, st));
                 BATCH_HIP(hipEventRecord(ev[s], st));
                 ++replays;
                 ++inflight;
@@ -823,7 +837,6 @@ struct Server {
     std::unordered_map<int32_t, Clock::time_point> admitted;
     hipGraphExec_t exec = nullptr;
     int32_t finished_total = 0;
-    std::vector<DevState> h;
 
     ~Server() { if (exec) (void)hipGraphExecDestroy(exec); }
 
@@ -853,9 +866,12 @@ struct Server {
         bool changed = false;
         if (!live.empty()) {
             std::vector<int32_t> still;
+            std::atomic_thread_fence(std::memory_order_acquire);
             for (int32_t k : live) {
-                const DevState& s = h[2 * k];
-                if (!s.done) { still.push_back(k); continue; }
+                if (!((volatile int32_t*)pool.done_h)[k]) { still.push_back(k); continue; }
+                DevState s;
+                BATCH_HIP(hipMemcpy(&s, pool.dstate.get<DevState>() + 2 * k, sizeof(DevState),
+                                    hipMemcpyDeviceToHost));
                 auto it = running.find(k);
                 Finished f;
                 f.m = pool.inst[k].m;
@@ -918,7 +934,6 @@ struct Server {
             const int reps = so.replays_per_step > 0 ? so.replays_per_step : 1;
             for (int q = 0; q < reps; ++q) BATCH_HIP(hipGraphLaunch(exec, pool.stream));
             BATCH_HIP(hipStreamSynchronize(pool.stream));
-            pool.read_states(h);
         }
         *n_running = (int32_t)live.size();
         {

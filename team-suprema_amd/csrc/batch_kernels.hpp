@@ -574,11 +574,12 @@ __global__ __launch_bounds__(kBlock) void kb_switch_products(const DevState* __r
 }
 
 // Check step 5 (k_restart_fin), one block per instance: primal weight and
-// restart counters, the FP64 hand-over, and S[2k] = S[2k+1] for the next
-// replay.
+// restart counters, the FP64 hand-over, S[2k] = S[2k+1] for the next replay,
+// and the instance's done flag for the host.
 __global__ __launch_bounds__(kBlock) void kb_restart_fin(DevState* __restrict__ S, BatchGeo g,
                                                          const double* __restrict__ part_rn,
-                                                         const double* __restrict__ part_rm) {
+                                                         const double* __restrict__ part_rm,
+                                                         int32_t* __restrict__ done_flags) {
     const int k = g.ids[blockIdx.x];
     const BatchInst I = g.inst[k];
     DevState* st = S + 2 * k + 1;
@@ -591,7 +592,10 @@ __global__ __launch_bounds__(kBlock) void kb_restart_fin(DevState* __restrict__ 
             if (st->switch_req) switch_to_fp64(*st);
         }
     }
-    if (threadIdx.x == 0) S[2 * k] = *st;
+    if (threadIdx.x == 0) {
+        S[2 * k] = *st;
+        if (done_flags) done_flags[k] = st->done;   // host-coherent, polled by the server
+    }
 }
 
 // After each replay: how many instances of the solve are still running, and

@@ -485,18 +485,24 @@ __device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2
 
 // Loads the state S[p] into LDS, applies the pending step decision (if the
 // previous launch ran an attempt), and publishes the result to S[p ^ 1]
-// (the publisher block only).  Returns the block's copy.
+// (the publisher block only).  Returns the block's copy.  The dual partials
+// and the primal sum are loaded together with the state -- one memory
+// latency on the kernel's critical path instead of two -- and used only when
+// a decision is pending.
 __device__ __forceinline__ void load_and_decide(DevState* __restrict__ S, int p,
                                                 const double* __restrict__ part_m, int nb_m,
                                                 const double* __restrict__ dx2_slot,
                                                 DevState& Q, bool publisher) {
-    if (threadIdx.x == 0) Q = S[p];
-    __syncthreads();
+    double dx2 = 0.0;
+    if (threadIdx.x == 0) {
+        Q = S[p];
+        dx2 = *dx2_slot;
+    }
+    double sm[2];
+    sum_partials<2>(part_m, nb_m, sm);   // its barriers also make Q visible
     if (Q.decide_pending) {
-        double sm[2];
-        sum_partials<2>(part_m, nb_m, sm);
         if (threadIdx.x == 0) {
-            decide_step(&Q, *dx2_slot, sm[0], sm[1]);
+            decide_step(&Q, dx2, sm[0], sm[1]);
             Q.decide_pending = 0;
         }
         __syncthreads();

@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <atomic>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -276,6 +277,7 @@ class Solver {
           Dr_(Drv_.data()), Dc_(Dcv_.data()), o_(o), m_(S.m), n_(S.n) {}
     ~Solver() {
         for (auto& g : graph_) if (g) (void)hipGraphExecDestroy(g);
+        if (mirror_) (void)hipHostFree(mirror_);
     }
 
     // Uploads the LP, lays out the matrices, tunes the SpMV schedules.
@@ -345,6 +347,8 @@ class Solver {
     double *part_n_ = nullptr, *part_m_ = nullptr, *part_kn_ = nullptr, *part_km_ = nullptr,
            *part_rn_ = nullptr, *part_rm_ = nullptr, *dx2_slot_ = nullptr;
     hipGraphExec_t graph_[2] = {nullptr, nullptr};   // per iterate precision
+    DevState* mirror_ = nullptr;       // host-coherent state copy (host pointer)
+    DevState* mirror_dev_ = nullptr;   // the same memory, device address
 
     template <typename T> void alloc_iter(Iter<T>& I);
     template <typename T> void reset_iter(Iter<T>& I);
@@ -678,6 +682,11 @@ void Solver::setup_device_resident(bool mixed) {
     part_rm_ = p; p += kMaxBlocks;
     dx2_slot_ = p;
     state_ = device_vec<DevState>(2);   // double-buffered (see pdlp_kernels.hpp)
+    // host-coherent pinned copy of the state, written by the last kernel of
+    // every graph replay and polled by the host (no copy on the stream)
+    SUPR_HIP(hipHostMalloc((void**)&mirror_, sizeof(DevState), hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(mirror_, 0, sizeof(DevState));
+    SUPR_HIP(hipHostGetDevicePointer((void**)&mirror_dev_, mirror_, 0));
 
     // SpMV schedules per iterate precision, timed on scratch vectors
     auto tune = [&](DevMatrix& M, int fixed, auto view, auto* xv, auto* yv) {
@@ -778,7 +787,7 @@ void Solver::enqueue_check(const Iter<T>& I) {
     hipLaunchKernelGGL(k_kkt_fin, dim3(1), dim3(kBlock), 0, s, S, part_kn_, gkn, part_km_, gkm);
     hipLaunchKernelGGL((k_check_restart<T>), dim3(en + em), dim3(kBlock), 0, s, S, n_, m_, en, B, K_,
                        part_rn_, part_rm_);
-    hipLaunchKernelGGL(k_restart_fin, dim3(1), dim3(kBlock), 0, s, S, part_rn_, en, part_rm_, em);
+    hipLaunchKernelGGL(k_restart_fin, dim3(1), dim3(kBlock), 0, s, S, part_rn_, en, part_rm_, em, mirror_dev_);
     SUPR_HIP(hipGetLastError());
 }
 
@@ -804,8 +813,10 @@ hipGraphExec_t Solver::capture_graph(const Iter<T>& I, int attempts) {
 }
 
 // Replays the graph until the device reports done / a precision switch, or
-// the stop callback fires.  A ring of pinned state copies keeps a few
-// replays in flight while the host polls the oldest one.
+// the stop callback fires.  A few replays stay in flight; after the oldest
+// one's event the host reads the host-coherent state copy the replays'
+// last kernel writes (it may already be a newer replay's: the flags only
+// move forward, and the final state is read back with a synchronous copy).
 template <typename T>
 int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx,
                            DevState& hs, bool& stopped) {
@@ -817,11 +828,9 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
     }
     hipGraphExec_t exec = graph_[prec];
     constexpr int R = 4;
-    DevState* ring = nullptr;
     hipEvent_t ev[R] = {};
     int replays = 0;
     try {
-        SUPR_HIP(hipHostMalloc((void**)&ring, sizeof(DevState) * R, hipHostMallocDefault));
         for (auto& e : ev) SUPR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         int last_attempts = -1, last_checks = -1, idle = 0;
         auto last_log = Clock::now();
@@ -829,7 +838,8 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
             if (replays >= R) {
                 const int slot = replays % R;
                 SUPR_HIP(hipEventSynchronize(ev[slot]));
-                hs = ring[slot];
+                std::atomic_thread_fence(std::memory_order_acquire);
+                std::memcpy(&hs, (const void*)mirror_, sizeof(DevState));
                 if (hs.done || hs.switch_req) break;
                 if (stop && stop(ctx)) { stopped = true; break; }
                 // A replay always advances attempts or checks; a loop that
@@ -850,7 +860,6 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
             }
             const int slot = replays % R;
             SUPR_HIP(hipGraphLaunch(exec, s));
-            SUPR_HIP(hipMemcpyAsync(&ring[slot], state_.as<void>(), sizeof(DevState), hipMemcpyDeviceToHost, s));
             SUPR_HIP(hipEventRecord(ev[slot], s));
             ++replays;
         }
@@ -859,11 +868,9 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
     } catch (...) {
         (void)hipStreamSynchronize(s);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
-        if (ring) (void)hipHostFree(ring);
         throw;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipHostFree(ring);
     return replays;
 }
 

@@ -440,10 +440,12 @@ __global__ __launch_bounds__(kBlock) void k_check_restart(const DevState* __rest
 
 // KKT check, step 5 (one block): primal-weight update and restart counters;
 // resumes the iterations unless finished or switching precision; copies the
-// state back to S[0] for the next replay (also when no check ran).
+// state back to S[0] for the next replay (also when no check ran) and, when
+// given, to the host-coherent mirror the host polls.
 __global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ S,
                                                         const double* __restrict__ part_n, int nb_n,
-                                                        const double* __restrict__ part_m, int nb_m) {
+                                                        const double* __restrict__ part_m, int nb_m,
+                                                        DevState* __restrict__ mirror) {
     DevState* st = S + 1;
     if (st->need_check) {
         double sn[1], sm[1];
@@ -451,7 +453,10 @@ __global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ S
         sum_partials<1>(part_m, nb_m, sm);
         if (threadIdx.x == 0) restart_fin_update(st, sn[0], sm[0]);
     }
-    if (threadIdx.x == 0) S[0] = *st;
+    if (threadIdx.x == 0) {
+        S[0] = *st;
+        if (mirror) *mirror = *st;   // host-coherent copy for the host's poll
+    }
 }
 
 // Precision switch (mixed mode): the FP64 iterate starts at the FP32 loop's
