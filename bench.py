@@ -25,6 +25,8 @@ Besides the headline (mixed FP32-iterate / FP64-residual engine) it reports:
     batching server, every result verified; p50/p99 latency and solves/s,
     against one-at-a-time plain FP64 solves of the same requests (their
     latency from a FIFO queue over the same arrivals);
+  * "c0": the random equality LP m = 2,000, n = 5,000 in each mode, beside the
+    reference's own CPU PDLP run to completion;
   * with --large: the power-law LP (m = 500,000, n = 2,000,000, 35.1M nnz).
 """
 import argparse
@@ -99,6 +101,46 @@ def cpu_baseline(lp, scaled, Dr, Dc, sample_attempts):
                    f"{attempts} attempts"),
         "host_cpus": os.cpu_count(),
     }
+
+
+def bench_c0():
+    """The reference's CPU-runnable case: the random equality LP m = 2,000,
+    n = 5,000 (6 nonzeros per column), solved to 1e-6 on the GPU in each mode
+    (verified) and by the reference's own CPU PDLP (oracle/_ref, one thread,
+    to completion)."""
+    lp = P.gen_random_lp(2000, 5000, 6, seed=42)
+    s, Dr, Dc = P.scale(lp)
+    out = {"workload": "random equality LP m=2000 n=5000 nnz=30000 (seed 42), tol 1e-6, verified"}
+    for name, mode in (("mixed", P.MODE_MIXED), ("fast_fp64", P.MODE_FAST_FP64)):
+        eng = P.Engine(lp, s, Dr, Dc, tol=TOL, mode=mode)
+        eng.solve()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = eng.solve()
+            ts.append(time.perf_counter() - t0)
+        eng.close()
+        rep = P.verify(lp, r.x, r.y)
+        if not r.converged or max(rep.eps_P, rep.eps_D, rep.eps_G) > TOL:
+            raise SystemExit(f"c0 {name}: not verified")
+        out[name] = dict(seconds=float(np.mean(ts)), attempts=r.iterations, objective=rep.objective)
+    t0 = time.perf_counter()
+    r = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=P.MODE_PLAIN_FP64)
+    out["plain_fp64"] = dict(seconds=time.perf_counter() - t0, attempts=r.iterations)
+    try:
+        sys.path.insert(0, ROOT)
+        from oracle import Checker, available
+        kind = "ref" if available("ref") else "oracle"
+        t0 = time.perf_counter()
+        ref = Checker(kind).pdlp(lp, s, Dr, Dc, 200000, 64, TOL)
+        out["cpu_reference"] = dict(seconds=time.perf_counter() - t0, attempts=ref.iterations, cores=1,
+                                    kind="reference" if kind == "ref" else "port")
+    except Exception as e:  # pragma: no cover
+        log(f"c0 CPU run unavailable: {e}")
+    log(f"c0: mixed {out['mixed']['seconds'] * 1e3:.1f} ms, fast64 {out['fast_fp64']['seconds'] * 1e3:.1f} ms, "
+        f"plain {out['plain_fp64']['seconds']:.2f} s, CPU reference "
+        f"{out.get('cpu_reference', {}).get('seconds', float('nan')):.2f} s")
+    return out
 
 
 def measured_traffic(kernel):
@@ -326,7 +368,7 @@ def main():
     ap.add_argument("--serving-full", action="store_true",
                     help="serving stream over the full nnz range [1e4, 2e7] (long)")
     ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
-    ap.add_argument("--only", choices=["batched", "serving", "serving-full", "large", "large-kernels"],
+    ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-kernels"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain device copy of known size (reference for HBM counters)")
@@ -350,7 +392,7 @@ def main():
         return
 
     if args.only:
-        fn = {"batched": bench_batched, "serving": bench_serving, "large": bench_large,
+        fn = {"c0": bench_c0, "batched": bench_batched, "serving": bench_serving, "large": bench_large,
               "large-kernels": bench_large_kernels,
               "serving-full": lambda: bench_serving(count=16, qps=0.5, nnz_hi=2e7,
                                                     plain_sample_attempts=1000)}[args.only]
@@ -425,6 +467,7 @@ def main():
         log(f"CPU baseline: reference CPU PDLP, {args.cpu_sample} attempts")
         cpu = cpu_baseline(lp, scaled, Dr, Dc, args.cpu_sample)
 
+    c0 = bench_c0()
     batched = None if args.no_batched else bench_batched()
     serving = None
     if not args.no_serving:
@@ -476,6 +519,7 @@ def main():
         },
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "c0": c0,
         "batched": batched,
         "serving": serving,
         "large": large,
