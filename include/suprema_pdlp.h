@@ -196,6 +196,23 @@ int supr_scale(const supr_lp* lp, const char* integer, int32_t ruiz_passes, doub
                double* c_s, double* rL_s, double* rU_s, double* l_s, double* u_s, double* Dr,
                double* Dc);
 
+/* scale() on the GPU (replaces src/scaling.cpp:25-84 in front of
+ * solve_pdhg_gpu, src/solve.cpp:194-195): the same arguments and outputs as
+ * supr_scale, bit for bit.  The matrix is laid out on the device as CSR and
+ * CSC by a stable sort (COO order kept inside every row and column), then
+ * scaled there.  seconds3 (may be NULL) receives the upload + layout, scaling
+ * and download times. */
+int supr_scale_gpu(const supr_lp* lp, const char* integer, int32_t ruiz_passes, int32_t device,
+                   double* val_s, double* c_s, double* rL_s, double* rU_s, double* l_s,
+                   double* u_s, double* Dr, double* Dc, double* seconds3);
+
+/* The CSR of A (transposed = 0: the reference's to_csr, include/sparse.hpp:
+ * 31-47) or of A^T (transposed = 1: to_csc_as_transposed_csr, :50-57) built
+ * on the GPU by a stable radix sort: ptr (rows + 1), idx and perm (nnz; perm
+ * = COO position of each entry), equal to the host counting sort's. */
+int supr_csr_gpu(const supr_lp* lp, int32_t transposed, int32_t device, int32_t* ptr,
+                 int32_t* idx, int32_t* perm);
+
 /* build_pdlp_form(): dims4 = m2, n2, nnz2, n_slack. */
 int supr_pdlp_form_sizes(const supr_lp* lp, int32_t* dims4);
 int supr_pdlp_form(const supr_lp* lp, int32_t* row_idx2, int32_t* col_idx2, double* val2,

@@ -39,6 +39,7 @@
 #include <string>
 #include <vector>
 
+#include "hip_util.hpp"
 #include "host_lp.hpp"
 #include "pdlp_kernels.hpp"
 #include "suprema_pdlp.h"
@@ -46,62 +47,6 @@
 namespace supr {
 
 namespace {
-
-struct HipFailure : std::runtime_error {
-    explicit HipFailure(const std::string& m) : std::runtime_error(m) {}
-};
-struct BadInput : std::runtime_error {
-    explicit BadInput(const std::string& m) : std::runtime_error(m) {}
-};
-
-#define SUPR_HIP(expr)                                                                       \
-    do {                                                                                     \
-        hipError_t _e = (expr);                                                              \
-        if (_e != hipSuccess)                                                                \
-            throw HipFailure(std::string("HIP error at " __FILE__ ":") +                    \
-                             std::to_string(__LINE__) + " (" #expr ") -- " +               \
-                             hipGetErrorString(_e));                                         \
-    } while (0)
-
-using Clock = std::chrono::steady_clock;
-double seconds_since(Clock::time_point t0) {
-    return std::chrono::duration<double>(Clock::now() - t0).count();
-}
-
-// Owning device allocation (move-only).
-class DevBuf {
-   public:
-    DevBuf() = default;
-    explicit DevBuf(size_t bytes) { alloc(bytes); }
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
-    DevBuf& operator=(DevBuf&& o) noexcept { std::swap(p_, o.p_); return *this; }
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { if (p_) (void)hipFree(p_); }
-    void alloc(size_t bytes) {
-        if (p_) { (void)hipFree(p_); p_ = nullptr; }
-        SUPR_HIP(hipMalloc(&p_, std::max<size_t>(bytes, 256)));
-    }
-    template <typename T> T* as() const { return static_cast<T*>(p_); }
-    explicit operator bool() const { return p_ != nullptr; }
-
-   private:
-    void* p_ = nullptr;
-};
-
-// Synchronous upload: the host vector may be a temporary.
-template <typename T>
-DevBuf upload(const std::vector<T>& h, hipStream_t s) {
-    DevBuf d(h.size() * sizeof(T));
-    if (!h.empty()) {
-        SUPR_HIP(hipMemcpyAsync(d.as<void>(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-        SUPR_HIP(hipStreamSynchronize(s));
-    }
-    return d;
-}
-
-template <typename T>
-DevBuf device_vec(size_t len) { return DevBuf(len * sizeof(T)); }
 
 // Row-loop grids are rounded up to a multiple of the XCD count (the kernels'
 // XCD-aware block mapping needs it; extra blocks find no rows).
@@ -1024,27 +969,6 @@ void Solver::bench_kernels(int reps, double* out) {
     out[4] = A_.lanes; out[5] = AT_.lanes; out[6] = A_.ell_w; out[7] = AT_.ell_w;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-}
-
-int map_exception() {
-    try {
-        throw;
-    } catch (const BadInput& e) {
-        set_error(e.what());
-        return SUPR_EINVAL;
-    } catch (const HipFailure& e) {
-        set_error(e.what());
-        return SUPR_EDEVICE;
-    } catch (const std::bad_alloc&) {
-        set_error("out of host memory");
-        return SUPR_ENOMEM;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return SUPR_EINTERNAL;
-    } catch (...) {
-        set_error("unknown exception");
-        return SUPR_EINTERNAL;
-    }
 }
 
 int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const double* Dc,

@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
+    "extract_solution", "scale_gpu", "csr_gpu", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -142,6 +142,35 @@ def scale(lp, ruiz_passes=10, integer=None):
     check(lib().supr_scale(ctypes.byref(s), integ, int(ruiz_passes), *[_dp(a) for a in outs]))
     val, c, rL, rU, l, u, Dr, Dc = outs
     return lp.with_values(val, c, rL, rU, l, u, lp.name + " (scaled)"), Dr, Dc
+
+
+def scale_gpu(lp, ruiz_passes=10, integer=None, device=0, timing=False):
+    """scale() on the GPU: the same result as scale(), bit for bit.  With
+    timing=True also returns (upload + layout, scaling, download) seconds."""
+    outs = [np.zeros(lp.nnz), np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.m), np.zeros(lp.n),
+            np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.n)]
+    integ = None
+    if integer is not None:
+        integ = np.ascontiguousarray(integer, dtype=np.int8).tobytes()
+    secs = np.zeros(3)
+    s = lp.cstruct()
+    check(lib().supr_scale_gpu(ctypes.byref(s), integ, int(ruiz_passes), int(device),
+                               *[_dp(a) for a in outs], _dp(secs)))
+    val, c, rL, rU, l, u, Dr, Dc = outs
+    out = (lp.with_values(val, c, rL, rU, l, u, lp.name + " (scaled)"), Dr, Dc)
+    return out + (tuple(secs),) if timing else out
+
+
+def csr_gpu(lp, transposed=False, device=0):
+    """CSR of A (or of A^T) built on the GPU: (ptr, idx, perm), perm[p] = the
+    COO position of entry p; entries of a row keep COO order."""
+    rows = lp.n if transposed else lp.m
+    ptr = np.zeros(rows + 1, dtype=_i32)
+    idx, perm = np.zeros(lp.nnz, dtype=_i32), np.zeros(lp.nnz, dtype=_i32)
+    s = lp.cstruct()
+    check(lib().supr_csr_gpu(ctypes.byref(s), int(bool(transposed)), int(device), _ip(ptr),
+                             _ip(idx), _ip(perm)))
+    return ptr, idx, perm
 
 
 @dataclass

@@ -91,6 +91,10 @@ SIGNATURES = [
     ("supr_verify", ctypes.c_int, [_LP, c_double_p, c_double_p, c_double_p]),
     ("supr_scale", ctypes.c_int,
      [_LP, ctypes.c_char_p, ctypes.c_int32] + [c_double_p] * 8),
+    ("supr_scale_gpu", ctypes.c_int,
+     [_LP, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32] + [c_double_p] * 9),
+    ("supr_csr_gpu", ctypes.c_int,
+     [_LP, ctypes.c_int32, ctypes.c_int32, c_int32_p, c_int32_p, c_int32_p]),
     ("supr_pdlp_form_sizes", ctypes.c_int, [_LP, c_int32_p]),
     ("supr_pdlp_form", ctypes.c_int,
      [_LP, c_int32_p, c_int32_p] + [c_double_p] * 6 + [c_int32_p, c_double_p]),
