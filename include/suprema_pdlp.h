@@ -164,6 +164,21 @@ int supr_engine_solve(supr_engine* engine, supr_stop_fn stop, void* stop_ctx, do
                       double* y_out, supr_pdlp_info* info);
 void supr_engine_destroy(supr_engine* engine);
 
+/* scale() then solve_pdhg_gpu() with the scaling on the device (the
+ * reference's src/solve.cpp:195-207 sequence: scale(pdlp_form), then
+ * solve_pdhg_gpu(pdlp_form, scaled, Dr, Dc, ...)).  form is the UNSCALED
+ * PDLP form; integer (n flags, may be NULL) and ruiz_passes are scale()'s
+ * arguments (src/scaling.cpp:25).  The matrix is uploaded once, laid out as
+ * CSR / CSC and scaled on the device (bit-identical to supr_scale), so no
+ * host CSR build or host scaling precedes the solve; the result is the one
+ * supr_solve_pdhg_gpu gives for supr_scale's output, bit for bit. */
+int supr_solve_form_gpu(const supr_lp* form, const char* integer, int32_t ruiz_passes,
+                        const supr_pdlp_options* opt, supr_stop_fn stop, void* stop_ctx,
+                        double* x_out, double* y_out, supr_pdlp_info* info);
+/* A resident engine built the same way (see supr_engine_create). */
+int supr_engine_create_form(const supr_lp* form, const char* integer, int32_t ruiz_passes,
+                            const supr_pdlp_options* opt, supr_engine** engine);
+
 /* y = A x (transposed = 0; y has m entries) or A^T x (transposed = 1; y has
  * n entries) on the device, with lanes-per-row `lanes` (0: autotuned) and
  * FP32 (fp32 = 1: values, x and the accumulation in FP32) or FP64 arithmetic. */

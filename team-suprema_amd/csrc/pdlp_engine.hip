@@ -42,6 +42,7 @@
 #include "hip_util.hpp"
 #include "host_lp.hpp"
 #include "pdlp_kernels.hpp"
+#include "prep.hpp"
 #include "suprema_pdlp.h"
 
 namespace supr {
@@ -145,6 +146,44 @@ void build_dev_matrix(DevMatrix& D, const Csr& H, const double* unscaled_coo, bo
     SUPR_HIP(hipStreamSynchronize(s));   // host staging vectors die here
 }
 
+// The same layout from a pattern built on the device (DevLp): scaled and
+// unscaled values in CSR order; the pattern and value buffers are taken over
+// (moved) where the CSR layout keeps them.
+void build_dev_matrix(DevMatrix& D, DevPattern& P, DevBuf& scaled, DevBuf& unscaled, bool want32,
+                      bool want64, bool wantu, hipStream_t s) {
+    D.rows = P.rows;
+    D.cols = P.cols;
+    D.nnz = P.nnz;
+    const bool uniform = P.rows > 0 && P.min_len == P.max_len;
+    D.ell_w = (uniform && P.max_len >= 1 && P.max_len <= kMaxEllWidth) ? P.max_len : 0;
+    const int64_t nnz = P.nnz;
+    const int g = grid_elems(nnz);
+    if (D.ell_w > 0) {
+        auto ell = [&](auto* src, auto* dst) {
+            hipLaunchKernelGGL(k_ell_from_csr, dim3(g), dim3(kBlock), 0, s, P.rows, D.ell_w, src, dst);
+        };
+        D.idx = device_vec<int32_t>(nnz);
+        ell(P.idx.as<const int32_t>(), D.idx.as<int32_t>());
+        if (want32) { D.v32 = device_vec<float>(nnz); ell(scaled.as<const double>(), D.v32.as<float>()); }
+        if (want64) { D.v64 = device_vec<double>(nnz); ell(scaled.as<const double>(), D.v64.as<double>()); }
+        if (wantu) { D.vu = device_vec<double>(nnz); ell(unscaled.as<const double>(), D.vu.as<double>()); }
+        SUPR_HIP(hipGetLastError());
+        SUPR_HIP(hipStreamSynchronize(s));
+    } else {
+        D.ptr = std::move(P.ptr);
+        D.idx = std::move(P.idx);
+        if (want32) {
+            D.v32 = device_vec<float>(nnz);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kBlock), 0, s, nnz,
+                               scaled.as<const double>(), D.v32.as<float>());
+            SUPR_HIP(hipGetLastError());
+            SUPR_HIP(hipStreamSynchronize(s));
+        }
+        if (want64) D.v64 = std::move(scaled);
+        if (wantu) D.vu = std::move(unscaled);
+    }
+}
+
 template <typename V>
 void launch_spmv(const DevMatrix& M, const MatView<V>& view, const V* x, V* y, hipStream_t s) {
     SUPR_LANES(M.lanes, hipLaunchKernelGGL((k_spmv<LL, V, V, V>), dim3(grid_rows(M.rows, LL)),
@@ -199,15 +238,20 @@ double finite_inf_norm(const double* v, int len) {
 double merit_of(const double* e) { return std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]); }
 
 // Deep copy of a caller's LP, so an engine outlives the caller's arrays.
+// Without the matrix, the COO pointers are null (the vectors are copied).
 struct OwnedLp {
     std::vector<int32_t> row, col;
     std::vector<double> val, c, rL, rU, l, u;
     supr_lp v;
-    explicit OwnedLp(const supr_lp& s)
-        : row(s.row_idx, s.row_idx + s.nnz), col(s.col_idx, s.col_idx + s.nnz),
-          val(s.val, s.val + s.nnz), c(s.c, s.c + s.n), rL(s.rL, s.rL + s.m),
-          rU(s.rU, s.rU + s.m), l(s.l, s.l + s.n), u(s.u, s.u + s.n), v(s) {
-        v.row_idx = row.data(); v.col_idx = col.data(); v.val = val.data();
+    explicit OwnedLp(const supr_lp& s, bool with_matrix = true)
+        : row(with_matrix ? s.row_idx : nullptr, with_matrix ? s.row_idx + s.nnz : nullptr),
+          col(with_matrix ? s.col_idx : nullptr, with_matrix ? s.col_idx + s.nnz : nullptr),
+          val(with_matrix ? s.val : nullptr, with_matrix ? s.val + s.nnz : nullptr),
+          c(s.c, s.c + s.n), rL(s.rL, s.rL + s.m), rU(s.rU, s.rU + s.m), l(s.l, s.l + s.n),
+          u(s.u, s.u + s.n), v(s) {
+        v.row_idx = with_matrix ? row.data() : nullptr;
+        v.col_idx = with_matrix ? col.data() : nullptr;
+        v.val = with_matrix ? val.data() : nullptr;
         v.c = c.data(); v.rL = rL.data(); v.rU = rU.data(); v.l = l.data(); v.u = u.data();
     }
     OwnedLp(const OwnedLp&) = delete;
@@ -220,6 +264,16 @@ class Solver {
            const supr_pdlp_options& o)
         : Uown_(U), Sown_(S), Drv_(Dr, Dr + S.m), Dcv_(Dc, Dc + S.n), U_(Uown_.v), S_(Sown_.v),
           Dr_(Drv_.data()), Dc_(Dcv_.data()), o_(o), m_(S.m), n_(S.n) {}
+    // From the unscaled form alone: the matrix is laid out and scaled on the
+    // device (scale(), src/scaling.cpp:25-84, bit-identical to the host's);
+    // the scaled vectors and Dr, Dc come back to the host (n + m values).
+    Solver(const supr_lp& U, const char* integer, int ruiz_passes, const supr_pdlp_options& o)
+        : Uown_(U, o.mode == SUPR_MODE_PLAIN_FP64), Sown_(U, false), Drv_(U.m), Dcv_(U.n),
+          U_(Uown_.v), S_(Sown_.v), Dr_(Drv_.data()), Dc_(Dcv_.data()), o_(o), m_(U.m), n_(U.n),
+          device_prep_(true), passes_(ruiz_passes) {
+        if (integer) integer_.assign(integer, integer + U.n);
+        dlp_ = std::make_unique<DevLp>(U, stream_.s);   // uploads the caller's arrays now
+    }
     ~Solver() {
         for (auto& g : graph_) if (g) (void)hipGraphExecDestroy(g);
         if (mirror_) (void)hipHostFree(mirror_);
@@ -248,6 +302,17 @@ class Solver {
     std::vector<double> x0_;
     double setup_s_ = 0.0;
     bool ready_ = false;
+
+    // device preprocessing (the unscaled-form constructor)
+    bool device_prep_ = false;
+    int passes_ = 10;
+    std::vector<char> integer_;
+    std::unique_ptr<DevLp> dlp_;
+    std::unique_ptr<DevScaled> dsc_;
+    double amax_ = 0.0;                 // max |scaled a_ij|, from the device
+    std::unique_ptr<OwnedLp> uhost_;    // U with its matrix, fetched when needed
+    void prepare_on_device();
+    const supr_lp& unscaled_with_matrix();
 
     void initial_point();
     void setup_matrices(bool want32, bool want64, bool wantu);
@@ -315,8 +380,9 @@ void Solver::initial_point() {
     // omega0 = |c| / |finite rL| (1 when either is tiny).
     x0_.resize(n_);
     for (int j = 0; j < n_; ++j) x0_[j] = smin(smax(0.0, S_.l[j]), S_.u[j]);
-    double amax = 0.0;
-    for (int k = 0; k < S_.nnz; ++k) amax = smax(amax, std::fabs(S_.val[k]));
+    double amax = amax_;
+    if (!device_prep_)
+        for (int k = 0; k < S_.nnz; ++k) amax = smax(amax, std::fabs(S_.val[k]));
     eta0_ = amax > 0 ? 1.0 / amax : 1.0;
     double cn = 0.0, bn = 0.0;
     for (int j = 0; j < n_; ++j) cn += S_.c[j] * S_.c[j];
@@ -328,14 +394,62 @@ void Solver::initial_point() {
 }
 
 void Solver::setup_matrices(bool want32, bool want64, bool wantu) {
+    if (device_prep_) {
+        build_dev_matrix(A_, dlp_->A, dsc_->vA, dlp_->vA, want32, want64, wantu, stream_.s);
+        build_dev_matrix(AT_, dlp_->AT, dsc_->vAT, dlp_->vAT, want32, want64, wantu, stream_.s);
+        // keep only the COO arrays (the host verifier's fallback input)
+        dsc_.reset();
+        dlp_->A = DevPattern();
+        dlp_->AT = DevPattern();
+        dlp_->vA.reset();
+        dlp_->vAT.reset();
+        return;
+    }
     const Csr A = build_csr(S_, S_.val, false);
     build_dev_matrix(A_, A, U_.val, want32, want64, wantu, stream_.s);
     const Csr AT = build_csr(S_, S_.val, true);
     build_dev_matrix(AT_, AT, U_.val, want32, want64, wantu, stream_.s);
 }
 
+// scale() on the device; the scaled vectors, Dr and Dc come back to the host
+// (they seed the initial point and the unscaling of the results).
+void Solver::prepare_on_device() {
+    hipStream_t s = stream_.s;
+    dsc_ = std::make_unique<DevScaled>();
+    scale_on_device(*dlp_, integer_.empty() ? nullptr : integer_.data(), passes_, *dsc_, s);
+    amax_ = device_absmax(dsc_->vA.as<double>(), dlp_->nnz, s);
+    auto down = [&](double* dst, const DevBuf& src, int len) {
+        if (len > 0)
+            SUPR_HIP(hipMemcpyAsync(dst, src.as<void>(), sizeof(double) * len, hipMemcpyDeviceToHost, s));
+    };
+    down(Sown_.c.data(), dsc_->c, n_); down(Sown_.l.data(), dsc_->l, n_); down(Sown_.u.data(), dsc_->u, n_);
+    down(Sown_.rL.data(), dsc_->rL, m_); down(Sown_.rU.data(), dsc_->rU, m_);
+    down(Drv_.data(), dsc_->Dr, m_); down(Dcv_.data(), dsc_->Dc, n_);
+    SUPR_HIP(hipStreamSynchronize(s));
+}
+
+// The unscaled form with its COO matrix on the host (the caller's copy, or
+// fetched from the device for an engine built from the unscaled form).
+const supr_lp& Solver::unscaled_with_matrix() {
+    if (U_.nnz == 0 || U_.val != nullptr) return U_;
+    if (!uhost_) {
+        supr_lp shape = U_;
+        std::vector<int32_t> r(U_.nnz), c(U_.nnz);
+        std::vector<double> v(U_.nnz);
+        hipStream_t s = stream_.s;
+        SUPR_HIP(hipMemcpyAsync(r.data(), dlp_->row.as<void>(), sizeof(int32_t) * U_.nnz, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(c.data(), dlp_->col.as<void>(), sizeof(int32_t) * U_.nnz, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(v.data(), dlp_->val.as<void>(), sizeof(double) * U_.nnz, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+        shape.row_idx = r.data(); shape.col_idx = c.data(); shape.val = v.data();
+        uhost_ = std::make_unique<OwnedLp>(shape);
+    }
+    return uhost_->v;
+}
+
 void Solver::setup() {
     const auto t0 = Clock::now();
+    if (device_prep_) prepare_on_device();
     switch (o_.mode) {
         case SUPR_MODE_PLAIN_FP64: setup_plain(); break;
         case SUPR_MODE_FAST_FP64: setup_device_resident(false); break;
@@ -880,8 +994,9 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
             for (int i = 0; i < m_; ++i) yh[i] *= Dr_[i];
         }
         double rep[7];
-        const Csr Au = build_csr(U_, U_.val, false), AuT = build_csr(U_, U_.val, true);
-        verify_csr(U_, Au, AuT, xh.data(), yh.data(), rep);
+        const supr_lp& Uh = unscaled_with_matrix();
+        const Csr Au = build_csr(Uh, Uh.val, false), AuT = build_csr(Uh, Uh.val, true);
+        verify_csr(Uh, Au, AuT, xh.data(), yh.data(), rep);
         if (std::isfinite(merit_of(rep))) {
             has_point = true;
             std::copy(xh.begin(), xh.end(), x_out);
@@ -971,6 +1086,8 @@ void Solver::bench_kernels(int reps, double* out) {
     (void)hipEventDestroy(e1);
 }
 
+int check_options(const supr_lp* S, const supr_pdlp_options* opt);
+
 int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const double* Dc,
                      const supr_pdlp_options* opt) {
     if (!check_lp(U, "unscaled_form") || !check_lp(S, "scaled_form")) return SUPR_EINVAL;
@@ -983,11 +1100,17 @@ int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const
             set_error("unscaled and scaled forms must share one COO pattern (entry " + std::to_string(k) + ")");
             return SUPR_EINVAL;
         }
+    if (!Dr || !Dc) { set_error("null Dr or Dc"); return SUPR_EINVAL; }
+    return check_options(S, opt);
+}
+
+// Shape and option checks shared by every single-LP entry point.
+int check_options(const supr_lp* S, const supr_pdlp_options* opt) {
     if (S->m < 1 || S->n < 1) {
         set_error("the PDLP form needs at least one row and one column");
         return SUPR_EINVAL;
     }
-    if (!Dr || !Dc || !opt) { set_error("null Dr, Dc or options"); return SUPR_EINVAL; }
+    if (!opt) { set_error("null options"); return SUPR_EINVAL; }
     if (opt->max_iterations < 0 || opt->check_every < 1 || !(opt->tol >= 0)) {
         set_error("bad options: max_iterations >= 0, check_every >= 1, tol >= 0 required");
         return SUPR_EINVAL;
@@ -1007,6 +1130,13 @@ int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const
         return SUPR_EINVAL;
     }
     return SUPR_OK;
+}
+
+// Arguments of the unscaled-form entry points.
+int check_form_args(const supr_lp* form, int32_t ruiz_passes, const supr_pdlp_options* opt) {
+    if (!check_lp(form, "form")) return SUPR_EINVAL;
+    if (ruiz_passes < 0) { set_error("negative ruiz_passes"); return SUPR_EINVAL; }
+    return check_options(form, opt);
 }
 
 }  // namespace
@@ -1052,6 +1182,44 @@ struct supr_engine {
     int device;
     std::unique_ptr<supr::Solver> solver;
 };
+
+int supr_solve_form_gpu(const supr_lp* form, const char* integer, int32_t ruiz_passes,
+                        const supr_pdlp_options* opt, supr_stop_fn stop, void* stop_ctx,
+                        double* x_out, double* y_out, supr_pdlp_info* info) {
+    using namespace supr;
+    int rc = check_form_args(form, ruiz_passes, opt);
+    if (rc != SUPR_OK) return rc;
+    if (!x_out || !y_out || !info) { set_error("null output"); return SUPR_EINVAL; }
+    try {
+        SUPR_HIP(hipSetDevice(opt->device));
+        Solver solver(*form, integer, ruiz_passes, *opt);
+        solver.setup();
+        solver.run(stop, stop_ctx, x_out, y_out, info);
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int supr_engine_create_form(const supr_lp* form, const char* integer, int32_t ruiz_passes,
+                            const supr_pdlp_options* opt, supr_engine** out) {
+    using namespace supr;
+    int rc = check_form_args(form, ruiz_passes, opt);
+    if (rc != SUPR_OK) return rc;
+    if (!out) { set_error("null engine output"); return SUPR_EINVAL; }
+    *out = nullptr;
+    try {
+        SUPR_HIP(hipSetDevice(opt->device));
+        auto e = std::make_unique<supr_engine>();
+        e->device = opt->device;
+        e->solver = std::make_unique<Solver>(*form, integer, ruiz_passes, *opt);
+        e->solver->setup();
+        *out = e.release();
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
 
 int supr_engine_create(const supr_lp* unscaled_form, const supr_lp* scaled_form, const double* Dr,
                        const double* Dc, const supr_pdlp_options* opt, supr_engine** out) {

@@ -485,4 +485,28 @@ __global__ __launch_bounds__(kBlock) void k_switch_m(const DevState* __restrict_
     }
 }
 
+// ------------------------------------------------------- matrix layout --
+
+// ELL slot k * rows + r <- CSR position r * w + k (every row has length w).
+template <typename S, typename D>
+__global__ __launch_bounds__(kBlock) void k_ell_from_csr(int32_t rows, int32_t w,
+                                                         const S* __restrict__ src,
+                                                         D* __restrict__ dst) {
+    const int64_t total = (int64_t)rows * w;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = q / w, k = q - r * w;
+        dst[k * rows + r] = (D)src[q];
+    }
+}
+
+// dst = (D) src, elementwise (FP64 -> FP32 rounds to nearest even).
+template <typename S, typename D>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t len, const S* __restrict__ src,
+                                                    D* __restrict__ dst) {
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < len;
+         q += (int64_t)gridDim.x * kBlock)
+        dst[q] = (D)src[q];
+}
+
 }  // namespace supr

@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "scale_gpu", "csr_gpu", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
+    "extract_solution", "scale_gpu", "csr_gpu", "solve_form_gpu", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -264,9 +264,49 @@ def solve_pdhg_gpu(unscaled_form, scaled_form, Dr, Dc, eta=0.99, max_iterations=
     return _result(info, x, y)
 
 
+def solve_form_gpu(form, ruiz_passes=10, integer=None, max_iterations=200000, check_every=64,
+                   tol=1e-6, stop_flag=None, mode=MODE_MIXED, **options):
+    """scale(form) then solve_pdhg_gpu(form, scaled, Dr, Dc) with the scaling
+    on the device (src/solve.cpp:195-207); the same result as
+    solve_pdhg_gpu(form, *scale(form)), bit for bit."""
+    o = _options(mode, max_iterations, check_every, tol, **options)
+    x, y = np.zeros(form.n), np.zeros(form.m)
+    info = _lib.SuprInfo()
+    cb = _stop_cb(stop_flag)
+    sf = form.cstruct()
+    check(lib().supr_solve_form_gpu(ctypes.byref(sf), _flags(integer, form.n), int(ruiz_passes),
+                                    ctypes.byref(o), cb, None, _dp(x), _dp(y), ctypes.byref(info)))
+    return _result(info, x, y)
+
+
+def _flags(integer, n):
+    if integer is None:
+        return None
+    f = np.ascontiguousarray(integer, dtype=np.int8)
+    if f.shape != (n,):
+        raise ValueError("integer flags must have length n")
+    return f.tobytes()
+
+
 class Engine:
     """A resident engine (supr_engine_*): the LP is uploaded and tuned once,
-    then solve() runs cold-start solves with the data already in HBM."""
+    then solve() runs cold-start solves with the data already in HBM.
+    Engine.from_form(form) builds it from the unscaled form, with the layout
+    and scale() on the device."""
+
+    @classmethod
+    def from_form(cls, form, ruiz_passes=10, integer=None, max_iterations=200000,
+                  check_every=64, tol=1e-6, mode=MODE_MIXED, **options):
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        self.m, self.n = form.m, form.n
+        o = _options(mode, max_iterations, check_every, tol, **options)
+        h = ctypes.c_void_p()
+        sf = form.cstruct()
+        check(lib().supr_engine_create_form(ctypes.byref(sf), _flags(integer, form.n),
+                                            int(ruiz_passes), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        return self
 
     def __init__(self, unscaled_form, scaled_form, Dr, Dc, max_iterations=200000,
                  check_every=64, tol=1e-6, mode=MODE_MIXED, **options):
