@@ -175,6 +175,18 @@ void supr_engine_destroy(supr_engine* engine);
 int supr_solve_form_gpu(const supr_lp* form, const char* integer, int32_t ruiz_passes,
                         const supr_pdlp_options* opt, supr_stop_fn stop, void* stop_ctx,
                         double* x_out, double* y_out, supr_pdlp_info* info);
+/* The reference's solve_mps() PDLP stage after presolve (src/solve.cpp:
+ * 194-224) with every O(nnz) step on the GPU: build_pdlp_form, scale(), the
+ * solve, extract_solution, z = c - A^T y and verify() on lp itself; while the
+ * verifier's worst residual misses opt->tol (and the solve converged), it
+ * solves again with a 10x tighter internal tolerance, three attempts at most.
+ * lp is the LP as the reference's RangedLP (any row bounds); x_out (n),
+ * y_out (m) and z_out (n, may be NULL) are in lp's coordinates; report7 is
+ * verify()'s report on lp (as supr_verify's out7); info->iterations sums the
+ * attempts of every solve.  Device-resident modes only. */
+int supr_solve_lp_gpu(const supr_lp* lp, const supr_pdlp_options* opt, supr_stop_fn stop,
+                      void* stop_ctx, double* x_out, double* y_out, double* z_out,
+                      supr_pdlp_info* info, double* report7);
 /* A resident engine built the same way (see supr_engine_create). */
 int supr_engine_create_form(const supr_lp* form, const char* integer, int32_t ruiz_passes,
                             const supr_pdlp_options* opt, supr_engine** engine);
@@ -227,6 +239,18 @@ int supr_scale_gpu(const supr_lp* lp, const char* integer, int32_t ruiz_passes, 
  * = COO position of each entry), equal to the host counting sort's. */
 int supr_csr_gpu(const supr_lp* lp, int32_t transposed, int32_t device, int32_t* ptr,
                  int32_t* idx, int32_t* perm);
+
+/* build_pdlp_form() on the GPU (src/pdlp.cpp:8-82): the outputs of
+ * supr_pdlp_form (sized by supr_pdlp_form_sizes), bit for bit; dims4 (may be
+ * NULL) receives m2, n2, nnz2, n_slack as computed on the device. */
+int supr_pdlp_form_gpu(const supr_lp* lp, int32_t device, int32_t* row_idx2, int32_t* col_idx2,
+                       double* val2, double* c2, double* rL2, double* rU2, double* l2, double* u2,
+                       int32_t* row_perm, double* row_signs, int32_t* dims4);
+
+/* verify() (src/verify.cpp:24-103) with A x and A^T y formed on the GPU in
+ * the host's term order: the same out7 as supr_verify, bit for bit. */
+int supr_verify_gpu(const supr_lp* lp, const double* x, const double* y, int32_t device,
+                    double* out7);
 
 /* build_pdlp_form(): dims4 = m2, n2, nnz2, n_slack. */
 int supr_pdlp_form_sizes(const supr_lp* lp, int32_t* dims4);

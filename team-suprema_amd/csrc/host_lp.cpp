@@ -92,10 +92,15 @@ double finite_inf_norm(const double* v, int32_t len) {
 
 void verify_csr(const supr_lp& lp, const Csr& A, const Csr& AT, const double* x,
                 const double* y, double* out7) {
-    const int32_t m = lp.m, n = lp.n;
-    std::vector<double> ax(m), aty(n);
+    std::vector<double> ax(lp.m), aty(lp.n);
     csr_matvec(A, x, ax.data());
     csr_matvec(AT, y, aty.data());
+    verify_products(lp, x, y, ax.data(), aty.data(), out7);
+}
+
+void verify_products(const supr_lp& lp, const double* x, const double* y, const double* ax,
+                     const double* aty, double* out7) {
+    const int32_t m = lp.m, n = lp.n;
 
     // Primal: row and bound violations, 2-norm over rows then columns.
     double sp = 0.0, max_rv = 0.0, max_bv = 0.0;

@@ -41,4 +41,9 @@ bool check_lp(const supr_lp* lp, const char* what);
 void verify_csr(const supr_lp& lp, const Csr& A, const Csr& AT, const double* x,
                 const double* y, double* out7);
 
+// The same from the products ax = A x and aty = A^T y (left-to-right row
+// sums of the CSR / CSC): the O(m + n) tail of verify(), src/verify.cpp:34-103.
+void verify_products(const supr_lp& lp, const double* x, const double* y, const double* ax,
+                     const double* aty, double* out7);
+
 }  // namespace supr

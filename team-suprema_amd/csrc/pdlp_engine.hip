@@ -274,6 +274,17 @@ class Solver {
         if (integer) integer_.assign(integer, integer + U.n);
         dlp_ = std::make_unique<DevLp>(U, stream_.s);   // uploads the caller's arrays now
     }
+    // From a form already on the device (built there from the original LP);
+    // Uvec carries the form's sizes and host vectors (null COO pointers).
+    Solver(std::unique_ptr<DevLp> form, const supr_lp& Uvec, const supr_pdlp_options& o)
+        : Uown_(Uvec, false), Sown_(Uvec, false), Drv_(Uvec.m), Dcv_(Uvec.n), U_(Uown_.v),
+          S_(Sown_.v), Dr_(Drv_.data()), Dc_(Dcv_.data()), o_(o), m_(Uvec.m), n_(Uvec.n),
+          device_prep_(true), passes_(10) {
+        if (o.mode == SUPR_MODE_PLAIN_FP64)
+            throw BadInput("the plain FP64 mode takes host forms (supr_solve_pdhg_gpu)");
+        dlp_ = std::move(form);
+    }
+    void set_tol(double tol) { o_.tol = tol; }
     ~Solver() {
         for (auto& g : graph_) if (g) (void)hipGraphExecDestroy(g);
         if (mirror_) (void)hipHostFree(mirror_);
@@ -1195,6 +1206,97 @@ int supr_solve_form_gpu(const supr_lp* form, const char* integer, int32_t ruiz_p
         Solver solver(*form, integer, ruiz_passes, *opt);
         solver.setup();
         solver.run(stop, stop_ctx, x_out, y_out, info);
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+// The reference's solve_mps() PDLP stage after presolve (src/solve.cpp:194-
+// 224), with every O(nnz) step on the device: build_pdlp_form, scale(), the
+// solve, then extract_solution, z = c - A^T y and verify() on the original
+// LP; while the verifier misses tol, solve again with a 10x tighter internal
+// tolerance (three attempts at most).
+int supr_solve_lp_gpu(const supr_lp* lp, const supr_pdlp_options* opt, supr_stop_fn stop,
+                      void* stop_ctx, double* x_out, double* y_out, double* z_out,
+                      supr_pdlp_info* info, double* report7) {
+    using namespace supr;
+    if (!check_lp(lp, "supr_solve_lp_gpu")) return SUPR_EINVAL;
+    if (!opt || !x_out || !y_out || !info || !report7) { set_error("null argument"); return SUPR_EINVAL; }
+    if (opt->mode == SUPR_MODE_PLAIN_FP64) {
+        set_error("supr_solve_lp_gpu runs the device-resident modes");
+        return SUPR_EINVAL;
+    }
+    try {
+        SUPR_HIP(hipSetDevice(opt->device));
+        StreamGuard sg;
+        hipStream_t s = sg.s;
+        const auto t0 = Clock::now();
+        DevLp orig(*lp, s);   // the original LP, laid out for the verifier
+        DevForm F;
+        build_form_on_device(orig, F, s);
+        // the form's vectors on the host (sizes, objective, KKT denominators)
+        std::vector<double> c2(F.n2), l2(F.n2), u2(F.n2), rL2(F.m2), rU2(F.m2), signs(F.m2);
+        std::vector<int32_t> perm(F.m2);
+        auto down = [&](void* dst, const DevBuf& src, size_t bytes) {
+            if (bytes > 0) SUPR_HIP(hipMemcpyAsync(dst, src.as<void>(), bytes, hipMemcpyDeviceToHost, s));
+        };
+        down(c2.data(), F.c, 8 * c2.size()); down(l2.data(), F.l, 8 * l2.size());
+        down(u2.data(), F.u, 8 * u2.size()); down(rL2.data(), F.rL, 8 * rL2.size());
+        down(rU2.data(), F.rU, 8 * rU2.size()); down(signs.data(), F.row_signs, 8 * signs.size());
+        down(perm.data(), F.row_perm, 4 * perm.size());
+        SUPR_HIP(hipStreamSynchronize(s));
+        supr_lp fv{};
+        fv.m = F.m2; fv.n = F.n2; fv.nnz = F.nnz2;
+        fv.c = c2.data(); fv.l = l2.data(); fv.u = u2.data(); fv.rL = rL2.data(); fv.rU = rU2.data();
+        fv.obj_offset = lp->obj_offset;
+        if (fv.m < 1 || fv.n < 1) throw BadInput("the PDLP form needs at least one row and one column");
+        auto fdev = std::make_unique<DevLp>(F.m2, F.n2, F.nnz2, std::move(F.row), std::move(F.col),
+                                            std::move(F.val), std::move(F.c), std::move(F.l),
+                                            std::move(F.u), std::move(F.rL), std::move(F.rU), s);
+        int rc = check_options(&fv, opt);
+        if (rc != SUPR_OK) return rc;
+        Solver solver(std::move(fdev), fv, *opt);
+        solver.setup();
+        const double setup_s = seconds_since(t0);
+        std::vector<double> x2(F.n2), y2(F.m2);
+        DevBuf dx = device_vec<double>(lp->n), dy = device_vec<double>(lp->m);
+        DevBuf dax = device_vec<double>(lp->m), daty = device_vec<double>(lp->n);
+        std::vector<double> ax(lp->m), aty(lp->n);
+        int total = 0, restarts = 0;
+        double inner = opt->tol, solve_s = 0.0;
+        supr_pdlp_info step{};
+        for (int attempt = 0; attempt < 3; ++attempt, inner *= 0.1) {
+            solver.set_tol(inner);
+            solver.run(stop, stop_ctx, x2.data(), y2.data(), &step);
+            total += step.iterations;
+            restarts = step.restarts;
+            solve_s += step.seconds_solve;
+            // extract_solution (src/pdlp.cpp:84-90)
+            std::copy(x2.begin(), x2.begin() + lp->n, x_out);
+            std::fill(y_out, y_out + lp->m, 0.0);
+            for (int32_t p = 0; p < F.m2; ++p) y_out[perm[p]] = signs[p] * y2[p];
+            if (lp->n > 0) SUPR_HIP(hipMemcpyAsync(dx.as<void>(), x_out, 8 * (size_t)lp->n, hipMemcpyHostToDevice, s));
+            if (lp->m > 0) SUPR_HIP(hipMemcpyAsync(dy.as<void>(), y_out, 8 * (size_t)lp->m, hipMemcpyHostToDevice, s));
+            verify_products_on_device(orig, dx.as<double>(), dy.as<double>(), dax.as<double>(),
+                                      daty.as<double>(), s);
+            down(ax.data(), dax, 8 * ax.size());
+            down(aty.data(), daty, 8 * aty.size());
+            SUPR_HIP(hipStreamSynchronize(s));
+            verify_products(*lp, x_out, y_out, ax.data(), aty.data(), report7);
+            if (z_out)
+                for (int32_t j = 0; j < lp->n; ++j) z_out[j] = lp->c[j] - aty[j];
+            const double worst = std::max({report7[0], report7[1], report7[2]});
+            if (!step.converged || worst <= opt->tol) break;
+        }
+        *info = step;
+        info->iterations = total;
+        info->restarts = restarts;
+        info->eps_P = report7[0]; info->eps_D = report7[1]; info->eps_G = report7[2];
+        info->objective = report7[3];
+        info->seconds_setup = setup_s;
+        info->seconds_solve = solve_s;
+        info->seconds_total = seconds_since(t0);
         return SUPR_OK;
     } catch (...) {
         return map_exception();

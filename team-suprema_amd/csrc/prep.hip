@@ -131,7 +131,7 @@ DevBuf copy_of(const DevBuf& src, size_t bytes, hipStream_t s) {
 
 }  // namespace
 
-DevLp::DevLp(const supr_lp& lp, hipStream_t s) : m(lp.m), n(lp.n), nnz(lp.nnz) {
+DevLp::DevLp(const supr_lp& lp, hipStream_t s, bool with_layout) : m(lp.m), n(lp.n), nnz(lp.nnz) {
     row = upload(lp.row_idx, nnz, s);
     col = upload(lp.col_idx, nnz, s);
     val = upload(lp.val, nnz, s);
@@ -140,6 +140,17 @@ DevLp::DevLp(const supr_lp& lp, hipStream_t s) : m(lp.m), n(lp.n), nnz(lp.nnz) {
     u = upload(lp.u, n, s);
     rL = upload(lp.rL, m, s);
     rU = upload(lp.rU, m, s);
+    if (with_layout) layout(s);
+}
+
+DevLp::DevLp(int32_t m_, int32_t n_, int32_t nnz_, DevBuf&& row_, DevBuf&& col_, DevBuf&& val_,
+             DevBuf&& c_, DevBuf&& l_, DevBuf&& u_, DevBuf&& rL_, DevBuf&& rU_, hipStream_t s)
+    : m(m_), n(n_), nnz(nnz_), row(std::move(row_)), col(std::move(col_)), val(std::move(val_)),
+      c(std::move(c_)), l(std::move(l_)), u(std::move(u_)), rL(std::move(rL_)), rU(std::move(rU_)) {
+    layout(s);
+}
+
+void DevLp::layout(hipStream_t s) {
     build_pattern(A, row.as<int32_t>(), col.as<int32_t>(), nnz, m, n, s);
     build_pattern(AT, col.as<int32_t>(), row.as<int32_t>(), nnz, n, m, s);
     gather_values(A, val.as<double>(), vA, s);
@@ -217,6 +228,107 @@ void scale_on_device(const DevLp& lp, const char* integer, int ruiz_passes, DevS
     SUPR_HIP(hipStreamSynchronize(s));   // temporaries are freed on return
 }
 
+void build_form_on_device(const DevLp& lp, DevForm& f, hipStream_t s) {
+    const int32_t m = lp.m, n = lp.n, nnz = lp.nnz;
+    f.m_orig = m;
+    f.n_orig = n;
+    DevBuf fE = device_vec<int32_t>(m), fI = device_vec<int32_t>(m), fR = device_vec<int32_t>(m);
+    DevBuf sE = device_vec<int32_t>(m), sI = device_vec<int32_t>(m), sR = device_vec<int32_t>(m);
+    int32_t tot[3] = {0, 0, 0};
+    if (m > 0) {
+        hipLaunchKernelGGL(kf_flags, dim3(grid_for(m)), dim3(kThreads), 0, s, m,
+                           lp.rL.as<const double>(), lp.rU.as<const double>(), fE.as<int32_t>(),
+                           fI.as<int32_t>(), fR.as<int32_t>());
+        SUPR_HIP(hipGetLastError());
+        exclusive_scan(fE.as<int32_t>(), sE.as<int32_t>(), m, s);
+        exclusive_scan(fI.as<int32_t>(), sI.as<int32_t>(), m, s);
+        exclusive_scan(fR.as<int32_t>(), sR.as<int32_t>(), m, s);
+        DevBuf dt = device_vec<int32_t>(3);
+        hipLaunchKernelGGL(kf_totals, dim3(1), dim3(64), 0, s, m, sE.as<const int32_t>(),
+                           fE.as<const int32_t>(), sI.as<const int32_t>(), fI.as<const int32_t>(),
+                           sR.as<const int32_t>(), fR.as<const int32_t>(), dt.as<int32_t>());
+        SUPR_HIP(hipGetLastError());
+        SUPR_HIP(hipMemcpyAsync(tot, dt.as<void>(), sizeof(tot), hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+    }
+    const int32_t n_eq = tot[0], n_ineq = tot[1];
+    f.n_slack = tot[2];
+    f.m2 = n_eq + n_ineq;
+    f.n2 = n + f.n_slack;
+    f.row_perm = device_vec<int32_t>(f.m2);
+    f.row_signs = device_vec<double>(f.m2);
+    f.rL = device_vec<double>(f.m2);
+    f.rU = device_vec<double>(f.m2);
+    f.c = device_vec<double>(f.n2);
+    f.l = device_vec<double>(f.n2);
+    f.u = device_vec<double>(f.n2);
+    if (n > 0) {
+        SUPR_HIP(hipMemcpyAsync(f.c.as<void>(), lp.c.as<void>(), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        SUPR_HIP(hipMemcpyAsync(f.l.as<void>(), lp.l.as<void>(), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        SUPR_HIP(hipMemcpyAsync(f.u.as<void>(), lp.u.as<void>(), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    if (f.n_slack > 0)
+        hipLaunchKernelGGL(kp_fill, dim3(grid_for(f.n_slack)), dim3(kThreads), 0, s,
+                           f.c.as<double>() + n, f.n_slack, 0.0);
+    DevBuf pos_of = device_vec<int32_t>(m), slack_pos = device_vec<int32_t>(f.n_slack);
+    if (m > 0)
+        hipLaunchKernelGGL(kf_rows, dim3(grid_for(m)), dim3(kThreads), 0, s, m, n, n_eq,
+                           lp.rL.as<const double>(), lp.rU.as<const double>(),
+                           sE.as<const int32_t>(), sI.as<const int32_t>(), sR.as<const int32_t>(),
+                           f.row_perm.as<int32_t>(), f.row_signs.as<double>(), f.rL.as<double>(),
+                           f.rU.as<double>(), pos_of.as<int32_t>(), slack_pos.as<int32_t>(),
+                           f.l.as<double>(), f.u.as<double>());
+    SUPR_HIP(hipGetLastError());
+    // entries: the kept ones in COO order, then one per slack
+    int32_t kept = 0;
+    DevBuf fK = device_vec<int32_t>(nnz), q = device_vec<int32_t>(nnz);
+    if (nnz > 0) {
+        hipLaunchKernelGGL(kf_keep, dim3(grid_for(nnz)), dim3(kThreads), 0, s, nnz,
+                           lp.row.as<const int32_t>(), pos_of.as<const int32_t>(), fK.as<int32_t>());
+        SUPR_HIP(hipGetLastError());
+        exclusive_scan(fK.as<int32_t>(), q.as<int32_t>(), nnz, s);
+        int32_t last[2];
+        SUPR_HIP(hipMemcpyAsync(&last[0], q.as<int32_t>() + nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipMemcpyAsync(&last[1], fK.as<int32_t>() + nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+        kept = last[0] + last[1];
+    }
+    f.nnz2 = kept + f.n_slack;
+    f.row = device_vec<int32_t>(f.nnz2);
+    f.col = device_vec<int32_t>(f.nnz2);
+    f.val = device_vec<double>(f.nnz2);
+    if (nnz > 0)
+        hipLaunchKernelGGL(kf_entries, dim3(grid_for(nnz)), dim3(kThreads), 0, s, nnz,
+                           lp.row.as<const int32_t>(), lp.col.as<const int32_t>(),
+                           lp.val.as<const double>(), pos_of.as<const int32_t>(),
+                           f.row_signs.as<const double>(), fK.as<const int32_t>(),
+                           q.as<const int32_t>(), f.row.as<int32_t>(), f.col.as<int32_t>(),
+                           f.val.as<double>());
+    if (f.n_slack > 0)
+        hipLaunchKernelGGL(kf_slack_entries, dim3(grid_for(f.n_slack)), dim3(kThreads), 0, s,
+                           f.n_slack, kept, n, slack_pos.as<const int32_t>(), f.row.as<int32_t>(),
+                           f.col.as<int32_t>(), f.val.as<double>());
+    SUPR_HIP(hipGetLastError());
+    SUPR_HIP(hipStreamSynchronize(s));   // temporaries are freed on return
+}
+
+void verify_products_on_device(const DevLp& lp, const double* x, const double* y, double* ax,
+                               double* aty, hipStream_t s) {
+    auto run = [&](const DevPattern& P, const DevBuf& v, const double* in, double* out) {
+        if (P.rows == 0) return;
+        hipLaunchKernelGGL(kp_spmv_seq_short, dim3(grid_for(P.rows)), dim3(kThreads), 0, s, P.rows,
+                           P.ptr.as<const int32_t>(), P.idx.as<const int32_t>(), v.as<const double>(),
+                           in, out);
+        if (P.max_len > kLongRow)
+            hipLaunchKernelGGL(kp_spmv_seq_long, dim3(grid_for(P.rows, kThreads / 64)), dim3(kThreads),
+                               0, s, P.rows, P.ptr.as<const int32_t>(), P.idx.as<const int32_t>(),
+                               v.as<const double>(), in, out);
+        SUPR_HIP(hipGetLastError());
+    };
+    run(lp.A, lp.vA, x, ax);
+    run(lp.AT, lp.vAT, y, aty);
+}
+
 double device_absmax(const double* v, int32_t len, hipStream_t s) {
     if (len <= 0) return 0.0;
     const int g = grid_for(len);
@@ -288,6 +400,82 @@ int supr_scale_gpu(const supr_lp* lp, const char* integer, int32_t ruiz_passes, 
             seconds3[1] = t_scale;
             seconds3[2] = seconds_since(t2);
         }
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+// build_pdlp_form on the device; outputs as supr_pdlp_form's.
+int supr_pdlp_form_gpu(const supr_lp* lp, int32_t device, int32_t* row_idx2, int32_t* col_idx2,
+                       double* val2, double* c2, double* rL2, double* rU2, double* l2, double* u2,
+                       int32_t* row_perm, double* row_signs, int32_t* dims4) {
+    using namespace supr;
+    if (!check_lp(lp, "supr_pdlp_form_gpu")) return SUPR_EINVAL;
+    hipStream_t s = nullptr;
+    try {
+        SUPR_HIP(hipSetDevice(device));
+        SUPR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamOwner {
+            hipStream_t s;
+            ~StreamOwner() { (void)hipStreamDestroy(s); }
+        } own{s};
+        DevLp D(*lp, s, false);
+        DevForm F;
+        build_form_on_device(D, F, s);
+        if (dims4) { dims4[0] = F.m2; dims4[1] = F.n2; dims4[2] = F.nnz2; dims4[3] = F.n_slack; }
+        auto down = [&](void* dst, const DevBuf& src, size_t bytes) {
+            if (bytes > 0) {
+                if (!dst) throw BadInput("supr_pdlp_form_gpu: null output");
+                SUPR_HIP(hipMemcpyAsync(dst, src.as<void>(), bytes, hipMemcpyDeviceToHost, s));
+            }
+        };
+        down(row_idx2, F.row, sizeof(int32_t) * F.nnz2);
+        down(col_idx2, F.col, sizeof(int32_t) * F.nnz2);
+        down(val2, F.val, sizeof(double) * F.nnz2);
+        down(c2, F.c, sizeof(double) * F.n2);
+        down(l2, F.l, sizeof(double) * F.n2);
+        down(u2, F.u, sizeof(double) * F.n2);
+        down(rL2, F.rL, sizeof(double) * F.m2);
+        down(rU2, F.rU, sizeof(double) * F.m2);
+        down(row_perm, F.row_perm, sizeof(int32_t) * F.m2);
+        down(row_signs, F.row_signs, sizeof(double) * F.m2);
+        SUPR_HIP(hipStreamSynchronize(s));
+        return SUPR_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+// verify() with its two products on the device (left-to-right row sums, as
+// the host's) and the O(m + n) tail on the host: the same report, bit for bit.
+int supr_verify_gpu(const supr_lp* lp, const double* x, const double* y, int32_t device,
+                    double* out7) {
+    using namespace supr;
+    if (!check_lp(lp, "supr_verify_gpu")) return SUPR_EINVAL;
+    if ((lp->n > 0 && !x) || (lp->m > 0 && !y) || !out7) {
+        set_error("supr_verify_gpu: null x, y or out");
+        return SUPR_EINVAL;
+    }
+    hipStream_t s = nullptr;
+    try {
+        SUPR_HIP(hipSetDevice(device));
+        SUPR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamOwner {
+            hipStream_t s;
+            ~StreamOwner() { (void)hipStreamDestroy(s); }
+        } own{s};
+        DevLp D(*lp, s);
+        DevBuf dx = upload(x, lp->n, s), dy = upload(y, lp->m, s);
+        DevBuf ax = device_vec<double>(lp->m), aty = device_vec<double>(lp->n);
+        verify_products_on_device(D, dx.as<double>(), dy.as<double>(), ax.as<double>(), aty.as<double>(), s);
+        std::vector<double> hax(lp->m), haty(lp->n);
+        if (lp->m > 0)
+            SUPR_HIP(hipMemcpyAsync(hax.data(), ax.as<void>(), sizeof(double) * lp->m, hipMemcpyDeviceToHost, s));
+        if (lp->n > 0)
+            SUPR_HIP(hipMemcpyAsync(haty.data(), aty.as<void>(), sizeof(double) * lp->n, hipMemcpyDeviceToHost, s));
+        SUPR_HIP(hipStreamSynchronize(s));
+        verify_products(*lp, x, y, hax.data(), haty.data(), out7);
         return SUPR_OK;
     } catch (...) {
         return map_exception();

@@ -21,12 +21,30 @@ struct DevPattern {
 // A (rows) and A^T (columns), and A's values in both orders.
 class DevLp {
    public:
-    DevLp(const supr_lp& lp, hipStream_t s);
+    // Uploads lp (validated by the caller); with_layout builds A, AT, vA, vAT.
+    DevLp(const supr_lp& lp, hipStream_t s, bool with_layout = true);
+    // Takes over device arrays of an LP with the given sizes and lays it out.
+    DevLp(int32_t m, int32_t n, int32_t nnz, DevBuf&& row, DevBuf&& col, DevBuf&& val, DevBuf&& c,
+          DevBuf&& l, DevBuf&& u, DevBuf&& rL, DevBuf&& rU, hipStream_t s);
     int32_t m = 0, n = 0, nnz = 0;
     DevBuf row, col, val, c, l, u, rL, rU;
     DevPattern A, AT;
     DevBuf vA, vAT;   // FP64 values in CSR (A) and CSC (A^T) order
+    void layout(hipStream_t s);
 };
+
+// build_pdlp_form (src/pdlp.cpp:8-82) of a DevLp, on the device: the form's
+// COO arrays and vectors, and the meta row_perm / row_signs (m2 entries).
+struct DevForm {
+    int32_t m2 = 0, n2 = 0, nnz2 = 0, n_slack = 0, n_orig = 0, m_orig = 0;
+    DevBuf row, col, val, c, l, u, rL, rU, row_perm, row_signs;
+};
+void build_form_on_device(const DevLp& lp, DevForm& f, hipStream_t s);
+
+// ax = A x and aty = A^T y (device vectors) over a laid-out DevLp, each row
+// summed left to right in COO order like the host verifier's matvecs.
+void verify_products_on_device(const DevLp& lp, const double* x, const double* y, double* ax,
+                               double* aty, hipStream_t s);
 
 // scale() of a DevLp on the device.
 struct DevScaled {

@@ -422,5 +422,179 @@ __global__ __launch_bounds__(kThreads) void kp_absmax(const double* __restrict__
     }
 }
 
+// ---------------------------------------------------- verifier products --
+
+// out[row] = sum_p v[p] * x[idx[p]] left to right, each product rounded and
+// then added, as the host's sequential matvec (include/sparse.hpp:59-66):
+// one thread per row of at most kLongRow entries ...
+__global__ __launch_bounds__(kThreads) void kp_spmv_seq_short(int32_t rows,
+                                                              const int32_t* __restrict__ ptr,
+                                                              const int32_t* __restrict__ idx,
+                                                              const double* __restrict__ v,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ out) {
+    for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < rows;
+         row += (int64_t)gridDim.x * kThreads) {
+        const int32_t b = ptr[row], e = ptr[row + 1];
+        if (e - b > kLongRow) continue;
+        double sum = 0.0;
+        for (int32_t p = b; p < e; ++p) sum = sum + v[p] * x[idx[p]];
+        out[row] = sum;
+    }
+}
+
+// ... and one wavefront per longer row (each lane forms one product of a
+// 64-term chunk; the chain adds them in lane order).
+__global__ __launch_bounds__(kThreads) void kp_spmv_seq_long(int32_t rows,
+                                                             const int32_t* __restrict__ ptr,
+                                                             const int32_t* __restrict__ idx,
+                                                             const double* __restrict__ v,
+                                                             const double* __restrict__ x,
+                                                             double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    constexpr int W = kThreads / 64;
+    for (int64_t row = (int64_t)blockIdx.x * W + (threadIdx.x >> 6); row < rows;
+         row += (int64_t)gridDim.x * W) {
+        const int32_t b = __builtin_amdgcn_readfirstlane(ptr[row]);
+        const int32_t e = __builtin_amdgcn_readfirstlane(ptr[row + 1]);
+        if (e - b <= kLongRow) continue;
+        double sum = 0.0;
+        for (int32_t c0 = b; c0 < e; c0 += 64) {
+            const int32_t p = c0 + lane;
+            const double t = p < e ? v[p] * x[idx[p]] : 0.0;
+            const int cnt = e - c0 < 64 ? e - c0 : 64;
+            for (int k = 0; k < cnt; ++k) sum = sum + readlane_f64(t, k);
+        }
+        if (lane == 0) out[row] = sum;
+    }
+}
+
+// z = c - aty (src/verify.cpp:33).
+__global__ __launch_bounds__(kThreads) void kp_sub(int32_t len, const double* __restrict__ c,
+                                                   const double* __restrict__ a,
+                                                   double* __restrict__ z) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < len;
+         i += (int64_t)gridDim.x * kThreads)
+        z[i] = c[i] - a[i];
+}
+
+// ------------------------------------------------------------ PDLP form --
+// build_pdlp_form (src/pdlp.cpp:8-82): equality and ranged rows first, in
+// their original order; then <= rows (negated into >=) and >= rows; free rows
+// dropped; a ranged row becomes an equality row with b = 0 and a slack
+// column (coefficient -1) carrying the range.
+
+enum FormRowKind : int { kRowEq = 0, kRowRanged, kRowLe, kRowGe, kRowFree };
+
+__device__ __forceinline__ int form_row_kind(double lo, double hi) {
+    if (lo == hi) return kRowEq;
+    if (isfinite(lo) && isfinite(hi)) return kRowRanged;
+    if (isfinite(hi)) return kRowLe;
+    if (isfinite(lo)) return kRowGe;
+    return kRowFree;
+}
+
+// fE: equality or ranged; fI: <= or >=; fR: ranged.
+__global__ __launch_bounds__(kThreads) void kf_flags(int32_t m, const double* __restrict__ rL,
+                                                     const double* __restrict__ rU,
+                                                     int32_t* __restrict__ fE,
+                                                     int32_t* __restrict__ fI,
+                                                     int32_t* __restrict__ fR) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * kThreads) {
+        const int k = form_row_kind(rL[i], rU[i]);
+        fE[i] = (k == kRowEq || k == kRowRanged);
+        fI[i] = (k == kRowLe || k == kRowGe);
+        fR[i] = (k == kRowRanged);
+    }
+}
+
+// totals[t] = scan[t][len-1] + flag[t][len-1] for the three scans (len > 0).
+__global__ void kf_totals(int32_t len, const int32_t* __restrict__ sE,
+                          const int32_t* __restrict__ fE, const int32_t* __restrict__ sI,
+                          const int32_t* __restrict__ fI, const int32_t* __restrict__ sR,
+                          const int32_t* __restrict__ fR, int32_t* __restrict__ totals) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        totals[0] = sE[len - 1] + fE[len - 1];
+        totals[1] = sI[len - 1] + fI[len - 1];
+        totals[2] = sR[len - 1] + fR[len - 1];
+    }
+}
+
+// Row placement: row_perm, row_signs, rL2, rU2 (m2 entries), pos_of (m),
+// slack_pos (the form row of slack s) and the slack bounds l2, u2[n + s].
+__global__ __launch_bounds__(kThreads) void kf_rows(
+    int32_t m, int32_t n, int32_t n_eq, const double* __restrict__ rL,
+    const double* __restrict__ rU, const int32_t* __restrict__ posE,
+    const int32_t* __restrict__ posI, const int32_t* __restrict__ sid,
+    int32_t* __restrict__ row_perm, double* __restrict__ row_signs, double* __restrict__ rL2,
+    double* __restrict__ rU2, int32_t* __restrict__ pos_of, int32_t* __restrict__ slack_pos,
+    double* __restrict__ l2, double* __restrict__ u2) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double lo = rL[i], hi = rU[i];
+        const int k = form_row_kind(lo, hi);
+        if (k == kRowFree) { pos_of[i] = -1; continue; }
+        const int32_t p = (k == kRowEq || k == kRowRanged) ? posE[i] : n_eq + posI[i];
+        pos_of[i] = p;
+        row_perm[p] = (int32_t)i;
+        row_signs[p] = k == kRowLe ? -1.0 : 1.0;
+        if (k == kRowEq) {
+            rL2[p] = lo;
+            rU2[p] = lo;
+        } else if (k == kRowRanged) {
+            rL2[p] = 0.0;
+            rU2[p] = 0.0;
+            const int32_t sl = sid[i];
+            slack_pos[sl] = p;
+            l2[n + sl] = lo;
+            u2[n + sl] = hi;
+        } else {
+            rL2[p] = k == kRowLe ? -hi : lo;
+            rU2[p] = INFINITY;
+        }
+    }
+}
+
+// fK[k] = the entry's row is kept.
+__global__ __launch_bounds__(kThreads) void kf_keep(int32_t nnz, const int32_t* __restrict__ row,
+                                                    const int32_t* __restrict__ pos_of,
+                                                    int32_t* __restrict__ fK) {
+    for (int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * kThreads)
+        fK[k] = pos_of[row[k]] >= 0;
+}
+
+// Kept entries in COO order (q = exclusive scan of fK), then the slacks'.
+__global__ __launch_bounds__(kThreads) void kf_entries(
+    int32_t nnz, const int32_t* __restrict__ row, const int32_t* __restrict__ col,
+    const double* __restrict__ val, const int32_t* __restrict__ pos_of,
+    const double* __restrict__ row_signs, const int32_t* __restrict__ fK,
+    const int32_t* __restrict__ q, int32_t* __restrict__ row2, int32_t* __restrict__ col2,
+    double* __restrict__ val2) {
+    for (int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * kThreads) {
+        if (!fK[k]) continue;
+        const int32_t p = pos_of[row[k]], Q = q[k];
+        row2[Q] = p;
+        col2[Q] = col[k];
+        val2[Q] = row_signs[p] * val[k];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void kf_slack_entries(int32_t n_slack, int32_t base,
+                                                             int32_t n,
+                                                             const int32_t* __restrict__ slack_pos,
+                                                             int32_t* __restrict__ row2,
+                                                             int32_t* __restrict__ col2,
+                                                             double* __restrict__ val2) {
+    for (int64_t sl = (int64_t)blockIdx.x * kThreads + threadIdx.x; sl < n_slack;
+         sl += (int64_t)gridDim.x * kThreads) {
+        row2[base + sl] = slack_pos[sl];
+        col2[base + sl] = n + (int32_t)sl;
+        val2[base + sl] = -1.0;
+    }
+}
+
 }  // namespace prep
 }  // namespace supr

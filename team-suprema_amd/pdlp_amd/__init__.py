@@ -17,7 +17,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "scale_gpu", "csr_gpu", "solve_form_gpu", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
+    "extract_solution", "scale_gpu", "csr_gpu", "solve_form_gpu", "build_pdlp_form_gpu", "verify_gpu", "solve_lp_gpu", "LpSolution", "certified_status", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -200,6 +200,40 @@ def build_pdlp_form(lp):
     return form, PdlpFormMeta(lp.n, lp.m, n_slack, perm, signs)
 
 
+def build_pdlp_form_gpu(lp, device=0):
+    """build_pdlp_form() on the GPU: the same (form LP, meta) as
+    build_pdlp_form(), bit for bit."""
+    s = lp.cstruct()
+    dims = np.zeros(4, dtype=_i32)
+    check(lib().supr_pdlp_form_sizes(ctypes.byref(s), _ip(dims)))
+    m2, n2, nnz2, n_slack = (int(v) for v in dims)
+    ri, ci = np.zeros(nnz2, dtype=_i32), np.zeros(nnz2, dtype=_i32)
+    v = np.zeros(nnz2)
+    c, l, u = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+    rL, rU = np.zeros(m2), np.zeros(m2)
+    perm, signs = np.zeros(m2, dtype=_i32), np.zeros(m2)
+    got = np.zeros(4, dtype=_i32)
+    check(lib().supr_pdlp_form_gpu(ctypes.byref(s), int(device), _ip(ri), _ip(ci), _dp(v), _dp(c),
+                                   _dp(rL), _dp(rU), _dp(l), _dp(u), _ip(perm), _dp(signs),
+                                   _ip(got)))
+    if not np.array_equal(got, dims):
+        raise SupremaError(f"device form sizes {got} differ from {dims}")
+    form = LP(m2, n2, ri, ci, v, c, rL, rU, l, u, lp.obj_offset, lp.name + " (pdlp-form)")
+    return form, PdlpFormMeta(lp.n, lp.m, n_slack, perm, signs)
+
+
+def verify_gpu(lp, x, y, device=0):
+    """verify() with its products on the GPU: the same KKTReport as verify()."""
+    x = np.ascontiguousarray(x, dtype=_f64)
+    y = np.ascontiguousarray(y, dtype=_f64)
+    if x.shape != (lp.n,) or y.shape != (lp.m,):
+        raise ValueError("x must have length n and y length m")
+    out = np.zeros(7)
+    s = lp.cstruct()
+    check(lib().supr_verify_gpu(ctypes.byref(s), _dp(x), _dp(y), int(device), _dp(out)))
+    return KKTReport(*[float(v) for v in out])
+
+
 def extract_solution(meta, x2, y2):
     """extract_solution() -- src/pdlp.cpp:84-90."""
     x2 = np.ascontiguousarray(x2, dtype=_f64)
@@ -277,6 +311,50 @@ def solve_form_gpu(form, ruiz_passes=10, integer=None, max_iterations=200000, ch
     check(lib().supr_solve_form_gpu(ctypes.byref(sf), _flags(integer, form.n), int(ruiz_passes),
                                     ctypes.byref(o), cb, None, _dp(x), _dp(y), ctypes.byref(info)))
     return _result(info, x, y)
+
+
+@dataclass
+class LpSolution:
+    """The PDLP stage's answer on the original LP (src/solve.cpp:219-231):
+    x, y, z = c - A^T y, the verifier's report on the LP, the certified
+    status (src/solve.cpp:47-51 with opt_tol = 10 tol), total attempts."""
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    report: "KKTReport"
+    status: str
+    iterations: int
+    restarts: int
+    converged: bool
+    info: dict
+
+
+def certified_status(engine_status, eps, opt_tol):
+    """src/solve.cpp:47-51"""
+    if engine_status not in ("optimal", "near_optimal"):
+        return engine_status
+    if eps <= opt_tol:
+        return engine_status
+    return "near_optimal" if eps <= 1e-4 else "inaccurate"
+
+
+def solve_lp_gpu(lp, max_iterations=200000, check_every=64, tol=1e-6, stop_flag=None,
+                 mode=MODE_MIXED, **options):
+    """solve_mps()'s PDLP stage after presolve on the GPU: form, scale, solve,
+    extract, z, verify on lp, and the 10x-tighter retries."""
+    o = _options(mode, max_iterations, check_every, tol, **options)
+    x, y, z, rep = np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.n), np.zeros(7)
+    info = _lib.SuprInfo()
+    cb = _stop_cb(stop_flag)
+    s = lp.cstruct()
+    check(lib().supr_solve_lp_gpu(ctypes.byref(s), ctypes.byref(o), cb, None, _dp(x), _dp(y), _dp(z),
+                                  ctypes.byref(info), _dp(rep)))
+    report = KKTReport(*[float(v) for v in rep])
+    status = certified_status("optimal" if info.converged else "iteration_limit",
+                              max(report.eps_P, report.eps_D, report.eps_G), 10.0 * tol)
+    d = {k: getattr(info, k) for k, _ in _lib.SuprInfo._fields_}
+    return LpSolution(x, y, z, report, status, int(info.iterations), int(info.restarts),
+                      bool(info.converged), d)
 
 
 def _flags(integer, n):

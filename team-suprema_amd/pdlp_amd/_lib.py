@@ -96,9 +96,16 @@ SIGNATURES = [
     ("supr_solve_form_gpu", ctypes.c_int,
      [_LP, ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(SuprOptions), STOP_FN, ctypes.c_void_p,
       c_double_p, c_double_p, ctypes.POINTER(SuprInfo)]),
+    ("supr_solve_lp_gpu", ctypes.c_int,
+     [_LP, ctypes.POINTER(SuprOptions), STOP_FN, ctypes.c_void_p, c_double_p, c_double_p,
+      c_double_p, ctypes.POINTER(SuprInfo), c_double_p]),
     ("supr_engine_create_form", ctypes.c_int,
      [_LP, ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(SuprOptions),
       ctypes.POINTER(ctypes.c_void_p)]),
+    ("supr_pdlp_form_gpu", ctypes.c_int,
+     [_LP, ctypes.c_int32, c_int32_p, c_int32_p] + [c_double_p] * 6 + [c_int32_p, c_double_p,
+                                                                       c_int32_p]),
+    ("supr_verify_gpu", ctypes.c_int, [_LP, c_double_p, c_double_p, ctypes.c_int32, c_double_p]),
     ("supr_csr_gpu", ctypes.c_int,
      [_LP, ctypes.c_int32, ctypes.c_int32, c_int32_p, c_int32_p, c_int32_p]),
     ("supr_pdlp_form_sizes", ctypes.c_int, [_LP, c_int32_p]),

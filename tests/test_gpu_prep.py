@@ -188,3 +188,99 @@ def test_engine_from_form_at_1m_variables():
     say(f"1M vars: identical ({a.iterations} attempts); setup device-scaled "
         f"{a.info['seconds_setup']:.3f}s vs host scale {t_host_scale:.3f}s + setup "
         f"{b.info['seconds_setup']:.3f}s")
+
+
+def all_row_kinds_lp(seed=3):
+    """Equality, ranged, <=, >= and free rows (each kind about a fifth),
+    some rows longer than 256 entries."""
+    rng = np.random.default_rng(seed)
+    m, n = 40, 600
+    dense = rng.random((m, n)) < np.where(np.arange(m)[:, None] < 3, 0.7, 0.05)
+    rows, cols = np.nonzero(dense)
+    p = rng.permutation(rows.size)
+    rows, cols = rows[p].astype(np.int32), cols[p].astype(np.int32)
+    vals = rng.standard_normal(rows.size)
+    u = rng.uniform(1, 10, n)
+    a = np.zeros(m)
+    np.add.at(a, rows, vals * rng.uniform(0, 1, n)[cols] * u[cols])   # A x0, x0 inside the box
+    kind = np.arange(m) % 5
+    lo, hi = a - rng.uniform(0, 2, m), a + rng.uniform(0, 2, m)
+    rL = np.where(kind == 0, a, np.where((kind == 1) | (kind == 3), lo, -np.inf))
+    rU = np.where(kind == 0, a, np.where((kind == 1) | (kind == 2), hi, np.inf))
+    return P.LP(m, n, rows, cols, vals, rng.uniform(-1, 1, n), rL, rU, np.zeros(n), u, 0.5,
+                "all row kinds")
+
+
+def original_lps(golden):
+    from conftest import load_fixture
+    out = [load_fixture(nm)[0] for nm in ["afiro", "sc50a", "blend", "tiny_range", "share1b"]]
+    out += [P.gen_blend(30, 12, seed=3), all_row_kinds_lp()]
+    return out
+
+
+def test_pdlp_form_gpu_is_bit_identical(golden):
+    for lp in original_lps(golden):
+        hf, hm = P.build_pdlp_form(lp)
+        df, dm = P.build_pdlp_form_gpu(lp)
+        for nm in ("row", "col", "val", "c", "l", "u", "rL", "rU"):
+            assert np.asarray(getattr(hf, nm)).tobytes() == np.asarray(getattr(df, nm)).tobytes(), (lp.name, nm)
+        assert hm.row_perm.tobytes() == dm.row_perm.tobytes() and hm.row_signs.tobytes() == dm.row_signs.tobytes()
+        assert (hm.n_slack, hf.m, hf.n) == (dm.n_slack, df.m, df.n)
+    say("build_pdlp_form on the device: bit-identical on", len(original_lps(golden)), "LPs")
+
+
+def test_verify_gpu_is_bit_identical(golden):
+    rng = np.random.default_rng(5)
+    lps = original_lps(golden) + [dense_rows_lp(), P.gen_transport(1000, 1000, seed=42)]
+    for lp in lps:
+        x, y = rng.standard_normal(lp.n) * 3, rng.standard_normal(lp.m)
+        h, d = P.verify(lp, x, y), P.verify_gpu(lp, x, y)
+        assert bits_equal([h.eps_P, h.eps_D, h.eps_G, h.objective, h.dual_objective,
+                           h.max_row_violation, h.max_bound_violation],
+                          [d.eps_P, d.eps_D, d.eps_G, d.objective, d.dual_objective,
+                           d.max_row_violation, d.max_bound_violation]), lp.name
+
+
+def host_pipeline(lp, tol, mode, **kw):
+    """The reference's solve_mps() PDLP stage after presolve, on the host
+    helpers and the host-scaled solve: form, scale, solve, extract, verify,
+    10x-tighter retries (src/solve.cpp:194-224)."""
+    form, meta = P.build_pdlp_form(lp)
+    scaled, Dr, Dc = P.scale(form)
+    total, inner = 0, tol
+    for _ in range(3):
+        r = P.solve_pdhg_gpu(form, scaled, Dr, Dc, tol=inner, mode=mode, **kw)
+        total += r.iterations
+        x, y = P.extract_solution(meta, r.x, r.y)
+        rep = P.verify(lp, x, y)
+        if not r.converged or max(rep.eps_P, rep.eps_D, rep.eps_G) <= tol:
+            break
+        inner *= 0.1
+    return x, y, rep, total, r.converged
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_solve_lp_gpu_matches_host_pipeline(golden, mode):
+    """Form, scale, solve, extract, z, verify and retries with every O(nnz)
+    step on the device: the same x, y, report and attempt count as the host
+    pipeline, bit for bit; z = c - A^T y."""
+    for lp in original_lps(golden):
+        sol = P.solve_lp_gpu(lp, tol=1e-6, max_iterations=200000, mode=mode, **PIN)
+        x, y, rep, total, conv = host_pipeline(lp, 1e-6, mode, max_iterations=200000, **PIN)
+        assert sol.iterations == total and sol.converged == conv, (lp.name, sol.iterations, total)
+        assert bits_equal(sol.x, x) and bits_equal(sol.y, y), lp.name
+        assert bits_equal([sol.report.eps_P, sol.report.eps_D, sol.report.eps_G, sol.report.objective],
+                          [rep.eps_P, rep.eps_D, rep.eps_G, rep.objective]), lp.name
+        aty = P.spmv(lp, sol.y, transposed=True)
+        np.testing.assert_allclose(sol.z, lp.c - aty, rtol=1e-12, atol=1e-12)
+        assert sol.status == "optimal", (lp.name, sol.status, sol.report)
+    say(f"mode {mode}: solve_lp_gpu identical to the host pipeline on {len(original_lps(golden))} LPs")
+
+
+def test_solve_lp_gpu_refuses_plain_mode_and_bad_input():
+    lp = P.gen_random_lp(20, 50, 4, seed=1)
+    with pytest.raises(P.SupremaError):
+        P.solve_lp_gpu(lp, mode=P.MODE_PLAIN_FP64)
+    bad = replace(lp, col=np.where(np.arange(lp.nnz) == 3, 99, lp.col).astype(np.int32))
+    with pytest.raises(P.SupremaError):
+        P.solve_lp_gpu(bad)
