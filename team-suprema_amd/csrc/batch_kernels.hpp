@@ -225,13 +225,17 @@ __global__ __launch_bounds__(kBlock) void kb_primal(DevState* __restrict__ S, in
     const BatchInst I = g.inst[k];
     const int lb = b - I.on;
     DevState* Sk = S + 2 * k;
-    load_and_decide(Sk, p, part_m + 2 * (size_t)I.om, I.bm, dx2 + k, Q, lb == 0);
+    // the publisher is the instance's last block (the fewest rows): the step
+    // powers it forms at the end stay off the launch's critical path
+    const bool publisher = lb == I.bn - 1;
+    load_and_decide(Sk, p, part_m + 2 * (size_t)I.om, I.bm, dx2 + k, Q, publisher);
     if (!Q.active) return;
-    if (lb == 0 && threadIdx.x == 0) Sk[p ^ 1].decide_pending = 1;   // kb_dual will run
+    if (publisher && threadIdx.x == 0) Sk[p ^ 1].decide_pending = 1;   // kb_dual will run
     double acc[1];
     acc[0] = Q.phase == 0 ? primal_rows<float>(g, I, lb, Q, V, F) : primal_rows<double>(g, I, lb, Q, V, D);
     block_sum<1>(acc);
     if (threadIdx.x == 0) part_n[b] = acc[0];
+    if (publisher && threadIdx.x == 0) publish_step_powers(Sk, p, Q);
 }
 
 // Dual half (k_dual of pdlp_kernels.hpp) over one instance's rows.
@@ -286,7 +290,7 @@ __device__ __forceinline__ void dual_rows(const BatchGeo& g, const BatchInst& I,
     });
 }
 
-// Row side of an attempt; the first block of each instance also sums its
+// Row side of an attempt; the last block of each instance also sums its
 // primal partials into dx2[k].
 __global__ __launch_bounds__(kBlock) void kb_dual(const DevState* __restrict__ S, int p1,
                                                   BatchGeo g, BatchVals V, IterBufs<float> F,
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void kb_dual(const DevState* __restrict__ S
         part_m[(size_t)b * 2] = acc[0];
         part_m[(size_t)b * 2 + 1] = acc[1];
     }
-    if (lb == 0) {
+    if (lb == I.bm - 1) {   // the instance's last block has the fewest rows
         double sn[1];
         sum_partials<1>(part_n + I.on, I.bn, sn);
         if (threadIdx.x == 0) dx2[k] = sn[0];

@@ -54,6 +54,10 @@ struct DevState {
     double last_obj;
     double tol, switch_tol;
     double denom_p, denom_d, obj_offset;
+    // (accepted + 2)^-0.3 and ^-0.6 for the next step decision, formed by a
+    // block of the previous primal launch off the decision's critical path;
+    // valid when pw_k1 == accepted + 2 (otherwise decide_step calls pow)
+    double pw3, pw6;
     int32_t attempts, accepted, since_restart, restarts, checks;
     int32_t max_iter, check_every, stall_checks, stall;
     int32_t cur;             // which half of each double buffer is current
@@ -66,7 +70,7 @@ struct DevState {
     int32_t has_best, best_which, use_avg, restart_now;
     int32_t switch_req, phase;   // switch_req: reason code (1 merit, 2 stall near floor, 3 long stall); phase 0: FP32 iterates, 1: FP64
     int32_t attempts_fp32;   // attempts when the FP32 loop handed over (mixed mode)
-    int32_t pad1;
+    int32_t pw_k1;           // the k1 that pw3 / pw6 belong to (0: none)
 };
 
 template <typename V> struct MatView {
@@ -168,6 +172,33 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, in
         for (int k = 0; k < K; ++k) out[q] += v[k][q];
     }
     block_sum<N>(out);
+}
+
+// The same sum by wavefront 0 alone (no block barrier): lane l adds blocks
+// l, l + 64, l + 128, ... in that order (independent loads), then a fixed
+// shuffle tree.  Valid in lane 0; called by the 64 lanes of wavefront 0.
+template <int N>
+__device__ __forceinline__ void wave_sum_partials(const double* __restrict__ part, int nb,
+                                                  double (&out)[N]) {
+    constexpr int G = 8;   // loads in flight per lane and quantity
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < N; ++q) out[q] = 0.0;
+    for (int k0 = 0; k0 * 64 < nb; k0 += G) {
+        double v[G][N];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int b = lane + (k0 + k) * 64;
+#pragma unroll
+            for (int q = 0; q < N; ++q) v[k][q] = b < nb ? part[(size_t)b * N + q] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+#pragma unroll
+            for (int q = 0; q < N; ++q) out[q] += v[k][q];
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) out[q] = wave_sum(out[q]);
 }
 
 // Sum over a group of L lanes (L <= 64: xor shuffles inside the wavefront;
@@ -464,8 +495,11 @@ __device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2
     const double num = omega * dx2 + dy2 / omega;
     const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : INFINITY;
     const double k1 = st->accepted + 2.0;
-    const double grow = (1.0 + pow(k1, -0.6)) * eta;
-    const double eta_next = isfinite(eta_bar) ? smin((1.0 - pow(k1, -0.3)) * eta_bar, grow) : grow;
+    const bool pre = st->pw_k1 == st->accepted + 2;
+    const double p6 = pre ? st->pw6 : pow(k1, -0.6);
+    const double p3 = pre ? st->pw3 : pow(k1, -0.3);
+    const double grow = (1.0 + p6) * eta;
+    const double eta_next = isfinite(eta_bar) ? smin((1.0 - p3) * eta_bar, grow) : grow;
     if (eta <= eta_bar) {
         st->cur ^= 1;
         st->pending = 1;
@@ -485,33 +519,41 @@ __device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2
 
 // Loads the state S[p] into LDS, applies the pending step decision (if the
 // previous launch ran an attempt), and publishes the result to S[p ^ 1]
-// (the publisher block only).  Returns the block's copy.  The dual partials
-// and the primal sum are loaded together with the state -- one memory
-// latency on the kernel's critical path instead of two -- and used only when
-// a decision is pending.
+// (the publisher block only).  Returns the block's copy.  Wavefront 0 alone
+// loads the state, the primal sum and the dual partials (issued together:
+// one memory latency), reduces the partials without a block barrier, and
+// decides; one barrier then hands the state to the other wavefronts.
 __device__ __forceinline__ void load_and_decide(DevState* __restrict__ S, int p,
                                                 const double* __restrict__ part_m, int nb_m,
                                                 const double* __restrict__ dx2_slot,
                                                 DevState& Q, bool publisher) {
-    double dx2 = 0.0;
-    if (threadIdx.x == 0) {
-        Q = S[p];
-        dx2 = *dx2_slot;
-    }
-    double sm[2];
-    sum_partials<2>(part_m, nb_m, sm);   // its barriers also make Q visible
-    if (Q.decide_pending) {
+    if (threadIdx.x < 64) {
+        double dx2 = 0.0;
         if (threadIdx.x == 0) {
-            decide_step(&Q, dx2, sm[0], sm[1]);
-            Q.decide_pending = 0;
+            Q = S[p];
+            dx2 = *dx2_slot;
         }
-        __syncthreads();
+        double sm[2];
+        wave_sum_partials<2>(part_m, nb_m, sm);
+        if (threadIdx.x == 0) {
+            if (Q.decide_pending) {
+                decide_step(&Q, dx2, sm[0], sm[1]);
+                Q.decide_pending = 0;
+            }
+            if (publisher) S[p ^ 1] = Q;
+        }
     }
-    if (publisher && threadIdx.x == 0) {
-        DevState out = Q;
-        out.decide_pending = 0;
-        S[p ^ 1] = out;
-    }
+    __syncthreads();
+}
+
+// (accepted + 2)^-0.3 and ^-0.6 of the state a launch published to S[p ^ 1],
+// stored there for the next decision (one thread of one block).
+__device__ __forceinline__ void publish_step_powers(DevState* __restrict__ S, int p, const DevState& Q) {
+    const int k1 = Q.accepted + 2;
+    DevState* out = S + (p ^ 1);
+    out->pw6 = pow((double)k1, -0.6);
+    out->pw3 = pow((double)k1, -0.3);
+    out->pw_k1 = k1;
 }
 
 // KKT check, step 3, for one LP from its summed partials (thread 0):

@@ -891,11 +891,12 @@ template <typename T>
 int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx,
                            DevState& hs, bool& stopped) {
     hipStream_t s = stream_.s;
-    if (!graph_[prec]) {
-        int G = o_.graph_attempts > 0 ? o_.graph_attempts : 16;
-        G += G & 1;   // the state double buffer needs an even number of attempts per replay
-        graph_[prec] = capture_graph(I, G);
-    }
+    // graph_attempts < 0: the same kernel sequence launched on the stream
+    // directly (|graph_attempts| attempts + the check group per poll unit).
+    const bool direct = o_.graph_attempts < 0;
+    int G = o_.graph_attempts != 0 ? std::abs(o_.graph_attempts) : 16;
+    G += G & 1;   // the state double buffer needs an even number of attempts per replay
+    if (!direct && !graph_[prec]) graph_[prec] = capture_graph(I, G);
     hipGraphExec_t exec = graph_[prec];
     constexpr int R = 4;
     hipEvent_t ev[R] = {};
@@ -929,7 +930,12 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
                 }
             }
             const int slot = replays % R;
-            SUPR_HIP(hipGraphLaunch(exec, s));
+            if (direct) {
+                enqueue_attempts(I, G);
+                enqueue_check(I);
+            } else {
+                SUPR_HIP(hipGraphLaunch(exec, s));
+            }
             SUPR_HIP(hipEventRecord(ev[slot], s));
             ++replays;
         }

@@ -124,9 +124,12 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
                                                    const double* __restrict__ dx2_slot,
                                                    double* __restrict__ part_n) {
     __shared__ DevState Q;
-    load_and_decide(S, p, part_m, nb_m, dx2_slot, Q, blockIdx.x == 0);
+    // the publisher is the last block: its slot has the fewest rows, so the
+    // step powers it forms at the end stay off the launch's critical path
+    const bool publisher = blockIdx.x == gridDim.x - 1;
+    load_and_decide(S, p, part_m, nb_m, dx2_slot, Q, publisher);
     if (!Q.active) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) S[p ^ 1].decide_pending = 1;   // k_dual will run
+    if (publisher && threadIdx.x == 0) S[p ^ 1].decide_pending = 1;   // k_dual will run
     const int cur = Q.cur;
     const bool need_aty = Q.need_aty != 0;
     const bool pending = Q.pending != 0;
@@ -173,12 +176,13 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) part_n[blockIdx.x] = acc[0];
+    if (publisher && threadIdx.x == 0) publish_step_powers(S, p, Q);
 }
 
 // Dual half of one attempt, over the rows of A:
 //   axn = A xn, ad = A (xn - x);  [pending] ysum += eta_acc y;
 //   yn = proj(y + sigma (b - (axn + ad)));  partials of (yn-y)^2 and
-//   (yn-y)'ad.  Block 0 also sums k_primal's partials into *dx2_slot.
+//   (yn-y)'ad.  The last block also sums k_primal's partials into *dx2_slot.
 template <int L, typename T>
 __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S, int p1,
                                                  MatView<T> A, IterBufs<T> B,
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
         part_m[(size_t)blockIdx.x * 2] = acc[0];
         part_m[(size_t)blockIdx.x * 2 + 1] = acc[1];
     }
-    if (blockIdx.x == 0) {
+    if (blockIdx.x == gridDim.x - 1) {   // the last slot has the fewest rows
         double sn[1];
         sum_partials<1>(part_n, nb_n, sn);
         if (threadIdx.x == 0) *dx2_slot = sn[0];
