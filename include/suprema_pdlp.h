@@ -41,6 +41,9 @@
  *                         benchmarks reporting achieved HBM bandwidth)
  *   supr_bench_copy       no reference counterpart: a plain device copy (callers:
  *                         benchmarks calibrating HBM counters and bandwidth)
+ *   supr_mps_*            read_mps() + to_ranged_lp()  src/mps_reader.cpp:54-308,
+ *                         include/ranged_lp.hpp:27-62 (supr_mps_write: the MPS
+ *                         writing of tools/gen_lp.cpp:89-131, for any LP)
  *   supr_last_error       the what() text of the reference's std::runtime_error
  *                         (SOVEREIGN_CUDA_CHECK, gpu/pdlp_gpu_solve.cu:32-38)
  *
@@ -109,7 +112,8 @@ typedef struct supr_pdlp_options {
     double tol;                /* converged iff max(eps_P, eps_D, eps_G) <= tol */
     double switch_tol;         /* MIXED: FP32 -> FP64 when merit <= switch_tol (0: auto) */
     int32_t stall_checks;      /* MIXED: ... or after this many checks without a new best (0: auto) */
-    int32_t graph_attempts;    /* device-resident modes: attempts per graph replay (0: auto) */
+    int32_t graph_attempts;    /* device-resident modes: attempts per graph replay (0: auto = 16); < 0: the same
+                                  kernel sequence launched on the stream directly, -graph_attempts attempts per poll */
     int32_t spmv_lanes_A;      /* lanes per row for A (0: autotune; 1..64 powers of two, or 256 = one block per row) */
     int32_t spmv_lanes_AT;     /* lanes per row for A^T (0: autotune) */
     int32_t device;            /* HIP device ordinal */
@@ -360,6 +364,44 @@ int supr_gen_blend(int32_t K, int32_t P, uint64_t seed, int32_t* row_idx, int32_
                    double* val, double* c, double* rL, double* rU, double* l, double* u);
 int supr_gen_blend_sizes(int32_t count, uint64_t seed, int32_t K_lo, int32_t K_hi, int32_t P_lo,
                          int32_t P_hi, int32_t* K, int32_t* P);
+
+/* ------------------------------------------------------------ MPS ----- */
+/* read_mps() + to_ranged_lp() (src/mps_reader.cpp:54-308, include/ranged_lp.hpp:
+ * 27-62): the MPS file at path (free format, then fixed format if the free
+ * parse fails) as the reference's RangedLP.  Returns a handle that holds the
+ * LP; on a parse error, SUPR_EINVAL with "MPS parse failed: ..." (the text of
+ * the reference's std::runtime_error). */
+typedef struct supr_mps supr_mps;
+int supr_mps_read(const char* path, supr_mps** out);
+/* m, n, nnz of A; q_nnz of the QP's lower triangle (0 for an LP); maximize
+ * = 1 when OBJSENSE is MAX (c and Q are then negated, as to_ranged_lp does).
+ * Any pointer may be NULL. */
+int supr_mps_sizes(const supr_mps* h, int32_t* m, int32_t* n, int32_t* nnz, int32_t* q_nnz,
+                   int32_t* maximize);
+/* Copies the LP into caller arrays sized by supr_mps_sizes (any may be NULL):
+ * A in COO order (row_idx, col_idx, val: nnz), c, l, u (n), rL, rU (m),
+ * obj_offset, integer flags (n bytes), Q (q_nnz). */
+int supr_mps_get(const supr_mps* h, int32_t* row_idx, int32_t* col_idx, double* val, double* c,
+                 double* rL, double* rU, double* l, double* u, double* obj_offset, char* integer,
+                 int32_t* q_row, int32_t* q_col, double* q_val);
+/* Names held by the handle: kind 0 the problem (NAME), 1 column index, 2 row
+ * index, 3 the objective row; NULL when out of range. */
+const char* supr_mps_name(const supr_mps* h, int32_t kind, int32_t index);
+void supr_mps_free(supr_mps* h);
+/* Writes lp as a free-format MPS file that supr_mps_read (and the
+ * reference's read_mps) reads back to the same LP: row kinds E / L / G from
+ * rL, rU (a ranged row as G or L with RANGES, whichever reproduces both
+ * bounds; a free row as L with an infinite right-hand side), every column
+ * with its objective coefficient, bounds, the objective offset as the
+ * objective row's RHS, integer columns between INTORG / INTEND markers
+ * (integer may be NULL).  Names: col_names / row_names (no white space), or
+ * C<j> / R<i> when NULL.  Values print with 17 significant digits, so they
+ * read back exactly; explicit zero coefficients are dropped, as the reader
+ * drops them.  (No reference counterpart: tools/gen_lp.cpp:89-131 writes
+ * the transportation family this way.) */
+int supr_mps_write(const char* path, const supr_lp* lp, const char* name,
+                   const char* const* col_names, const char* const* row_names,
+                   const char* integer);
 
 #ifdef __cplusplus
 }

@@ -145,9 +145,10 @@ class Checker:
         return ptr, idx, val
 
 
-def read_mps_ref(path):
+def read_mps_ref(path, full=False):
     """read_mps + to_ranged_lp of the reference (only where _ref is built).
-    Returns a dict of arrays."""
+    Returns a dict of arrays; full=True adds integer flags, the QP triangle
+    and the names."""
     lib = ctypes.CDLL(PATHS["ref"])
     dims = np.zeros(3, dtype=np.int32)
     lib.ref_mps_read.restype = ctypes.c_int
@@ -160,5 +161,24 @@ def read_mps_ref(path):
     off = ctypes.c_double(0.0)
     sense = lib.ref_mps_arrays(_i(ri), _i(ci), _d(v), _d(c), _d(rL), _d(rU), _d(l), _d(u),
                                ctypes.byref(off))
-    return dict(m=m, n=n, row=ri, col=ci, val=v, c=c, rL=rL, rU=rU, l=l, u=u,
-                obj_offset=off.value, maximize=bool(sense))
+    out = dict(m=m, n=n, row=ri, col=ci, val=v, c=c, rL=rL, rU=rU, l=l, u=u,
+               obj_offset=off.value, maximize=bool(sense))
+    if full:
+        lib.ref_mps_q_nnz.restype = ctypes.c_int
+        qn = int(lib.ref_mps_q_nnz())
+        integ = ctypes.create_string_buffer(max(1, n))
+        qr, qc, qv = np.zeros(qn, dtype=np.int32), np.zeros(qn, dtype=np.int32), np.zeros(qn)
+        lib.ref_mps_extra(integ, _i(qr), _i(qc), _d(qv))
+        lib.ref_mps_name.restype = ctypes.c_char_p
+        out.update(integer=np.frombuffer(integ.raw[:n], dtype=np.int8).copy(), q_row=qr, q_col=qc,
+                   q_val=qv, name=lib.ref_mps_name(0, 0).decode(),
+                   col_names=[lib.ref_mps_name(1, j).decode() for j in range(n)],
+                   row_names=[lib.ref_mps_name(2, i).decode() for i in range(m)])
+    return out
+
+
+def read_mps_ref_error(path):
+    """The reference read_mps's exception text on path ('' when it parses)."""
+    lib = ctypes.CDLL(PATHS["ref"])
+    lib.ref_mps_error.restype = ctypes.c_char_p
+    return lib.ref_mps_error(path.encode()).decode()

@@ -9,7 +9,8 @@
 //   ref_verify       -> verify()             src/verify.cpp:18-103
 //   ref_pdlp_form_*  -> build_pdlp_form()    src/pdlp.cpp:8-82
 //   ref_pdlp         -> solve_pdhg()         src/pdlp.cpp:146-152 (pdlp_run)
-//   ref_mps_*        -> read_mps() + to_ranged_lp()
+//   ref_mps_*        -> read_mps() + to_ranged_lp() (arrays, integer flags,
+//                       QP triangle, names, error text)
 //                       src/mps_reader.cpp:54-308, include/ranged_lp.hpp:39-62
 // It exists to pin the oracle and to time the reference CPU engine; the
 // product never loads it.
@@ -173,6 +174,42 @@ int ref_mps_arrays(int* ri, int* ci, double* v, double* c, double* rL, double* r
     std::memcpy(u, r.u.data(), sizeof(double) * n);
     *obj_offset = r.obj_offset;
     return r.original_sense == "max" ? 1 : 0;
+}
+
+// The rest of the last MPS read: integer flags (n bytes), the QP lower
+// triangle (ref_mps_q_nnz entries), and names (kind 0 problem, 1 column,
+// 2 row; NULL when out of range).
+int ref_mps_q_nnz() { return g_mps.Q.nnz(); }
+
+int ref_mps_extra(char* integer, int* q_row, int* q_col, double* q_val) {
+    const RangedLP& r = g_mps;
+    for (int j = 0; j < r.n(); ++j) integer[j] = (j < (int)r.integer.size()) ? r.integer[j] : 0;
+    const int qn = r.Q.nnz();
+    std::memcpy(q_row, r.Q.row_idx.data(), sizeof(int) * qn);
+    std::memcpy(q_col, r.Q.col_idx.data(), sizeof(int) * qn);
+    std::memcpy(q_val, r.Q.val.data(), sizeof(double) * qn);
+    return 0;
+}
+
+const char* ref_mps_name(int kind, int index) {
+    const RangedLP& r = g_mps;
+    if (kind == 0) return r.name.c_str();
+    if (kind == 1) return index >= 0 && index < r.n() ? r.col_names[index].c_str() : nullptr;
+    if (kind == 2) return index >= 0 && index < r.m() ? r.row_names[index].c_str() : nullptr;
+    return nullptr;
+}
+
+// What the reference's read_mps throws on a file (its what() text), or an
+// empty string when it parses.
+const char* ref_mps_error(const char* path) {
+    static thread_local std::string msg;
+    try {
+        (void)read_mps(path);
+        msg.clear();
+    } catch (const std::exception& e) {
+        msg = e.what();
+    }
+    return msg.c_str();
 }
 
 }  // extern "C"

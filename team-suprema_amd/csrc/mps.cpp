@@ -1,0 +1,709 @@
+// mps.cpp -- MPS ingestion: the reference's read_mps (src/mps_reader.cpp:
+// 54-308) followed by to_ranged_lp (include/ranged_lp.hpp:27-62), and an MPS
+// writer for any supr_lp.
+//
+// The reader gives the reference's RangedLP for every file the reference
+// accepts, and fails where it fails:
+//  * free format first; if that parse fails anywhere, the file is parsed
+//    again as fixed format (fields cut at columns 2-3, 5-12, 15-22, 25-36,
+//    40-47, 50-61); if that fails too, the error is the free parse's;
+//  * sections NAME, OBJSENSE, ROWS, COLUMNS (INTORG / INTEND markers), RHS
+//    (an RHS on the objective row is minus the objective constant), RANGES,
+//    BOUNDS (UP LO FX FR MI PL BV LI UI, optional bound-set name),
+//    QUADOBJ / QMATRIX / QSECTION (lower triangle kept), ENDATA;
+//  * the first N row is the objective, later N rows are free and dropped;
+//  * numbers convert as std::stod does (strtod, trailing text ignored, a
+//    range error for results that overflow or underflow);
+//  * explicit zero coefficients and entries of undeclared rows are dropped;
+//    columns are numbered by first appearance; a row or name declared twice
+//    keeps the reference's map semantics (the last declaration wins).
+// It is one pass over the file held in memory with names as views into it
+// and one hash lookup per name, instead of a stream and a string map per
+// token.  Numbers with at most 19 significant digits and a decimal exponent
+// within +-22 convert by one exact integer-to-double conversion and one
+// correctly rounded multiply or divide by an exact power of ten, which is the
+// correctly rounded value strtod also returns; anything else goes to strtod.
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <unordered_map>
+#include <vector>
+
+#include "host_lp.hpp"
+
+namespace {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+bool is_ws(char ch) {
+    return ch == ' ' || ch == '\t' || ch == '\n' || ch == '\v' || ch == '\f' || ch == '\r';
+}
+
+// std::stod: leading white space skipped, the longest valid prefix
+// converted, std::invalid_argument if none, std::out_of_range on ERANGE.
+double parse_double_slow(std::string_view t) {
+    char small[64];
+    std::string big;
+    const char* s;
+    if (t.size() < sizeof(small)) {
+        std::memcpy(small, t.data(), t.size());
+        small[t.size()] = '\0';
+        s = small;
+    } else {
+        big.assign(t.data(), t.size());
+        s = big.c_str();
+    }
+    errno = 0;
+    char* end = nullptr;
+    const double v = std::strtod(s, &end);
+    if (end == s) throw std::invalid_argument("stod");
+    if (errno == ERANGE) throw std::out_of_range("stod");
+    return v;
+}
+
+double parse_double(std::string_view t) {
+    static const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                      1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                      1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+    const char* p = t.data();
+    const char* e = p + t.size();
+    bool neg = false;
+    if (p < e && *p == '-') { neg = true; ++p; }
+    uint64_t mant = 0;
+    int digits = 0, frac = 0;
+    bool any = false;
+    while (p < e && *p >= '0' && *p <= '9') {
+        if (mant != 0 || *p != '0') ++digits;
+        mant = mant * 10 + (uint64_t)(*p - '0');
+        ++p; any = true;
+        if (digits > 19) return parse_double_slow(t);
+    }
+    if (p < e && *p == '.') {
+        ++p;
+        while (p < e && *p >= '0' && *p <= '9') {
+            if (mant != 0 || *p != '0') ++digits;
+            mant = mant * 10 + (uint64_t)(*p - '0');
+            ++frac; ++p; any = true;
+            if (digits > 19) return parse_double_slow(t);
+        }
+    }
+    if (!any) return parse_double_slow(t);
+    int ex = 0;
+    if (p < e && (*p == 'e' || *p == 'E')) {
+        const char* q = p + 1;
+        bool eneg = false;
+        if (q < e && (*q == '+' || *q == '-')) { eneg = *q == '-'; ++q; }
+        if (!(q < e && *q >= '0' && *q <= '9')) return parse_double_slow(t);
+        while (q < e && *q >= '0' && *q <= '9') {
+            ex = ex * 10 + (*q - '0');
+            ++q;
+            if (ex > 10000) return parse_double_slow(t);
+        }
+        if (eneg) ex = -ex;
+        p = q;
+    }
+    if (p != e) return parse_double_slow(t);   // trailing text: strtod decides
+    const int e10 = ex - frac;
+    if (mant > (uint64_t(1) << 53) || e10 < -22 || e10 > 22) return parse_double_slow(t);
+    double v = (double)mant;
+    if (e10 < 0) v /= kPow10[-e10];
+    else if (e10 > 0) v *= kPow10[e10];
+    return neg ? -v : v;
+}
+
+// strtod consumed the whole token (the reference's is_number).
+bool is_number(std::string_view t) {
+    if (t.empty()) return false;
+    const std::string s(t);
+    char* end = nullptr;
+    std::strtod(s.c_str(), &end);
+    return end && *end == '\0';
+}
+
+std::string_view trim(std::string_view s) {
+    size_t a = 0, b = s.size();
+    while (a < b && (s[a] == ' ' || s[a] == '\t')) ++a;
+    while (b > a && (s[b - 1] == ' ' || s[b - 1] == '\t')) --b;
+    return s.substr(a, b - a);
+}
+
+bool unquoted_eq(std::string_view t, const char* word) {
+    size_t k = 0;
+    const size_t wl = std::strlen(word);
+    for (char ch : t) {
+        if (ch == '\'') continue;
+        if (k >= wl || ch != word[k]) return false;
+        ++k;
+    }
+    return k == wl;
+}
+
+// Names interned to dense ids (views into the file buffer or static text).
+struct Names {
+    std::unordered_map<std::string_view, int32_t> id;
+    std::vector<std::string_view> name;
+    int32_t get(std::string_view s) {
+        auto it = id.find(s);
+        if (it != id.end()) return it->second;
+        const int32_t k = (int32_t)name.size();
+        id.emplace(s, k);
+        name.push_back(s);
+        return k;
+    }
+    int32_t find(std::string_view s) const {
+        auto it = id.find(s);
+        return it == id.end() ? -1 : it->second;
+    }
+};
+
+struct Parsed {
+    std::string name;
+    bool maximize = false;
+    std::vector<std::string> col_names, row_names;
+    std::string obj_name;
+    std::vector<int32_t> row, col;
+    std::vector<double> val, c, rL, rU, l, u;
+    double obj_offset = 0.0;
+    std::vector<char> integer;
+    std::vector<int32_t> q_row, q_col;
+    std::vector<double> q_val;
+};
+
+enum Section { S_NONE, S_OBJSENSE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES, S_BOUNDS, S_QUAD, S_QMATRIX };
+
+[[noreturn]] void fail(const std::string& msg) { throw std::runtime_error(msg); }
+
+// One parse of the whole text (src/mps_reader.cpp:65-307 + to_ranged_lp).
+void parse(const std::string& text, bool fixed_format, Parsed& out) {
+    std::string sense = "min";
+    std::string_view pname;
+    bool have_name = false;
+    Section section = S_NONE;
+
+    Names rows;                       // every row name seen (ROWS, COLUMNS, RHS, RANGES)
+    std::vector<char> rtype;          // per row-name id: type char (0 = undeclared)
+    std::vector<char> rdecl;          // per row-name id: declared in ROWS
+    std::vector<char> rfree;          // per row-name id: a free (second or later N) row
+    std::vector<int32_t> row_order;   // ROWS order of non-N rows (row-name ids)
+    int32_t obj_id = -1;
+    std::string_view obj_name;
+
+    Names cols;
+    std::vector<double> obj;          // per column: objective coefficient
+    std::vector<char> integ, has_lo, has_hi, explicit_lo;
+    std::vector<double> blo, bhi;
+    bool int_marker = false;
+    auto add_col = [&](std::string_view nm) {
+        const int32_t before = (int32_t)cols.name.size();
+        const int32_t j = cols.get(nm);
+        if (j == before) {
+            obj.push_back(0.0); integ.push_back(0); has_lo.push_back(0); has_hi.push_back(0);
+            explicit_lo.push_back(0); blo.push_back(0.0); bhi.push_back(0.0);
+        }
+        return j;
+    };
+    auto row_id = [&](std::string_view nm) {
+        const int32_t before = (int32_t)rows.name.size();
+        const int32_t k = rows.get(nm);
+        if (k == before) { rtype.push_back(0); rdecl.push_back(0); rfree.push_back(0); }
+        return k;
+    };
+
+    struct Entry { int32_t row, col; double val; };
+    std::vector<Entry> entries;
+    std::vector<double> rhs_v, rng_v;   // per row-name id
+    std::vector<char> rhs_has, rng_has;
+    double obj_constant = 0.0;
+    struct QEntry { std::string_view a, b; double v; bool full; };
+    std::vector<QEntry> quad;
+
+    static const char* const kSections[] = {"NAME", "OBJSENSE", "ROWS", "COLUMNS", "RHS", "RANGES",
+                                            "BOUNDS", "ENDATA", "QUADOBJ", "QMATRIX", "QSECTION"};
+    std::vector<std::string_view> toks, tk;
+    toks.reserve(16);
+    tk.reserve(16);
+    const char* p = text.data();
+    const char* const end = p + text.size();
+    while (p < end) {
+        const char* nl = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+        const char* le = nl ? nl : end;
+        std::string_view line(p, (size_t)(le - p));
+        p = nl ? nl + 1 : end;
+        if (!line.empty() && line.back() == '\r') line.remove_suffix(1);
+        size_t first = 0;
+        while (first < line.size() && (line[first] == ' ' || line[first] == '\t')) ++first;
+        if (first == line.size()) continue;   // blank
+        if (line[first] == '*') continue;     // comment
+        toks.clear();
+        for (size_t i = 0; i < line.size();) {
+            while (i < line.size() && is_ws(line[i])) ++i;
+            if (i == line.size()) break;
+            size_t j = i;
+            while (j < line.size() && !is_ws(line[j])) ++j;
+            toks.push_back(line.substr(i, j - i));
+            i = j;
+        }
+        const std::string_view first_tok = toks.empty() ? std::string_view() : toks[0];
+        const bool col1 = line[0] != ' ' && line[0] != '\t';
+        int kw = -1;
+        for (int k = 0; k < 11; ++k)
+            if (first_tok == kSections[k]) { kw = k; break; }
+        if (kw >= 0 && (col1 || kw == 7)) {
+            if (kw == 0) {   // NAME
+                have_name = true;
+                pname = toks.size() > 1 ? toks[1] : std::string_view("UNNAMED");
+                section = S_NONE;
+                continue;
+            }
+            if (kw == 7) break;   // ENDATA
+            static const Section map[] = {S_NONE, S_OBJSENSE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES,
+                                          S_BOUNDS, S_NONE, S_QUAD, S_QMATRIX, S_QUAD};
+            section = map[kw];
+            continue;
+        }
+        const std::vector<std::string_view>* tp = &toks;
+        if (fixed_format && section != S_OBJSENSE) {
+            static const int st[6] = {1, 4, 14, 24, 39, 49};
+            static const int ln[6] = {2, 8, 8, 12, 8, 12};
+            std::string_view f[6];
+            for (int k = 0; k < 6; ++k) {
+                if ((int)line.size() <= st[k]) break;
+                f[k] = trim(line.substr((size_t)st[k], (size_t)ln[k]));
+            }
+            tk.clear();
+            if (section == S_ROWS) {
+                tk = {f[0], f[1]};
+            } else if (section == S_BOUNDS) {
+                tk = {f[0], f[1].empty() ? std::string_view("BND") : f[1], f[2]};
+                if (!f[3].empty()) tk.push_back(f[3]);
+            } else {
+                std::string_view lead = f[1];
+                if (lead.empty() && section != S_COLUMNS) lead = "RHS";
+                tk = {lead, f[2], f[3]};
+                if (!f[4].empty()) { tk.push_back(f[4]); tk.push_back(f[5]); }
+            }
+            tp = &tk;
+        }
+        const std::vector<std::string_view>& t = *tp;
+        switch (section) {
+        case S_OBJSENSE: {
+            if (t.empty()) fail("empty OBJSENSE line");
+            std::string s(t[0]);
+            for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
+            sense = s;
+            break;
+        }
+        case S_ROWS: {
+            if (!fixed_format && t.size() != 2) fail("ROWS line is not free format: " + std::string(line));
+            const char ty = t[0].empty() ? '\0' : (char)std::toupper((unsigned char)t[0][0]);
+            const int32_t k = row_id(t[1]);
+            rtype[k] = ty;
+            rdecl[k] = 1;
+            if (ty == 'N') {
+                if (obj_id < 0) { obj_id = k; obj_name = t[1]; }
+                else rfree[k] = 1;
+            } else {
+                row_order.push_back(k);
+            }
+            break;
+        }
+        case S_COLUMNS: {
+            bool marker = false;
+            for (auto& x : toks) if (unquoted_eq(x, "MARKER")) marker = true;
+            if (marker) {
+                for (auto& x : toks) {
+                    if (unquoted_eq(x, "INTORG")) int_marker = true;
+                    if (unquoted_eq(x, "INTEND")) int_marker = false;
+                }
+                break;
+            }
+            if (t.empty()) fail("empty COLUMNS line");
+            const int32_t j = add_col(t[0]);
+            if (int_marker) integ[j] = 1;
+            for (size_t i = 1; i + 1 < t.size(); i += 2) {
+                const double v = parse_double(t[i + 1]);
+                if (obj_id >= 0 && t[i] == obj_name) { obj[j] += v; continue; }
+                const int32_t k = row_id(t[i]);
+                if (rfree[k]) continue;
+                entries.push_back({k, j, v});
+            }
+            break;
+        }
+        case S_RHS:
+        case S_RANGES: {
+            const size_t s0 = (t.size() % 2 == 1) ? 1 : 0;
+            for (size_t i = s0; i + 1 < t.size(); i += 2) {
+                const double v = parse_double(t[i + 1]);
+                if (section == S_RHS && t[i] == obj_name && obj_id >= 0) { obj_constant = -v; continue; }
+                const int32_t k = row_id(t[i]);
+                if (section == S_RHS) {
+                    if ((size_t)k >= rhs_v.size()) { rhs_v.resize(k + 1, 0.0); rhs_has.resize(k + 1, 0); }
+                    rhs_v[k] = v; rhs_has[k] = 1;
+                } else {
+                    if ((size_t)k >= rng_v.size()) { rng_v.resize(k + 1, 0.0); rng_has.resize(k + 1, 0); }
+                    rng_v[k] = v; rng_has[k] = 1;
+                }
+            }
+            break;
+        }
+        case S_QUAD:
+        case S_QMATRIX:
+            if (t.size() < 3) break;
+            quad.push_back({t[0], t[1], parse_double(t[2]), section == S_QMATRIX});
+            break;
+        case S_BOUNDS: {
+            if (t.empty()) fail("empty BOUNDS line");
+            std::string bt(t[0]);
+            for (auto& ch : bt) ch = (char)std::toupper((unsigned char)ch);
+            const bool needs_val = bt == "UP" || bt == "LO" || bt == "FX" || bt == "LI" || bt == "UI";
+            bool has_set;
+            if (needs_val) has_set = t.size() >= 4;
+            else if (bt == "BV")
+                has_set = t.size() >= 4 || (t.size() == 3 && !(cols.find(t[1]) >= 0 && is_number(t[2])));
+            else has_set = t.size() >= 3;
+            const size_t ci = has_set ? 2 : 1;
+            if (t.size() <= ci) fail("malformed BOUNDS line: " + std::string(line));
+            const int32_t j = add_col(t[ci]);
+            const double v = t.size() > ci + 1 ? parse_double(t[ci + 1]) : 0.0;
+            if (bt == "UP") { bhi[j] = v; has_hi[j] = 1; }
+            else if (bt == "LI") { blo[j] = v; has_lo[j] = 1; explicit_lo[j] = 1; integ[j] = 1; }
+            else if (bt == "UI") { bhi[j] = v; has_hi[j] = 1; integ[j] = 1; }
+            else if (bt == "LO") { blo[j] = v; has_lo[j] = 1; explicit_lo[j] = 1; }
+            else if (bt == "FX") { blo[j] = v; bhi[j] = v; has_lo[j] = has_hi[j] = 1; explicit_lo[j] = 1; }
+            else if (bt == "FR") { blo[j] = -kInf; bhi[j] = kInf; has_lo[j] = has_hi[j] = 1; explicit_lo[j] = 1; }
+            else if (bt == "MI") { blo[j] = -kInf; has_lo[j] = 1; explicit_lo[j] = 1; }
+            else if (bt == "PL") { bhi[j] = kInf; has_hi[j] = 1; }
+            else if (bt == "BV") { blo[j] = 0.0; bhi[j] = 1.0; has_lo[j] = has_hi[j] = 1; explicit_lo[j] = 1; integ[j] = 1; }
+            else fail("Unsupported BOUNDS type '" + bt + "' in MPS file");
+            break;
+        }
+        case S_NONE:
+            break;
+        }
+    }
+    if (obj_id < 0) fail("No objective (N) row found in MPS file");
+
+    // rows: ROWS order; a name listed twice maps to its last position
+    const int32_t m = (int32_t)row_order.size(), n = (int32_t)cols.name.size();
+    std::vector<int32_t> pos(rows.name.size(), -1);
+    for (int32_t i = 0; i < m; ++i) pos[row_order[i]] = i;
+    out.row.clear(); out.col.clear(); out.val.clear();
+    out.row.reserve(entries.size()); out.col.reserve(entries.size()); out.val.reserve(entries.size());
+    for (const Entry& en : entries) {
+        if (en.val == 0.0) continue;
+        const int32_t i = pos[en.row];
+        if (i < 0) continue;   // a row never declared in ROWS
+        out.row.push_back(i);
+        out.col.push_back(en.col);
+        out.val.push_back(en.val);
+    }
+    // to_ranged_lp (include/ranged_lp.hpp:27-62)
+    out.maximize = sense == "max";
+    out.c.assign(n, 0.0);
+    for (int32_t j = 0; j < n; ++j) out.c[j] = out.maximize ? -obj[j] : obj[j];
+    out.l.assign(n, 0.0);
+    out.u.assign(n, kInf);
+    for (int32_t j = 0; j < n; ++j) {
+        if (has_lo[j]) out.l[j] = blo[j];
+        if (has_hi[j]) out.u[j] = bhi[j];
+        if (has_hi[j] && bhi[j] < 0 && !explicit_lo[j]) out.l[j] = -kInf;
+    }
+    out.rL.assign(m, 0.0);
+    out.rU.assign(m, 0.0);
+    for (int32_t i = 0; i < m; ++i) {
+        const int32_t k = row_order[i];
+        const char ty = rtype[k];
+        // a right-hand side goes to its name's last position only; types and
+        // ranges apply at every position of a twice-declared name
+        const double rhs = pos[k] == i && (size_t)k < rhs_has.size() && rhs_has[k] ? rhs_v[k] : 0.0;
+        const bool has_range = (size_t)k < rng_has.size() && rng_has[k];
+        double& lo = out.rL[i];
+        double& hi = out.rU[i];
+        if (!has_range) {
+            if (ty == 'L') { lo = -kInf; hi = rhs; }
+            else if (ty == 'G') { lo = rhs; hi = kInf; }
+            else if (ty == 'E') { lo = rhs; hi = rhs; }
+        } else {
+            const double rng = rng_v[k], r = std::fabs(rng);
+            if (ty == 'L') { lo = rhs - r; hi = rhs; }
+            else if (ty == 'G') { lo = rhs; hi = rhs + r; }
+            else if (ty == 'E') {
+                if (rng >= 0) { lo = rhs; hi = rhs + r; }
+                else { lo = rhs - r; hi = rhs; }
+            }
+        }
+    }
+    out.obj_offset = out.maximize ? -obj_constant : obj_constant;
+    out.integer.assign(integ.begin(), integ.end());
+    out.q_row.clear(); out.q_col.clear(); out.q_val.clear();
+    for (const QEntry& q : quad) {
+        int32_t i = cols.find(q.a), j = cols.find(q.b);
+        if (i < 0 || j < 0) fail("QUADOBJ refers to unknown column");
+        if (q.full && i < j) continue;
+        if (i < j) std::swap(i, j);
+        out.q_row.push_back(i);
+        out.q_col.push_back(j);
+        out.q_val.push_back(out.maximize ? -q.v : q.v);
+    }
+    out.name = have_name && !pname.empty() ? std::string(pname) : std::string("UNNAMED");
+    out.obj_name = std::string(obj_name);
+    out.col_names.resize(n);
+    for (int32_t j = 0; j < n; ++j) out.col_names[j] = std::string(cols.name[j]);
+    out.row_names.resize(m);
+    for (int32_t i = 0; i < m; ++i) out.row_names[i] = std::string(rows.name[row_order[i]]);
+}
+
+std::string read_file(const char* path) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) fail(std::string("cannot open MPS file: ") + path);
+    std::string s;
+    char buf[1 << 16];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof(buf), f)) > 0) s.append(buf, k);
+    std::fclose(f);
+    return s;
+}
+
+}  // namespace
+
+struct supr_mps {
+    Parsed p;
+};
+
+extern "C" {
+
+int supr_mps_read(const char* path, supr_mps** out) {
+    if (!path || !out) {
+        supr::set_error("supr_mps_read: null argument");
+        return SUPR_EINVAL;
+    }
+    *out = nullptr;
+    try {
+        auto h = std::make_unique<supr_mps>();
+        std::string free_err;
+        try {
+            const std::string text = read_file(path);
+            try {
+                parse(text, false, h->p);
+            } catch (const std::exception& e) {
+                free_err = e.what();
+                parse(text, true, h->p);
+            }
+        } catch (const std::bad_alloc&) {
+            throw;
+        } catch (const std::exception& e) {
+            supr::set_error(std::string("MPS parse failed: ") + (free_err.empty() ? e.what() : free_err));
+            return SUPR_EINVAL;
+        }
+        *out = h.release();
+        return SUPR_OK;
+    } catch (const std::bad_alloc&) {
+        supr::set_error("out of host memory");
+        return SUPR_ENOMEM;
+    } catch (const std::exception& e) {
+        supr::set_error(e.what());
+        return SUPR_EINTERNAL;
+    }
+}
+
+int supr_mps_sizes(const supr_mps* h, int32_t* m, int32_t* n, int32_t* nnz, int32_t* q_nnz,
+                   int32_t* maximize) {
+    if (!h) {
+        supr::set_error("supr_mps_sizes: null handle");
+        return SUPR_EINVAL;
+    }
+    if (m) *m = (int32_t)h->p.row_names.size();
+    if (n) *n = (int32_t)h->p.col_names.size();
+    if (nnz) *nnz = (int32_t)h->p.val.size();
+    if (q_nnz) *q_nnz = (int32_t)h->p.q_val.size();
+    if (maximize) *maximize = h->p.maximize ? 1 : 0;
+    return SUPR_OK;
+}
+
+int supr_mps_get(const supr_mps* h, int32_t* row_idx, int32_t* col_idx, double* val, double* c,
+                 double* rL, double* rU, double* l, double* u, double* obj_offset, char* integer,
+                 int32_t* q_row, int32_t* q_col, double* q_val) {
+    if (!h) {
+        supr::set_error("supr_mps_get: null handle");
+        return SUPR_EINVAL;
+    }
+    const Parsed& p = h->p;
+    auto put = [](auto* dst, const auto& v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(row_idx, p.row); put(col_idx, p.col); put(val, p.val);
+    put(c, p.c); put(rL, p.rL); put(rU, p.rU); put(l, p.l); put(u, p.u);
+    put(integer, p.integer);
+    put(q_row, p.q_row); put(q_col, p.q_col); put(q_val, p.q_val);
+    if (obj_offset) *obj_offset = p.obj_offset;
+    return SUPR_OK;
+}
+
+const char* supr_mps_name(const supr_mps* h, int32_t kind, int32_t index) {
+    if (!h) return nullptr;
+    const Parsed& p = h->p;
+    switch (kind) {
+    case 0: return p.name.c_str();
+    case 1: return index >= 0 && (size_t)index < p.col_names.size() ? p.col_names[index].c_str() : nullptr;
+    case 2: return index >= 0 && (size_t)index < p.row_names.size() ? p.row_names[index].c_str() : nullptr;
+    case 3: return p.obj_name.c_str();
+    default: return nullptr;
+    }
+}
+
+void supr_mps_free(supr_mps* h) { delete h; }
+
+int supr_mps_write(const char* path, const supr_lp* lp, const char* name,
+                   const char* const* col_names, const char* const* row_names,
+                   const char* integer) {
+    if (!path || !supr::check_lp(lp, "supr_mps_write")) {
+        if (!path) supr::set_error("supr_mps_write: null path");
+        return SUPR_EINVAL;
+    }
+    try {
+        const int32_t m = lp->m, n = lp->n, nnz = lp->nnz;
+        auto bad_name = [](const char* nm) {
+            if (!nm || !*nm) return true;
+            for (const char* q = nm; *q; ++q) if (is_ws(*q)) return true;
+            return false;
+        };
+        std::vector<std::string> cn(n), rn(m);
+        for (int32_t j = 0; j < n; ++j) {
+            cn[j] = col_names ? std::string(col_names[j] ? col_names[j] : "") : "C" + std::to_string(j);
+            if (bad_name(cn[j].c_str())) throw std::invalid_argument("supr_mps_write: column name empty or with white space");
+        }
+        std::unordered_map<std::string, int> seen;
+        for (int32_t i = 0; i < m; ++i) {
+            rn[i] = row_names ? std::string(row_names[i] ? row_names[i] : "") : "R" + std::to_string(i);
+            if (bad_name(rn[i].c_str())) throw std::invalid_argument("supr_mps_write: row name empty or with white space");
+            seen[rn[i]] = 1;
+        }
+        std::string obj = "COST";
+        for (int k = 0; seen.count(obj); ++k) obj = "OBJ" + std::to_string(k);
+        for (int32_t i = 0; i < m; ++i)
+            if (!(lp->rL[i] <= lp->rU[i]) || lp->rL[i] == kInf || lp->rU[i] == -kInf)
+                throw std::invalid_argument("supr_mps_write: row " + std::to_string(i) + " has no feasible range");
+        // entries grouped by column, COO order within a column
+        std::vector<int32_t> ptr(n + 1, 0), ord(nnz);
+        for (int32_t k = 0; k < nnz; ++k) ++ptr[lp->col_idx[k] + 1];
+        for (int32_t j = 0; j < n; ++j) ptr[j + 1] += ptr[j];
+        {
+            std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
+            for (int32_t k = 0; k < nnz; ++k) ord[next[lp->col_idx[k]]++] = k;
+        }
+        std::FILE* f = std::fopen(path, "wb");
+        if (!f) throw std::runtime_error(std::string("cannot open ") + path + " for writing");
+        std::string buf;
+        buf.reserve(1 << 20);
+        char num[64];
+        auto g17 = [&](double v) {
+            std::snprintf(num, sizeof(num), "%.17g", v);
+            return num;
+        };
+        auto flush = [&](bool force) {
+            if (force || buf.size() > (1u << 20)) {
+                if (std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+                    std::fclose(f);
+                    throw std::runtime_error(std::string("write failed: ") + path);
+                }
+                buf.clear();
+            }
+        };
+        buf += "NAME "; buf += (name && *name && !bad_name(name)) ? name : "LP"; buf += "\n";
+        buf += "ROWS\n N  "; buf += obj; buf += "\n";
+        // row kinds: E (rL == rU), L (rL = -inf), G (rU = +inf or a range
+        // rL + (rU - rL) reproduces), L with a range (rU - (rU - rL)
+        // reproduces rL), free rows as L with an infinite right-hand side
+        std::vector<char> kind(m);
+        std::vector<double> rhs(m), rng(m);
+        std::vector<char> has_rng(m, 0);
+        for (int32_t i = 0; i < m; ++i) {
+            const double lo = lp->rL[i], hi = lp->rU[i];
+            if (lo == hi) { kind[i] = 'E'; rhs[i] = lo; }
+            else if (lo == -kInf) { kind[i] = 'L'; rhs[i] = hi; }
+            else if (hi == kInf) { kind[i] = 'G'; rhs[i] = lo; }
+            else {
+                const double r = hi - lo;
+                has_rng[i] = 1;
+                rng[i] = r;
+                if (lo + r == hi || !(hi - r == lo)) { kind[i] = 'G'; rhs[i] = lo; }
+                else { kind[i] = 'L'; rhs[i] = hi; }
+            }
+            buf += ' '; buf += kind[i]; buf += "  "; buf += rn[i]; buf += '\n';
+            flush(false);
+        }
+        buf += "COLUMNS\n";
+        bool in_int = false;
+        for (int32_t j = 0; j < n; ++j) {
+            const bool ij = integer && integer[j];
+            if (ij != in_int) {
+                buf += ij ? "    MARKER  'MARKER'  'INTORG'\n" : "    MARKER  'MARKER'  'INTEND'\n";
+                in_int = ij;
+            }
+            buf += "    "; buf += cn[j]; buf += "  "; buf += obj; buf += "  "; buf += g17(lp->c[j]); buf += '\n';
+            for (int32_t q = ptr[j]; q < ptr[j + 1]; ++q) {
+                const int32_t k = ord[q];
+                buf += "    "; buf += cn[j]; buf += "  "; buf += rn[lp->row_idx[k]]; buf += "  ";
+                buf += g17(lp->val[k]); buf += '\n';
+            }
+            flush(false);
+        }
+        if (in_int) buf += "    MARKER  'MARKER'  'INTEND'\n";
+        buf += "RHS\n";
+        if (lp->obj_offset != 0.0) { buf += "    RHS  "; buf += obj; buf += "  "; buf += g17(-lp->obj_offset); buf += '\n'; }
+        for (int32_t i = 0; i < m; ++i) {
+            if (rhs[i] == 0.0 && !std::signbit(rhs[i])) continue;
+            buf += "    RHS  "; buf += rn[i]; buf += "  "; buf += g17(rhs[i]); buf += '\n';
+            flush(false);
+        }
+        bool any_rng = false;
+        for (int32_t i = 0; i < m; ++i) any_rng = any_rng || has_rng[i];
+        if (any_rng) {
+            buf += "RANGES\n";
+            for (int32_t i = 0; i < m; ++i) {
+                if (!has_rng[i]) continue;
+                buf += "    RNG  "; buf += rn[i]; buf += "  "; buf += g17(rng[i]); buf += '\n';
+                flush(false);
+            }
+        }
+        buf += "BOUNDS\n";
+        for (int32_t j = 0; j < n; ++j) {
+            const double lo = lp->l[j], hi = lp->u[j];
+            auto line = [&](const char* t, const double* v) {
+                buf += ' '; buf += t; buf += " BND  "; buf += cn[j];
+                if (v) { buf += "  "; buf += g17(*v); }
+                buf += '\n';
+            };
+            if (lo == 0.0 && !std::signbit(lo) && hi == kInf) continue;
+            if (lo == hi) { line("FX", &lo); continue; }
+            if (lo == -kInf && hi == kInf) { line("FR", nullptr); continue; }
+            if (lo == -kInf) line("MI", nullptr);
+            else if (!(lo == 0.0 && !std::signbit(lo)) || hi < 0) line("LO", &lo);
+            if (hi != kInf) line("UP", &hi);
+            flush(false);
+        }
+        buf += "ENDATA\n";
+        flush(true);
+        if (std::fclose(f) != 0) throw std::runtime_error(std::string("write failed: ") + path);
+        return SUPR_OK;
+    } catch (const std::invalid_argument& e) {
+        supr::set_error(e.what());
+        return SUPR_EINVAL;
+    } catch (const std::bad_alloc&) {
+        supr::set_error("out of host memory");
+        return SUPR_ENOMEM;
+    } catch (const std::exception& e) {
+        supr::set_error(e.what());
+        return SUPR_EINTERNAL;
+    }
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@ eps_G, restarts), over numpy arrays.  Errors from the library raise
 SupremaError (a RuntimeError, as the reference's std::runtime_error).
 """
 import ctypes
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -17,7 +18,7 @@ from ._lib import (MODE_FAST_FP64, MODE_MIXED, MODE_PLAIN_FP64, SupremaError,  #
 
 __all__ = [
     "LP", "PdlpResult", "KKTReport", "solve_pdhg_gpu", "spmv", "verify", "scale", "build_pdlp_form",
-    "extract_solution", "scale_gpu", "csr_gpu", "solve_form_gpu", "build_pdlp_form_gpu", "verify_gpu", "solve_lp_gpu", "LpSolution", "certified_status", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count",
+    "extract_solution", "scale_gpu", "csr_gpu", "solve_form_gpu", "build_pdlp_form_gpu", "verify_gpu", "solve_lp_gpu", "LpSolution", "certified_status", "Engine", "BatchEngine", "Server", "gen_blend", "gen_blend_sizes", "gen_transport", "gen_random_lp", "zipf_degrees", "gen_power_law_lp", "bench_kernels", "bench_copy", "device_count", "read_mps", "write_mps", "MpsProblem",
     "SupremaError", "MODE_PLAIN_FP64", "MODE_FAST_FP64", "MODE_MIXED",
 ]
 
@@ -588,6 +589,69 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class MpsProblem:
+    """read_mps() + to_ranged_lp() (src/mps_reader.cpp:54-308,
+    include/ranged_lp.hpp:27-62): the LP, its names, the integer flags, the
+    QP lower triangle (empty for an LP) and whether the file maximizes."""
+    lp: LP
+    col_names: list
+    row_names: list
+    obj_name: str
+    integer: np.ndarray
+    maximize: bool
+    q_row: np.ndarray
+    q_col: np.ndarray
+    q_val: np.ndarray
+
+    @property
+    def name(self):
+        return self.lp.name
+
+
+def read_mps(path, names=True):
+    """Parses an MPS file as the reference's read_mps + to_ranged_lp do.
+    names=False skips building the Python name lists (large files)."""
+    h = ctypes.c_void_p()
+    check(lib().supr_mps_read(os.fsencode(path), ctypes.byref(h)))
+    try:
+        dims = [ctypes.c_int32() for _ in range(5)]
+        check(lib().supr_mps_sizes(h, *[ctypes.byref(d) for d in dims]))
+        m, n, nnz, qn, mx = (d.value for d in dims)
+        row, col = np.zeros(nnz, dtype=_i32), np.zeros(nnz, dtype=_i32)
+        val, c, rL, rU, l, u = (np.zeros(k) for k in (nnz, n, m, m, n, n))
+        integ = ctypes.create_string_buffer(max(1, n))
+        qr, qc, qv = np.zeros(qn, dtype=_i32), np.zeros(qn, dtype=_i32), np.zeros(qn)
+        off = ctypes.c_double(0.0)
+        check(lib().supr_mps_get(h, _ip(row), _ip(col), _dp(val), _dp(c), _dp(rL), _dp(rU), _dp(l),
+                                 _dp(u), ctypes.byref(off), integ, _ip(qr), _ip(qc), _dp(qv)))
+        nm = lib().supr_mps_name(h, 0, 0).decode()
+        cols = [lib().supr_mps_name(h, 1, j).decode() for j in range(n)] if names else []
+        rows = [lib().supr_mps_name(h, 2, i).decode() for i in range(m)] if names else []
+        obj = lib().supr_mps_name(h, 3, 0).decode()
+    finally:
+        lib().supr_mps_free(h)
+    lp = LP(m, n, row, col, val, c, rL, rU, l, u, off.value, nm)
+    flags = np.frombuffer(integ.raw[:n], dtype=np.int8).copy()
+    return MpsProblem(lp, cols, rows, obj, flags, bool(mx), qr, qc, qv)
+
+
+def write_mps(path, lp, col_names=None, row_names=None, integer=None):
+    """Writes lp as a free-format MPS file (supr_mps_write)."""
+    def arr(names, k):
+        if names is None:
+            return None
+        if len(names) != k:
+            raise ValueError("one name per column / row")
+        a = (ctypes.c_char_p * k)()
+        a[:] = [str(v).encode() for v in names]
+        return a
+    s = lp.cstruct()
+    check(lib().supr_mps_write(os.fsencode(path), ctypes.byref(s), lp.name.encode(),
+                               arr(col_names, lp.n), arr(row_names, lp.m),
+                               _flags(integer, lp.n) if integer is not None else None))
 
 
 def gen_blend(K, P, seed=1):

@@ -146,6 +146,16 @@ SIGNATURES = [
     ("supr_gen_blend_sizes", ctypes.c_int,
      [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
       ctypes.c_int32, c_int32_p, c_int32_p]),
+    ("supr_mps_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("supr_mps_sizes", ctypes.c_int, [ctypes.c_void_p] + [c_int32_p] * 5),
+    ("supr_mps_get", ctypes.c_int,
+     [ctypes.c_void_p, c_int32_p, c_int32_p] + [c_double_p] * 7 + [ctypes.c_char_p, c_int32_p,
+                                                                    c_int32_p, c_double_p]),
+    ("supr_mps_name", ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("supr_mps_free", None, [ctypes.c_void_p]),
+    ("supr_mps_write", ctypes.c_int,
+     [ctypes.c_char_p, _LP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+      ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p]),
     ("supr_gen_random_lp", ctypes.c_int,
      [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_int32_p, ctypes.c_uint64,
       ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_int32_p, c_int32_p] + [c_double_p] * 6),
