@@ -27,6 +27,9 @@ Besides the headline (mixed FP32-iterate / FP64-residual engine) it reports:
     latency from a FIFO queue over the same arrivals);
   * "c0": the random equality LP m = 2,000, n = 5,000 in each mode, beside the
     reference's own CPU PDLP run to completion;
+  * "ingest": the same 1M-variable LP as an MPS file, read by this library's
+    reader and by the reference's (same LP bit for bit), and file -> verified
+    solution through solve_lp_gpu;
   * with --large: the power-law LP (m = 500,000, n = 2,000,000, 35.1M nnz).
 """
 import argparse
@@ -322,6 +325,71 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
     }
 
 
+def bench_ingest(S=1000, D=1000):
+    """MPS ingestion of the 1M-variable transportation LP (the input format of
+    the reference's published 48.4 s / 8.1 s run: tools/gen_lp.cpp writes it,
+    src/solve.cpp:150-232 reads and solves it): the LP written as MPS with
+    gen_lp's names (x_i_j, S<i>, D<j>), read back by this library's reader
+    and by the reference's own reader (oracle/_ref, one thread), checked to
+    be the same LP bit for bit; then file -> verified solution: read_mps +
+    solve_lp_gpu (form, scale() on the device, the mixed solve, extract,
+    verify on the LP; the reference's solve_mps without presolve)."""
+    import shutil
+    import tempfile
+    lp = P.gen_transport(S, D, seed=42)
+    cols = [f"x_{i}_{j}" for i in range(S) for j in range(D)]
+    rows = [f"S{i}" for i in range(S)] + [f"D{j}" for j in range(D)]
+    d = tempfile.mkdtemp(prefix="supr_mps_")
+    try:
+        path = os.path.join(d, f"transport_{S}x{D}.mps")
+        t0 = time.perf_counter()
+        P.write_mps(path, lp, col_names=cols, row_names=rows)
+        t_write = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t_read = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            mp = P.read_mps(path, names=False)
+            t_read.append(time.perf_counter() - t0)
+        ours = mp.lp
+        out = {"workload": f"transportation LP {S}x{D} as MPS ({size / 1e6:.1f} MB, gen_lp names)",
+               "file_bytes": size, "write_seconds": t_write, "read_seconds": float(min(t_read)),
+               "read_MBps": size / min(t_read) / 1e6, "m": ours.m, "n": ours.n, "nnz": ours.nnz}
+        try:
+            sys.path.insert(0, ROOT)
+            from oracle import available, read_mps_ref
+            if available("ref"):
+                t0 = time.perf_counter()
+                r = read_mps_ref(path)
+                t_ref = time.perf_counter() - t0
+                same = all(np.asarray(getattr(ours, k)).tobytes() == np.asarray(r[k]).tobytes()
+                           for k in ("row", "col", "val", "c", "rL", "rU", "l", "u"))
+                if not same:
+                    raise SystemExit("ingest: the MPS reader differs from the reference reader")
+                out["reference_read_seconds"] = t_ref
+                out["speedup_vs_reference_reader"] = t_ref / min(t_read)
+                out["identical_to_reference_reader"] = True
+        except ImportError as e:   # pragma: no cover
+            log(f"ingest: reference reader unavailable: {e}")
+        P.solve_lp_gpu(ours, tol=TOL)   # warm-up (device initialisation, kernel loads)
+        t0 = time.perf_counter()
+        mp = P.read_mps(path, names=False)
+        sol = P.solve_lp_gpu(mp.lp, tol=TOL)
+        e2e = time.perf_counter() - t0
+        worst = max(sol.report.eps_P, sol.report.eps_D, sol.report.eps_G)
+        if sol.status != "optimal" or worst > TOL:
+            raise SystemExit(f"ingest: file -> solution not verified ({sol.status}, eps {worst:.3e})")
+        out["file_to_verified_solution_seconds"] = e2e
+        out["file_to_solution"] = {"status": sol.status, "attempts": sol.iterations,
+                                   "objective": sol.report.objective, "verified_eps": worst}
+        log(f"ingest: {size / 1e6:.1f} MB MPS read in {min(t_read):.3f} s"
+            + (f" (reference reader {out['reference_read_seconds']:.2f} s)" if "reference_read_seconds" in out else "")
+            + f"; file -> verified solution {e2e:.2f} s")
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def bench_large_kernels(reps=50):
     """The power-law LP's iteration kernels alone (HIP events), with their
     algorithmic bytes -- the run to profile them."""
@@ -368,7 +436,9 @@ def main():
     ap.add_argument("--serving-full", action="store_true",
                     help="serving stream over the full nnz range [1e4, 2e7] (long)")
     ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
-    ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-kernels"],
+    ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-kernels",
+                                       "ingest"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain device copy of known size (reference for HBM counters)")
@@ -393,6 +463,7 @@ def main():
 
     if args.only:
         fn = {"c0": bench_c0, "batched": bench_batched, "serving": bench_serving, "large": bench_large,
+              "ingest": bench_ingest,
               "large-kernels": bench_large_kernels,
               "serving-full": lambda: bench_serving(count=16, qps=0.5, nnz_hi=2e7,
                                                     plain_sample_attempts=1000)}[args.only]
@@ -474,6 +545,7 @@ def main():
         serving = (bench_serving(count=16, qps=0.5, nnz_hi=2e7, plain_sample_attempts=1000)
                    if args.serving_full else bench_serving())
     large = bench_large() if args.large else None
+    ingest = None if args.no_ingest else bench_ingest()
 
     head = results["mixed"]
     best_name = min(results, key=lambda k: results[k]["seconds"])
@@ -523,6 +595,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "large": large,
+        "ingest": ingest,
     }
     print(json.dumps(out))
 

@@ -146,21 +146,63 @@ bool unquoted_eq(std::string_view t, const char* word) {
     return k == wl;
 }
 
-// Names interned to dense ids (views into the file buffer or static text).
+// Names interned to dense ids (views into the file buffer or static text):
+// an open-addressing table of (hash, id) slots, linear probing, at most half
+// full -- no allocation per name, one cache line per probe.
 struct Names {
-    std::unordered_map<std::string_view, int32_t> id;
+    struct Slot { uint64_t hash; int64_t id; };   // id -1: empty
     std::vector<std::string_view> name;
+    std::vector<Slot> slots;
+    size_t mask = 0;
+
+    static uint64_t hash(std::string_view s) {
+        uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)s.size();
+        size_t i = 0;
+        for (; i + 8 <= s.size(); i += 8) {
+            uint64_t w;
+            std::memcpy(&w, s.data() + i, 8);
+            h = (h ^ w) * 0xFF51AFD7ED558CCDull;
+            h ^= h >> 32;
+        }
+        uint64_t w = 0;
+        std::memcpy(&w, s.data() + i, s.size() - i);
+        h = (h ^ w) * 0xC4CEB9FE1A85EC53ull;
+        return h ^ (h >> 29);
+    }
+    void grow() {
+        const size_t cap = mask ? 2 * (mask + 1) : 1024;
+        std::vector<Slot> ns(cap, Slot{0, -1});
+        for (const Slot& e : slots) {
+            if (e.id < 0) continue;
+            size_t q = e.hash & (cap - 1);
+            while (ns[q].id >= 0) q = (q + 1) & (cap - 1);
+            ns[q] = e;
+        }
+        slots.swap(ns);
+        mask = cap - 1;
+    }
     int32_t get(std::string_view s) {
-        auto it = id.find(s);
-        if (it != id.end()) return it->second;
+        if (2 * (name.size() + 1) > mask + 1) grow();
+        const uint64_t h = hash(s);
+        size_t q = h & mask;
+        while (slots[q].id >= 0) {
+            if (slots[q].hash == h && name[slots[q].id] == s) return (int32_t)slots[q].id;
+            q = (q + 1) & mask;
+        }
         const int32_t k = (int32_t)name.size();
-        id.emplace(s, k);
+        slots[q] = Slot{h, k};
         name.push_back(s);
         return k;
     }
     int32_t find(std::string_view s) const {
-        auto it = id.find(s);
-        return it == id.end() ? -1 : it->second;
+        if (!mask) return -1;
+        const uint64_t h = hash(s);
+        size_t q = h & mask;
+        while (slots[q].id >= 0) {
+            if (slots[q].hash == h && name[slots[q].id] == s) return (int32_t)slots[q].id;
+            q = (q + 1) & mask;
+        }
+        return -1;
     }
 };
 
@@ -201,6 +243,8 @@ void parse(const std::string& text, bool fixed_format, Parsed& out) {
     std::vector<char> integ, has_lo, has_hi, explicit_lo;
     std::vector<double> blo, bhi;
     bool int_marker = false;
+    int32_t last_col = -1;            // the column of the previous COLUMNS line
+    std::string_view last_col_name;
     auto add_col = [&](std::string_view nm) {
         const int32_t before = (int32_t)cols.name.size();
         const int32_t j = cols.get(nm);
@@ -326,7 +370,9 @@ void parse(const std::string& text, bool fixed_format, Parsed& out) {
                 break;
             }
             if (t.empty()) fail("empty COLUMNS line");
-            const int32_t j = add_col(t[0]);
+            // a column's lines are usually consecutive: reuse the last lookup
+            if (last_col < 0 || t[0] != last_col_name) { last_col = add_col(t[0]); last_col_name = t[0]; }
+            const int32_t j = last_col;
             if (int_marker) integ[j] = 1;
             for (size_t i = 1; i + 1 < t.size(); i += 2) {
                 const double v = parse_double(t[i + 1]);
@@ -648,12 +694,17 @@ int supr_mps_write(const char* path, const supr_lp* lp, const char* name,
                 buf += ij ? "    MARKER  'MARKER'  'INTORG'\n" : "    MARKER  'MARKER'  'INTEND'\n";
                 in_int = ij;
             }
-            buf += "    "; buf += cn[j]; buf += "  "; buf += obj; buf += "  "; buf += g17(lp->c[j]); buf += '\n';
+            // two (row, value) pairs per line, the objective first
+            buf += "    "; buf += cn[j]; buf += "  "; buf += obj; buf += "  "; buf += g17(lp->c[j]);
+            bool open = true;
             for (int32_t q = ptr[j]; q < ptr[j + 1]; ++q) {
                 const int32_t k = ord[q];
-                buf += "    "; buf += cn[j]; buf += "  "; buf += rn[lp->row_idx[k]]; buf += "  ";
-                buf += g17(lp->val[k]); buf += '\n';
+                if (!open) { buf += "    "; buf += cn[j]; }
+                buf += "  "; buf += rn[lp->row_idx[k]]; buf += "  "; buf += g17(lp->val[k]);
+                if (open) buf += '\n';
+                open = !open;
             }
+            if (open) buf += '\n';
             flush(false);
         }
         if (in_int) buf += "    MARKER  'MARKER'  'INTEND'\n";
