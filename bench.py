@@ -30,6 +30,8 @@ Besides the headline (mixed FP32-iterate / FP64-residual engine) it reports:
   * "ingest": the same 1M-variable LP as an MPS file, read by this library's
     reader and by the reference's (same LP bit for bit), and file -> verified
     solution through solve_lp_gpu;
+  * "prep": scale() on the device against the host version and the
+    reference's, bit for bit, at 2M and 7M nonzeros;
   * with --large: the power-law LP (m = 500,000, n = 2,000,000, 35.1M nnz).
 """
 import argparse
@@ -390,6 +392,58 @@ def bench_ingest(S=1000, D=1000):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def bench_prep():
+    """Preprocessing in front of every solve: scale() (10 Ruiz passes +
+    Pock-Chambolle, src/scaling.cpp:25-84) on the device (supr_scale_gpu:
+    upload, stable CSR / CSC build, scaling, download) against the host
+    version (supr_scale) and the reference's own scale() (oracle/_ref, one
+    thread), on the 1M-variable transportation LP and a 7M-nonzero power-law
+    LP; all three results compared bit for bit."""
+    out = {}
+    sys.path.insert(0, ROOT)
+    try:
+        from oracle import Checker, available
+        ref = Checker("ref") if available("ref") else None
+    except ImportError:   # pragma: no cover
+        ref = None
+    cases = (("transport_1000x1000", lambda: P.gen_transport(1000, 1000, seed=42)),
+             ("power_law_100000x400000", lambda: P.gen_power_law_lp(100000, 400000, seed=42)))
+    for name, make in cases:
+        lp = make()
+        P.scale_gpu(lp)   # warm-up
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            dev = P.scale_gpu(lp, timing=True)
+            t = time.perf_counter() - t0
+            if best is None or t < best[0]:
+                best = (t, dev)
+        t_dev, (ds, dDr, dDc, secs) = best
+        t0 = time.perf_counter()
+        hs, hDr, hDc = P.scale(lp)
+        t_host = time.perf_counter() - t0
+        mine = [ds.val, ds.c, ds.rL, ds.rU, ds.l, ds.u, dDr, dDc]
+        host = [hs.val, hs.c, hs.rL, hs.rU, hs.l, hs.u, hDr, hDc]
+        same = all(a.tobytes() == b.tobytes() for a, b in zip(mine, host))
+        rec = {"m": lp.m, "n": lp.n, "nnz": lp.nnz, "device_seconds": t_dev,
+               "device_layout_seconds": float(secs[0]), "device_scale_seconds": float(secs[1]),
+               "device_download_seconds": float(secs[2]), "host_seconds": t_host,
+               "device_equals_host": same}
+        if ref is not None:
+            t0 = time.perf_counter()
+            rs = ref.scale(lp)
+            rec["reference_seconds"] = time.perf_counter() - t0
+            rec["device_equals_reference"] = all(a.tobytes() == np.asarray(b).tobytes()
+                                                 for a, b in zip(mine, rs))
+            rec["speedup_vs_reference"] = rec["reference_seconds"] / t_dev
+        if not same or not rec.get("device_equals_reference", True):
+            raise SystemExit(f"prep: device scale() differs on {name}")
+        out[name] = rec
+        log(f"prep {name}: device scale {t_dev:.3f} s (scaling kernels {secs[1] * 1e3:.1f} ms), host "
+            f"{t_host:.2f} s" + (f", reference {rec['reference_seconds']:.2f} s" if ref else ""))
+    return out
+
+
 def bench_large_kernels(reps=50):
     """The power-law LP's iteration kernels alone (HIP events), with their
     algorithmic bytes -- the run to profile them."""
@@ -437,13 +491,16 @@ def main():
                     help="serving stream over the full nnz range [1e4, 2e7] (long)")
     ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--no-prep", action="store_true")
     ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-kernels",
-                                       "ingest"],
+                                       "ingest", "prep"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain device copy of known size (reference for HBM counters)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone iteration-kernel timing (for profiling it)")
+    ap.add_argument("--lanes", default=None,
+                    help="A,AT lanes per row for --kernels-only (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -463,7 +520,7 @@ def main():
 
     if args.only:
         fn = {"c0": bench_c0, "batched": bench_batched, "serving": bench_serving, "large": bench_large,
-              "ingest": bench_ingest,
+              "ingest": bench_ingest, "prep": bench_prep,
               "large-kernels": bench_large_kernels,
               "serving-full": lambda: bench_serving(count=16, qps=0.5, nnz_hi=2e7,
                                                     plain_sample_attempts=1000)}[args.only]
@@ -475,7 +532,11 @@ def main():
     scaled, Dr, Dc = P.scale(lp)
 
     if args.kernels_only:
-        kb = P.bench_kernels(lp, scaled, Dr, Dc, reps=200)
+        pin = {}
+        if args.lanes:
+            la, lat = (int(v) for v in args.lanes.split(","))
+            pin = dict(spmv_lanes_A=la, spmv_lanes_AT=lat)
+        kb = P.bench_kernels(lp, scaled, Dr, Dc, reps=200, **pin)
         print(json.dumps({"kernels_ms": kb}))
         return
 
@@ -546,6 +607,7 @@ def main():
                    if args.serving_full else bench_serving())
     large = bench_large() if args.large else None
     ingest = None if args.no_ingest else bench_ingest()
+    prep = None if args.no_prep else bench_prep()
 
     head = results["mixed"]
     best_name = min(results, key=lambda k: results[k]["seconds"])
@@ -596,6 +658,7 @@ def main():
         "serving": serving,
         "large": large,
         "ingest": ingest,
+        "prep": prep,
     }
     print(json.dumps(out))
 
