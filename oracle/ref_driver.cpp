@@ -8,7 +8,7 @@
 //   ref_scale        -> scale()              src/scaling.cpp:25-84
 //   ref_verify       -> verify()             src/verify.cpp:18-103
 //   ref_pdlp_form_*  -> build_pdlp_form()    src/pdlp.cpp:8-82
-//   ref_pdlp         -> solve_pdhg()         src/pdlp.cpp:146-152 (pdlp_run)
+//   ref_pdlp[_live]  -> solve_pdhg()         src/pdlp.cpp:146-152 (pdlp_run)
 //   ref_mps_*        -> read_mps() + to_ranged_lp() (arrays, integer flags,
 //                       QP triangle, names, error text)
 //                       src/mps_reader.cpp:54-308, include/ranged_lp.hpp:39-62
@@ -174,6 +174,34 @@ int ref_mps_arrays(int* ri, int* ci, double* v, double* c, double* rL, double* r
     std::memcpy(u, r.u.data(), sizeof(double) * n);
     *obj_offset = r.obj_offset;
     return r.original_sense == "max" ? 1 : 0;
+}
+
+// ref_pdlp with a live stop flag: flag points at a std::atomic<bool> the
+// caller may set from another thread while solve_pdhg runs (its relaxed load
+// once per outer iteration, include/pdlp_algo.hpp:104) -- for race tests.
+int ref_pdlp_live(int m, int n, int nnz, const int* ri, const int* ci,
+                  const double* val_u, const double* c_u, const double* rL_u, const double* rU_u,
+                  const double* l_u, const double* u_u, double obj_offset,
+                  const double* val_s, const double* c_s, const double* rL_s, const double* rU_s,
+                  const double* l_s, const double* u_s, const double* Dr, const double* Dc,
+                  int max_iterations, int check_every, double tol, void* flag,
+                  double* x_out, double* y_out, int* info_i, double* info_d) {
+    try {
+        RangedLP U = make_lp(m, n, nnz, ri, ci, val_u, c_u, rL_u, rU_u, l_u, u_u, obj_offset);
+        RangedLP S = make_lp(m, n, nnz, ri, ci, val_s, c_s, rL_s, rU_s, l_s, u_s, obj_offset);
+        std::vector<double> dr(Dr, Dr + m), dc(Dc, Dc + n);
+        PdlpResult r = solve_pdhg(U, S, dr, dc, 0.99, max_iterations, check_every, tol,
+                                  static_cast<std::atomic<bool>*>(flag));
+        const bool has = r.x.size() == (size_t)n && r.y.size() == (size_t)m;
+        if (has) {
+            std::memcpy(x_out, r.x.data(), sizeof(double) * n);
+            std::memcpy(y_out, r.y.data(), sizeof(double) * m);
+        }
+        info_i[0] = r.iterations; info_i[1] = r.restarts; info_i[2] = r.converged ? 1 : 0;
+        info_i[3] = has ? 1 : 0;
+        info_d[0] = r.eps_P; info_d[1] = r.eps_D; info_d[2] = r.eps_G;
+        return 0;
+    } catch (const std::exception&) { return -1; }
 }
 
 // The rest of the last MPS read: integer flags (n bytes), the QP lower

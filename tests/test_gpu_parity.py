@@ -233,6 +233,43 @@ def test_cpp_drop_in_api():
 
 
 @pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_direct_launches_match_graph_replay(mode):
+    """graph_attempts < 0 launches the replay's kernel sequence on the stream
+    directly: the same kernels in the same order, so the same bits."""
+    lp = P.gen_random_lp(300, 800, 6, seed=21)
+    s, Dr, Dc = P.scale(lp)
+    runs = [P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, graph_attempts=g,
+                             spmv_lanes_A=4, spmv_lanes_AT=1) for g in (16, -16, -6)]
+    assert runs[0].converged
+    for r in runs[1:]:
+        assert r.iterations == runs[0].iterations and r.restarts == runs[0].restarts
+        assert np.array_equal(r.x, runs[0].x) and np.array_equal(r.y, runs[0].y)
+
+
+def test_cpu_gpu_race_through_the_drop_in():
+    """The reference's CPU / GPU race (src/solve.cpp:76-148): the reference's
+    own CPU solve_pdhg (oracle/_ref) and this engine on two threads sharing one
+    stop flag.  The GPU wins a 300,000-variable LP (verified) and the CPU stops
+    early; with a staggered GPU start the CPU wins a small LP and the GPU
+    returns at its first poll without converging."""
+    import json
+    import os
+    import subprocess
+    from conftest import ROOT
+    from oracle import PATHS, available
+    if not available("ref"):
+        pytest.skip("reference CPU engine (oracle/_ref) not built")
+    exe = os.path.join(ROOT, "team-suprema_amd", "_build", "race")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    r = subprocess.run([exe, PATHS["ref"]], capture_output=True, text=True, timeout=120)
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    print(out)
+    assert r.returncode == 0 and out["failures"] == 0, (r.stdout, r.stderr)
+    assert out["gpu_wins"]["winner"] == 1 and out["gpu_wins"]["verified_eps"] <= 1e-6
+    assert out["cpu_wins"]["winner"] == 0 and out["cpu_wins"]["gpu_converged"] == 0
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
 def test_staged_schedules_solve_a_tall_lp(oracle, mode):
     """A (140,000 rows) and A^T (200,000 rows) both run the LDS-staged
     schedules (lanes forced to 4 and 2): the solve must reach a point the
