@@ -27,6 +27,8 @@ Besides the headline (mixed FP32-iterate / FP64-residual engine) it reports:
     latency from a FIFO queue over the same arrivals);
   * "c0": the random equality LP m = 2,000, n = 5,000 in each mode, beside the
     reference's own CPU PDLP run to completion;
+  * "polish_1e-8": the same LP to a verified 1e-8 (the FP64 polish);
+  * each mode's objective error against the reference CPU PDLP's;
   * "ingest": the same 1M-variable LP as an MPS file, read by this library's
     reader and by the reference's (same LP bit for bit), and file -> verified
     solution through solve_lp_gpu;
@@ -72,6 +74,27 @@ def kernel_bytes(lp, s, ell_AT, ell_A):
     primal = nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + 8 * s * n + (8 * n if pair else 0)
     dual = nnz * (4 + s) + (0 if ell_A else 4 * (m + 1)) + (8 if pair else s) * n + 7 * s * m + m
     return primal, dual
+
+
+def moved_bytes(lp, attempts, attempts_fp32):
+    """Algorithmic bytes the iteration kernels move over a solve of lp (CSR
+    layout, as in the batched and served engines): FP32-loop attempts at the
+    FP32 figure, the rest at the FP64 one."""
+    p32, d32 = kernel_bytes(lp, 4, False, False)
+    p64, d64 = kernel_bytes(lp, 8, False, False)
+    return attempts_fp32 * (p32 + d32) + (attempts - attempts_fp32) * (p64 + d64)
+
+
+def cpu_per_attempt(U, S, Dr, Dc, sample_attempts):
+    """Seconds per attempt of the reference's CPU PDLP (oracle/_ref, one
+    thread; the oracle's restatement where _ref is absent) over a bounded
+    prefix of the solve, and the kind of engine timed."""
+    sys.path.insert(0, ROOT)
+    from oracle import Checker, available
+    kind = "ref" if available("ref") else "oracle"
+    t0 = time.perf_counter()
+    out = Checker(kind).pdlp(U, S, Dr, Dc, sample_attempts, 64, TOL)
+    return (time.perf_counter() - t0) / max(1, out.iterations), ("reference" if kind == "ref" else "port")
 
 
 def cpu_baseline(lp, scaled, Dr, Dc, sample_attempts):
@@ -138,8 +161,12 @@ def bench_c0():
         kind = "ref" if available("ref") else "oracle"
         t0 = time.perf_counter()
         ref = Checker(kind).pdlp(lp, s, Dr, Dc, 200000, 64, TOL)
-        out["cpu_reference"] = dict(seconds=time.perf_counter() - t0, attempts=ref.iterations, cores=1,
+        dt = time.perf_counter() - t0
+        ref_obj = P.verify(lp, ref.x, ref.y).objective
+        out["cpu_reference"] = dict(seconds=dt, attempts=ref.iterations, cores=1, objective=ref_obj,
                                     kind="reference" if kind == "ref" else "port")
+        for name in ("mixed", "fast_fp64"):
+            out[name]["objective_rel_error_vs_cpu"] = abs(out[name]["objective"] - ref_obj) / (1 + abs(ref_obj))
     except Exception as e:  # pragma: no cover
         log(f"c0 CPU run unavailable: {e}")
     log(f"c0: mixed {out['mixed']['seconds'] * 1e3:.1f} ms, fast64 {out['fast_fp64']['seconds'] * 1e3:.1f} ms, "
@@ -204,8 +231,25 @@ def bench_batched(count=512, sample=8):
                 raise SystemExit(f"batched baseline {label}: instance {i} did not converge")
         seq[label] = dict(mean_seconds=float(np.mean(ts)), solves_per_s=1.0 / float(np.mean(ts)))
     sps = count / secs
+    moved = sum(moved_bytes(f[0], r.iterations, r.info["iterations_fp32"]) for f, r in zip(forms, res))
+    cpu = None
+    try:
+        est = []
+        for i in pick:
+            per, kind = cpu_per_attempt(*forms[i], min(3000, res[i].iterations))
+            est.append(per * res[i].iterations)
+        cores = os.cpu_count() or 1
+        cpu = dict(kind=kind, cores=1, solves_per_s=1.0 / float(np.mean(est)),
+                   all_cores_bound_solves_per_s=cores / float(np.mean(est)), host_cpus=cores,
+                   sample="the sequential sample's instances: the CPU PDLP timed over up to 3,000 attempts each, "
+                          "scaled by the instance's attempt count in the batch; all_cores_bound = host_cpus x "
+                          "the one-thread rate (independent instances, no contention)")
+    except Exception as e:  # pragma: no cover
+        log(f"batched: CPU baseline unavailable: {e}")
     log(f"batched: {count} LPs in {secs:.3f} s = {sps:.1f} solves/s; sequential plain "
-        f"{seq['plain_fp64']['solves_per_s']:.2f}/s, resident mixed {seq['resident_mixed']['solves_per_s']:.2f}/s")
+        f"{seq['plain_fp64']['solves_per_s']:.2f}/s, resident mixed {seq['resident_mixed']['solves_per_s']:.2f}/s"
+        + (f", CPU 1 thread {cpu['solves_per_s']:.2f}/s" if cpu else "")
+        + f"; {moved / secs / 1e9:.0f} GB/s of algorithmic bytes")
     return {
         "workload": f"{count} blending LPs (K 20-400 x P 10-200 log-uniform, seed 2026), mixed mode, tol 1e-6, "
                     "every result verified on the original LP",
@@ -218,6 +262,10 @@ def bench_batched(count=512, sample=8):
         "sequential_sample": {"instances": pick, **seq},
         "speedup_vs_sequential_plain": sps / seq["plain_fp64"]["solves_per_s"],
         "speedup_vs_sequential_resident_mixed": sps / seq["resident_mixed"]["solves_per_s"],
+        "roofline": {"bound": "hbm", "achieved": moved / secs / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": moved / secs / 1e9 / HBM_PEAK_GBS,
+                     "note": "algorithmic bytes of every instance's attempts (CSR, FP32 / FP64 loop) over the batch time"},
+        "cpu_baseline": cpu,
     }
 
 
@@ -286,10 +334,14 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
             raise SystemExit(f"serving: not all requests verified (worst eps {worst:.3e})")
         done = np.array(done)
         lat = done - (arrivals if open_loop else 0.0)
+        moved = sum(moved_bytes(reqs[i][0], res[i].iterations, res[i].info["iterations_fp32"])
+                    for i in range(count))
         return dict(makespan=float(done.max()), solves_per_s=count / float(done.max()),
                     latency_p50=_pct(lat, 50), latency_p99=_pct(lat, 99),
                     worst_verified_eps=worst, device_bytes=dev,
-                    attempts=[int(r.iterations) for r in res])
+                    attempts=[int(r.iterations) for r in res],
+                    achieved_GBps=moved / float(done.max()) / 1e9,
+                    frac_of_hbm_peak=moved / float(done.max()) / 1e9 / HBM_PEAK_GBS)
 
     sat = serve(False)
     ol = serve(True)
@@ -315,6 +367,19 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
                  measured=("full solves" if not plain_sample_attempts else
                            f"estimated: {plain_sample_attempts} timed attempts per request x the "
                            "request's attempt count in the server"))
+    cpu = None
+    try:
+        est = []
+        for i in range(count):
+            per, kind = cpu_per_attempt(*reqs[i], min(300, sat["attempts"][i]))
+            est.append(per * sat["attempts"][i])
+        cores = os.cpu_count() or 1
+        cpu = dict(kind=kind, cores=1, solves_per_s=count / float(np.sum(est)),
+                   all_cores_bound_solves_per_s=cores * count / float(np.sum(est)), host_cpus=cores,
+                   sample="each request's CPU PDLP timed over up to 300 attempts, scaled by its attempt count "
+                          "in the server; all_cores_bound = host_cpus x the one-thread rate")
+    except Exception as e:  # pragma: no cover
+        log(f"serving: CPU baseline unavailable: {e}")
     log(f"serving: saturated {sat['solves_per_s']:.2f} solves/s; open loop {qps} qps p50 "
         f"{ol['latency_p50']:.2f} s p99 {ol['latency_p99']:.2f} s; plain one-at-a-time "
         f"{plain['solves_per_s']:.3f} solves/s, p50 {plain['latency_p50']:.1f} s p99 {plain['latency_p99']:.1f} s")
@@ -324,7 +389,30 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
         "total_nnz": int(sum(r[0].nnz for r in reqs)),
         "saturated": sat, "open_loop": ol, "plain_fp64_one_at_a_time": plain,
         "speedup_solves_per_s_vs_plain": sat["solves_per_s"] / plain["solves_per_s"],
+        "cpu_baseline": cpu,
     }
+
+
+def bench_polish(lp, scaled, Dr, Dc, tol=1e-8):
+    """The 1M-variable LP to a tighter verified tolerance: the mixed mode's
+    FP32 loop hands over to the FP64 loop (the polish), which restores 1e-8
+    relative KKT; the device-resident FP64 mode beside it."""
+    out = {"workload": f"transportation LP 1000x1000, tol {tol:g} relative KKT (verified), LP resident in HBM"}
+    for name, mode in (("mixed", P.MODE_MIXED), ("fast_fp64", P.MODE_FAST_FP64)):
+        with P.Engine(lp, scaled, Dr, Dc, tol=tol, mode=mode, max_iterations=500000) as eng:
+            eng.solve()
+            t0 = time.perf_counter()
+            r = eng.solve()
+            dt = time.perf_counter() - t0
+        rep = P.verify(lp, r.x, r.y)
+        worst = max(rep.eps_P, rep.eps_D, rep.eps_G)
+        if not r.converged or worst > tol:
+            raise SystemExit(f"polish {name}: not verified at {tol:g} (eps {worst:.3e})")
+        out[name] = dict(seconds=dt, attempts=r.iterations, fp32_attempts=r.info["iterations_fp32"],
+                         verified_eps=[rep.eps_P, rep.eps_D, rep.eps_G], objective=rep.objective)
+    log(f"polish to {tol:g}: mixed {out['mixed']['seconds'] * 1e3:.1f} ms ({out['mixed']['attempts']} attempts, "
+        f"{out['mixed']['fp32_attempts']} FP32), fast64 {out['fast_fp64']['seconds'] * 1e3:.1f} ms")
+    return out
 
 
 def bench_ingest(S=1000, D=1000):
@@ -566,6 +654,16 @@ def main():
                              eps=[rep.eps_P, rep.eps_D, rep.eps_G], setup_seconds=setup_s,
                              fp32_iterations=res.info["iterations_fp32"])
 
+    ref_obj = None
+    ref_path = os.path.join(ROOT, "tests", "golden", "c1_reference.json")
+    if os.path.exists(ref_path):
+        with open(ref_path) as f:
+            ref_obj = json.load(f)["objective"]
+        for r in results.values():
+            r["objective_rel_error_vs_cpu_reference"] = abs(r["objective"] - ref_obj) / (1 + abs(ref_obj))
+
+    polish = bench_polish(lp, scaled, Dr, Dc)
+
     plain = None
     if not args.no_plain:
         log("plain FP64 mode (host-driven, reference GPU design): one solve")
@@ -659,6 +757,8 @@ def main():
         "large": large,
         "ingest": ingest,
         "prep": prep,
+        "polish_1e-8": polish,
+        "cpu_reference_objective": ref_obj,
     }
     print(json.dumps(out))
 

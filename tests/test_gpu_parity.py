@@ -138,6 +138,54 @@ def test_netlib_original_objective_matches_published(oracle, golden, name):
     assert max(rep[:3]) <= 1e-5
 
 
+@pytest.mark.parametrize("name", ["afiro", "blend", "israel", "share1b", "beaconfd", "med"])
+def test_mps_file_to_verified_solution(oracle, golden, tmp_path, name):
+    """The solve_mps path without presolve, from the MPS file itself: this
+    library's reader (tests/golden/mps/<name>.mps.gz) -> solve_lp_gpu (form,
+    scale, solve, extract, verify on the LP) -> the published optimum within
+    1e-4 where one exists (tools/audit_lp.cpp:270-284), the oracle's verifier
+    agreeing with the device's report, and the same LP as the fixture the
+    reference's reader made."""
+    import gzip
+    import os
+    from conftest import GOLDEN_DIR
+    with gzip.open(os.path.join(GOLDEN_DIR, "mps", name + ".mps.gz"), "rb") as f:
+        path = tmp_path / (name + ".mps")
+        path.write_bytes(f.read())
+    mp = P.read_mps(str(path))
+    ref_lp, maximize = load_fixture(name)
+    assert mp.maximize == maximize and np.array_equal(mp.lp.val, ref_lp.val)
+    sol = P.solve_lp_gpu(mp.lp, tol=TOL, max_iterations=500000)
+    assert sol.status in ("optimal", "near_optimal"), sol.status
+    rep = oracle.verify(mp.lp, sol.x, sol.y)
+    np.testing.assert_allclose([sol.report.eps_P, sol.report.eps_D, sol.report.eps_G], rep[:3],
+                               rtol=1e-9, atol=1e-13)
+    assert max(rep[:3]) <= 10 * TOL
+    pub = golden[name].get("published_optimum")
+    if pub is not None:
+        obj = -rep[3] if maximize else rep[3]
+        assert abs(obj - pub) / (1 + abs(pub)) <= 1e-4
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_polish_restores_1e8(oracle, mode):
+    """The FP32 loop cannot reach 1e-8; in the mixed mode the FP64 polish
+    takes over and restores it.  The random equality LP m = 2,000, n = 5,000
+    to 1e-8, certified by the oracle's verifier, with the objective the FP64
+    oracle reaches at 1e-8."""
+    lp = P.gen_random_lp(2000, 5000, 6, seed=42)
+    s, Dr, Dc = P.scale(lp)
+    res = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=1e-8, mode=mode, max_iterations=500000)
+    assert res.converged
+    rep = oracle.verify(lp, res.x, res.y)
+    assert max(rep[:3]) <= 1e-8, rep[:3]
+    if mode == P.MODE_MIXED:
+        assert 0 < res.info["iterations_fp32"] < res.iterations   # the polish ran
+    ref = oracle.pdlp(lp, s, Dr, Dc, 500000, 64, 1e-8)
+    ref_obj = oracle.verify(lp, ref.x, ref.y)[3]
+    assert abs(rep[3] - ref_obj) / (1 + abs(ref_obj)) <= 1e-7
+
+
 @pytest.fixture(scope="module")
 def transport_1m():
     lp = P.gen_transport(1000, 1000, seed=42)
