@@ -167,6 +167,34 @@ def test_mps_file_to_verified_solution(oracle, golden, tmp_path, name):
         assert abs(obj - pub) / (1 + abs(pub)) <= 1e-4
 
 
+@pytest.mark.parametrize("name", ["tiny_infeasible", "tiny_unbounded"])
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64, P.MODE_PLAIN_FP64])
+def test_infeasible_and_unbounded_samples_run_to_the_limit(oracle, tmp_path, name, mode):
+    """The reference's own infeasible / unbounded samples (sample_problems/):
+    PDLP has no infeasibility detection, so the reference engine (and the
+    oracle, checked here) runs to max_iterations and returns its best point,
+    not converged.  The GPU engine must do the same in every mode -- no fault,
+    no early convergence claim, a finite point -- and solve_lp_gpu must report
+    iteration_limit."""
+    import gzip
+    import os
+    from conftest import GOLDEN_DIR
+    with gzip.open(os.path.join(GOLDEN_DIR, "mps", name + ".mps.gz"), "rb") as f:
+        path = tmp_path / (name + ".mps")
+        path.write_bytes(f.read())
+    lp = P.read_mps(str(path)).lp
+    form, meta = P.build_pdlp_form(lp)
+    s, Dr, Dc = P.scale(form)
+    ref = oracle.pdlp(form, s, Dr, Dc, 5000, 64, TOL)
+    assert not ref.converged and ref.iterations == 5000
+    res = P.solve_pdhg_gpu(form, s, Dr, Dc, tol=TOL, max_iterations=5000, mode=mode)
+    assert not res.converged and res.iterations == 5000
+    assert res.x.shape == (form.n,) and np.isfinite(res.x).all() and np.isfinite(res.y).all()
+    if mode != P.MODE_PLAIN_FP64:
+        sol = P.solve_lp_gpu(lp, tol=TOL, max_iterations=5000, mode=mode)
+        assert sol.status == "iteration_limit" and not sol.converged
+
+
 @pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
 def test_polish_restores_1e8(oracle, mode):
     """The FP32 loop cannot reach 1e-8; in the mixed mode the FP64 polish
