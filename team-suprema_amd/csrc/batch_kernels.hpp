@@ -226,16 +226,18 @@ __global__ __launch_bounds__(kBlock) void kb_primal(DevState* __restrict__ S, in
     const int lb = b - I.on;
     DevState* Sk = S + 2 * k;
     // the publisher is the instance's last block (the fewest rows): the step
-    // powers it forms at the end stay off the launch's critical path
+    // powers it forms before its rows stay off the launch's critical path
     const bool publisher = lb == I.bn - 1;
     load_and_decide(Sk, p, part_m + 2 * (size_t)I.om, I.bm, dx2 + k, Q, publisher);
     if (!Q.active) return;
-    if (publisher && threadIdx.x == 0) Sk[p ^ 1].decide_pending = 1;   // kb_dual will run
+    if (publisher && threadIdx.x == 0) {
+        Sk[p ^ 1].decide_pending = 1;   // kb_dual will run
+        publish_step_powers(Sk, p, Q);
+    }
     double acc[1];
     acc[0] = Q.phase == 0 ? primal_rows<float>(g, I, lb, Q, V, F) : primal_rows<double>(g, I, lb, Q, V, D);
     block_sum<1>(acc);
     if (threadIdx.x == 0) part_n[b] = acc[0];
-    if (publisher && threadIdx.x == 0) publish_step_powers(Sk, p, Q);
 }
 
 // Dual half (k_dual of pdlp_kernels.hpp) over one instance's rows.

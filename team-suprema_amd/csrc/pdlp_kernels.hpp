@@ -125,11 +125,17 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
                                                    double* __restrict__ part_n) {
     __shared__ DevState Q;
     // the publisher is the last block: its slot has the fewest rows, so the
-    // step powers it forms at the end stay off the launch's critical path
+    // step powers it forms before its rows stay off the launch's critical
+    // path.  (Formed after the rows instead, the publisher flag and S stay
+    // live through the row loops, and the FP64 variant then reloads kernel
+    // arguments inside its pass loop: 40% slower on the power-law LP.)
     const bool publisher = blockIdx.x == gridDim.x - 1;
     load_and_decide(S, p, part_m, nb_m, dx2_slot, Q, publisher);
     if (!Q.active) return;
-    if (publisher && threadIdx.x == 0) S[p ^ 1].decide_pending = 1;   // k_dual will run
+    if (publisher && threadIdx.x == 0) {
+        S[p ^ 1].decide_pending = 1;   // k_dual will run
+        publish_step_powers(S, p, Q);
+    }
     const int cur = Q.cur;
     const bool need_aty = Q.need_aty != 0;
     const bool pending = Q.pending != 0;
@@ -176,7 +182,6 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) part_n[blockIdx.x] = acc[0];
-    if (publisher && threadIdx.x == 0) publish_step_powers(S, p, Q);
 }
 
 // Dual half of one attempt, over the rows of A:
