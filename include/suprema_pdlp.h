@@ -370,7 +370,12 @@ int supr_gen_blend_sizes(int32_t count, uint64_t seed, int32_t K_lo, int32_t K_h
  * 27-62): the MPS file at path (free format, then fixed format if the free
  * parse fails) as the reference's RangedLP.  Returns a handle that holds the
  * LP; on a parse error, SUPR_EINVAL with "MPS parse failed: ..." (the text of
- * the reference's std::runtime_error). */
+ * the reference's std::runtime_error).  The COLUMNS body of a free-format
+ * file is parsed on several threads (the hardware's, at most 16, at least
+ * 1 MiB of body each; the environment variable SUPR_MPS_THREADS sets the
+ * count, 1 for the one-thread parse, and SUPR_MPS_MIN_CHUNK the bytes per
+ * thread) with the one-thread parse's result; markers, rows not declared
+ * before COLUMNS and every error take the one-thread path. */
 typedef struct supr_mps supr_mps;
 int supr_mps_read(const char* path, supr_mps** out);
 /* m, n, nnz of A; q_nnz of the QP's lower triangle (0 for an LP); maximize

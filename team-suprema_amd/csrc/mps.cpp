@@ -17,13 +17,20 @@
 //  * explicit zero coefficients and entries of undeclared rows are dropped;
 //    columns are numbered by first appearance; a row or name declared twice
 //    keeps the reference's map semantics (the last declaration wins).
-// It is one pass over the file held in memory with names as views into it
-// and one hash lookup per name, instead of a stream and a string map per
-// token.  Numbers with at most 19 significant digits and a decimal exponent
+// It is one pass over the file held in memory (read with one call) with
+// names as views into it and one hash lookup per name, instead of a stream
+// and a string map per token; the COLUMNS body of a free-format file, most
+// of a large file, is split over threads (ParallelColumns), and the output
+// arrays are filled on threads too.  A line of only vertical tabs or form
+// feeds in COLUMNS is an error here ("empty COLUMNS line"); the reference
+// indexes its missing first token there (src/mps_reader.cpp:175).  Numbers with at most 19 significant digits and a decimal exponent
 // within +-22 convert by one exact integer-to-double conversion and one
 // correctly rounded multiply or divide by an exact power of ten, which is the
 // correctly rounded value strtod also returns; anything else goes to strtod.
+#include <algorithm>
+#include <atomic>
 #include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -34,8 +41,11 @@
 #include <stdexcept>
 #include <string>
 #include <string_view>
+#include <thread>
 #include <unordered_map>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "host_lp.hpp"
 
@@ -135,6 +145,50 @@ std::string_view trim(std::string_view s) {
     return s.substr(a, b - a);
 }
 
+// The line at p (p moves past its '\n'), a trailing '\r' removed.
+std::string_view take_line(const char*& p, const char* end) {
+    const char* nl = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+    const char* le = nl ? nl : end;
+    std::string_view line(p, (size_t)(le - p));
+    p = nl ? nl + 1 : end;
+    if (!line.empty() && line.back() == '\r') line.remove_suffix(1);
+    return line;
+}
+
+// Blank (spaces and tabs only) or a comment ('*' as the first other character).
+bool skip_line(std::string_view line) {
+    size_t first = 0;
+    while (first < line.size() && (line[first] == ' ' || line[first] == '\t')) ++first;
+    return first == line.size() || line[first] == '*';
+}
+
+void split(std::string_view line, std::vector<std::string_view>& toks) {
+    toks.clear();
+    for (size_t i = 0; i < line.size();) {
+        while (i < line.size() && is_ws(line[i])) ++i;
+        if (i == line.size()) break;
+        size_t j = i;
+        while (j < line.size() && !is_ws(line[j])) ++j;
+        toks.push_back(line.substr(i, j - i));
+        i = j;
+    }
+}
+
+constexpr int kEndata = 7;
+const char* const kSections[] = {"NAME", "OBJSENSE", "ROWS", "COLUMNS", "RHS", "RANGES",
+                                 "BOUNDS", "ENDATA", "QUADOBJ", "QMATRIX", "QSECTION"};
+
+// A section line: a keyword as the first token, in column 1 (ENDATA anywhere).
+// Returns the keyword's index in kSections, or -1 for a data line.
+int section_keyword(std::string_view line, const std::vector<std::string_view>& toks) {
+    if (toks.empty()) return -1;
+    int kw = -1;
+    for (int k = 0; k < 11; ++k)
+        if (toks[0] == kSections[k]) { kw = k; break; }
+    const bool col1 = line[0] != ' ' && line[0] != '\t';
+    return kw >= 0 && (col1 || kw == kEndata) ? kw : -1;
+}
+
 bool unquoted_eq(std::string_view t, const char* word) {
     size_t k = 0;
     const size_t wl = std::strlen(word);
@@ -144,6 +198,45 @@ bool unquoted_eq(std::string_view t, const char* word) {
         ++k;
     }
     return k == wl;
+}
+
+// Runs f(0) .. f(T-1) on T threads (f(0) on the caller's); false if any threw.
+template <class F>
+bool parallel_for(int T, F&& f) {
+    std::atomic<bool> ok{true};
+    auto body = [&](int t) {
+        try { f(t); } catch (...) { ok.store(false); }
+    };
+    std::vector<std::thread> th;
+    try {
+        th.reserve((size_t)T - 1);
+        for (int t = 1; t < T; ++t) th.emplace_back(body, t);
+    } catch (...) {
+        ok.store(false);
+    }
+    body(0);
+    for (auto& x : th) x.join();
+    return ok.load();
+}
+
+// Threads for `work` items, each thread taking at least `unit` of them: the
+// hardware's threads (at most 16), or SUPR_MPS_THREADS; SUPR_MPS_MIN_CHUNK
+// replaces unit (small values put small files on the parallel paths).
+int worker_count(size_t work, size_t unit) {
+    long T = (long)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (const char* e = std::getenv("SUPR_MPS_THREADS")) T = std::max(1L, std::strtol(e, nullptr, 10));
+    if (const char* e = std::getenv("SUPR_MPS_MIN_CHUNK")) unit = (size_t)std::max(1L, std::strtol(e, nullptr, 10));
+    return (int)std::max(1L, std::min(T, (long)(work / unit)));
+}
+
+// f(t, lo, hi) for T contiguous ranges of [0, n): on T threads, or one after
+// another on the caller's thread when T < 2 or a thread fails (so an
+// exception surfaces here; f must give the same result when run again).
+template <class F>
+void for_ranges(size_t n, int T, F&& f) {
+    auto lo = [&](int t) { return n * (size_t)t / (size_t)T; };
+    if (T >= 2 && parallel_for(T, [&](int t) { f(t, lo(t), lo(t + 1)); })) return;
+    for (int t = 0; t < T; ++t) f(t, lo(t), lo(t + 1));
 }
 
 // Names interned to dense ids (views into the file buffer or static text):
@@ -204,6 +297,227 @@ struct Names {
         }
         return -1;
     }
+    // The table of nm (distinct names, id = position; h their hashes), filled
+    // on T threads: each claims a free slot by compare-and-swap of its id and
+    // then writes the hash.  Slot order depends on the threads' timing, ids
+    // and lookups do not.  False if a thread could not run.
+    bool build(std::vector<std::string_view>&& nm, const std::vector<uint64_t>& h, int T) {
+        name = std::move(nm);
+        size_t cap = 1024;
+        while (2 * (name.size() + 1) > cap) cap *= 2;
+        slots.assign(cap, Slot{0, -1});
+        mask = cap - 1;
+        const size_t n = name.size();
+        return parallel_for(T, [&](int t) {
+                const size_t g1 = n * (size_t)(t + 1) / (size_t)T;
+                for (size_t g = n * (size_t)t / (size_t)T; g < g1; ++g) {
+                    if (g + 16 < g1) __builtin_prefetch(&slots[h[g + 16] & mask], 1);
+                    size_t q = h[g] & mask;
+                    for (;; q = (q + 1) & mask) {
+                        int64_t none = -1;
+                        if (__atomic_compare_exchange_n(&slots[q].id, &none, (int64_t)g, false,
+                                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+                            break;
+                    }
+                    slots[q].hash = h[g];
+                }
+            });
+    }
+};
+
+// One coefficient of A as read: row-name id, column id, value.
+struct Entry { int32_t row, col; double val; };
+
+// The body of a free-format COLUMNS section parsed on several threads, with
+// the result of the one-thread parse: the same column numbering (first
+// appearance), the same entries in file order, the objective summed in file
+// order.  The body is cut at line boundaries into one chunk per thread; each
+// thread reads its chunk as COLUMNS lines until a section line, splitting
+// it into runs (consecutive lines of one column name), and gives up at
+// anything the one-thread parse must see in order: an INTORG / INTEND
+// marker, a row name not yet declared, a number strtod rejects, an empty
+// line.  The first chunk that meets a section line ends the body (later
+// chunks read past it and are dropped); if a chunk gives up before that,
+// run() returns false, nothing is kept, and the caller parses the body
+// itself, so every error and every unusual file takes the one-thread path.
+// Runs are then numbered across chunks through hash-partitioned tables (a
+// name's first run gets the next id, later runs of it share that id) and the
+// column table is built from the ids on all threads.
+struct ParallelColumns {
+    Names cols;
+    std::vector<double> obj;
+    std::vector<Entry> entries;
+    const char* resume = nullptr;
+
+    struct Chunk {
+        const char* begin = nullptr;
+        const char* end = nullptr;
+        bool stopped = false, gave_up = false;
+        const char* stop = nullptr;            // the section line that ended the body
+        std::vector<std::string_view> run_name;
+        std::vector<uint64_t> run_hash;
+        struct Coef { int32_t row, run; double val; };
+        std::vector<Coef> coef;
+        struct Cost { int32_t run; double val; };
+        std::vector<Cost> cost;
+        std::vector<std::vector<int32_t>> part;   // per name partition: this chunk's runs in it
+        std::vector<int64_t> first;            // per run: -1, or (chunk << 32 | run) of the name's first run
+        std::vector<int32_t> id;               // per run: column id
+        int32_t fresh = 0;                     // runs here that are a name's first
+    };
+
+    // The partition of a name hash (the table inside a partition uses the low bits).
+    static int part_of(uint64_t h, int Q) { return (int)((h >> 40) % (uint64_t)Q); }
+
+    static void scan(Chunk& C, int index, std::atomic<int>& first_end, const Names& rows,
+                     const std::vector<char>& rfree, bool has_obj, std::string_view obj_name) {
+        auto finish = [&](bool stopped) {
+            (stopped ? C.stopped : C.gave_up) = true;
+            int cur = first_end.load();
+            while (index < cur && !first_end.compare_exchange_weak(cur, index)) {}
+        };
+        std::vector<std::string_view> toks;
+        toks.reserve(16);
+        std::string_view run;
+        const char* p = C.begin;
+        size_t lines = 0;
+        try {
+            while (p < C.end) {
+                // a chunk before this one has ended the body: this one is dropped
+                if ((++lines & 1023) == 0 && first_end.load(std::memory_order_relaxed) < index) return;
+                const char* at = p;
+                const std::string_view line = take_line(p, C.end);
+                if (skip_line(line)) continue;
+                split(line, toks);
+                if (section_keyword(line, toks) >= 0) { C.stop = at; finish(true); return; }
+                if (toks.empty()) { finish(false); return; }
+                for (auto& x : toks)
+                    if (unquoted_eq(x, "MARKER")) { finish(false); return; }
+                if (C.run_name.empty() || toks[0] != run) {
+                    run = toks[0];
+                    const uint64_t h = Names::hash(run);
+                    C.part[part_of(h, (int)C.part.size())].push_back((int32_t)C.run_name.size());
+                    C.run_name.push_back(run);
+                    C.run_hash.push_back(h);
+                }
+                const int32_t r = (int32_t)C.run_name.size() - 1;
+                for (size_t i = 1; i + 1 < toks.size(); i += 2) {
+                    const double v = parse_double(toks[i + 1]);
+                    if (has_obj && toks[i] == obj_name) { C.cost.push_back({r, v}); continue; }
+                    const int32_t k = rows.find(toks[i]);
+                    if (k < 0) { finish(false); return; }
+                    if (rfree[k]) continue;
+                    C.coef.push_back({k, r, v});
+                }
+            }
+        } catch (...) {
+            finish(false);
+        }
+    }
+
+    // p: the line after the COLUMNS line.  False: nothing parsed, the caller
+    // reads the body itself.
+    bool run(const char* p, const char* end, const Names& rows, const std::vector<char>& rfree,
+             bool has_obj, std::string_view obj_name) {
+        const int T = worker_count((size_t)(end - p), 1 << 20);   // at least 1 MiB of body each
+        if (T < 2) return false;
+        std::vector<Chunk> ch((size_t)T);
+        const char* at = p;
+        for (int t = 0; t < T; ++t) {
+            ch[t].begin = at;
+            const char* cut = t + 1 == T ? end : p + (size_t)(end - p) * (size_t)(t + 1) / (size_t)T;
+            if (cut < at) cut = at;
+            if (cut < end) {
+                const char* nl = static_cast<const char*>(std::memchr(cut, '\n', (size_t)(end - cut)));
+                cut = nl ? nl + 1 : end;
+            }
+            ch[t].end = at = cut;
+        }
+        for (auto& c : ch) c.part.resize((size_t)T);
+        std::atomic<int> first_end{INT_MAX};
+        if (!parallel_for(T, [&](int t) { scan(ch[t], t, first_end, rows, rfree, has_obj, obj_name); }))
+            return false;
+        // chunks 0 .. last-1 are whole COLUMNS lines; last ends the body or gave up
+        int last = first_end.load();
+        if (last == INT_MAX) last = T - 1;
+        if (ch[last].gave_up) return false;
+        resume = ch[last].stopped ? ch[last].stop : end;
+        const int nc = last + 1;
+
+        // number the runs: partition q owns the names whose hash's top bits say q
+        size_t runs = 0;
+        for (int c = 0; c < nc; ++c) {
+            ch[c].first.assign(ch[c].run_name.size(), -1);
+            runs += ch[c].run_name.size();
+        }
+        if (!parallel_for(T, [&](int q) {
+                size_t cnt = 0;
+                for (int c = 0; c < nc; ++c) cnt += ch[c].part[q].size();
+                size_t cap = 64;
+                while (cap < 2 * cnt + 2) cap *= 2;
+                struct Seen { uint64_t hash; int64_t at; };   // at -1: empty
+                std::vector<Seen> tab(cap, Seen{0, -1});
+                for (int c = 0; c < nc; ++c) {
+                    const Chunk& C = ch[c];
+                    for (const int32_t r : C.part[q]) {
+                        const uint64_t h = C.run_hash[r];
+                        size_t s = h & (cap - 1);
+                        for (;; s = (s + 1) & (cap - 1)) {
+                            if (tab[s].at < 0) { tab[s] = Seen{h, ((int64_t)c << 32) | (int64_t)r}; break; }
+                            const int64_t a = tab[s].at;
+                            if (tab[s].hash == h && ch[a >> 32].run_name[a & 0xffffffff] == C.run_name[r]) {
+                                ch[c].first[r] = a;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }))
+            return false;
+        std::vector<int32_t> base((size_t)nc + 1, 0);
+        for (int c = 0; c < nc; ++c) {
+            int32_t f = 0;
+            for (int64_t a : ch[c].first) f += a < 0;
+            ch[c].fresh = f;
+            base[c + 1] = base[c] + f;
+        }
+        const int32_t n = base[nc];
+        std::vector<std::string_view> name((size_t)n);
+        std::vector<uint64_t> hash((size_t)n);
+        if (!parallel_for(nc, [&](int c) {
+                Chunk& C = ch[c];
+                C.id.resize(C.first.size());
+                int32_t g = base[c];
+                for (size_t r = 0; r < C.first.size(); ++r)
+                    if (C.first[r] < 0) {
+                        C.id[r] = g;
+                        name[g] = C.run_name[r];
+                        hash[g] = C.run_hash[r];
+                        ++g;
+                    }
+            }))
+            return false;
+        if (!parallel_for(nc, [&](int c) {
+                Chunk& C = ch[c];
+                for (size_t r = 0; r < C.first.size(); ++r)
+                    if (C.first[r] >= 0) C.id[r] = ch[C.first[r] >> 32].id[C.first[r] & 0xffffffff];
+            }))
+            return false;
+        if (!cols.build(std::move(name), hash, T)) return false;
+
+        obj.assign((size_t)n, 0.0);
+        for (int c = 0; c < nc; ++c)   // file order, as the one-thread parse adds
+            for (const auto& e : ch[c].cost) obj[ch[c].id[e.run]] += e.val;
+        std::vector<size_t> off((size_t)nc + 1, 0);
+        for (int c = 0; c < nc; ++c) off[c + 1] = off[c] + ch[c].coef.size();
+        entries.resize(off[nc]);
+        return parallel_for(nc, [&](int c) {
+            const Chunk& C = ch[c];
+            Entry* out = entries.data() + off[c];
+            for (size_t i = 0; i < C.coef.size(); ++i)
+                out[i] = Entry{C.coef[i].row, C.id[C.coef[i].run], C.coef[i].val};
+        });
+    }
 };
 
 struct Parsed {
@@ -224,7 +538,7 @@ enum Section { S_NONE, S_OBJSENSE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES, S_BOUNDS,
 [[noreturn]] void fail(const std::string& msg) { throw std::runtime_error(msg); }
 
 // One parse of the whole text (src/mps_reader.cpp:65-307 + to_ranged_lp).
-void parse(const std::string& text, bool fixed_format, Parsed& out) {
+void parse(std::string_view text, bool fixed_format, Parsed& out) {
     std::string sense = "min";
     std::string_view pname;
     bool have_name = false;
@@ -261,7 +575,6 @@ void parse(const std::string& text, bool fixed_format, Parsed& out) {
         return k;
     };
 
-    struct Entry { int32_t row, col; double val; };
     std::vector<Entry> entries;
     std::vector<double> rhs_v, rng_v;   // per row-name id
     std::vector<char> rhs_has, rng_has;
@@ -269,48 +582,41 @@ void parse(const std::string& text, bool fixed_format, Parsed& out) {
     struct QEntry { std::string_view a, b; double v; bool full; };
     std::vector<QEntry> quad;
 
-    static const char* const kSections[] = {"NAME", "OBJSENSE", "ROWS", "COLUMNS", "RHS", "RANGES",
-                                            "BOUNDS", "ENDATA", "QUADOBJ", "QMATRIX", "QSECTION"};
     std::vector<std::string_view> toks, tk;
     toks.reserve(16);
     tk.reserve(16);
     const char* p = text.data();
     const char* const end = p + text.size();
     while (p < end) {
-        const char* nl = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
-        const char* le = nl ? nl : end;
-        std::string_view line(p, (size_t)(le - p));
-        p = nl ? nl + 1 : end;
-        if (!line.empty() && line.back() == '\r') line.remove_suffix(1);
-        size_t first = 0;
-        while (first < line.size() && (line[first] == ' ' || line[first] == '\t')) ++first;
-        if (first == line.size()) continue;   // blank
-        if (line[first] == '*') continue;     // comment
-        toks.clear();
-        for (size_t i = 0; i < line.size();) {
-            while (i < line.size() && is_ws(line[i])) ++i;
-            if (i == line.size()) break;
-            size_t j = i;
-            while (j < line.size() && !is_ws(line[j])) ++j;
-            toks.push_back(line.substr(i, j - i));
-            i = j;
-        }
-        const std::string_view first_tok = toks.empty() ? std::string_view() : toks[0];
-        const bool col1 = line[0] != ' ' && line[0] != '\t';
-        int kw = -1;
-        for (int k = 0; k < 11; ++k)
-            if (first_tok == kSections[k]) { kw = k; break; }
-        if (kw >= 0 && (col1 || kw == 7)) {
+        const std::string_view line = take_line(p, end);
+        if (skip_line(line)) continue;
+        split(line, toks);
+        const int kw = section_keyword(line, toks);
+        if (kw >= 0) {
             if (kw == 0) {   // NAME
                 have_name = true;
                 pname = toks.size() > 1 ? toks[1] : std::string_view("UNNAMED");
                 section = S_NONE;
                 continue;
             }
-            if (kw == 7) break;   // ENDATA
+            if (kw == kEndata) break;
             static const Section map[] = {S_NONE, S_OBJSENSE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES,
                                           S_BOUNDS, S_NONE, S_QUAD, S_QMATRIX, S_QUAD};
             section = map[kw];
+            // the first COLUMNS section of a free-format file, before any
+            // column is known: its body is parsed on several threads
+            if (section == S_COLUMNS && !fixed_format && cols.name.empty() && !int_marker) {
+                ParallelColumns pc;
+                if (pc.run(p, end, rows, rfree, obj_id >= 0, obj_name)) {
+                    cols = std::move(pc.cols);
+                    const size_t n = cols.name.size();
+                    obj = std::move(pc.obj);
+                    integ.assign(n, 0); has_lo.assign(n, 0); has_hi.assign(n, 0);
+                    explicit_lo.assign(n, 0); blo.assign(n, 0.0); bhi.assign(n, 0.0);
+                    entries = std::move(pc.entries);
+                    p = pc.resume;   // the section line that ended the body, or the end
+                }
+            }
             continue;
         }
         const std::vector<std::string_view>* tp = &toks;
@@ -441,15 +747,27 @@ void parse(const std::string& text, bool fixed_format, Parsed& out) {
     const int32_t m = (int32_t)row_order.size(), n = (int32_t)cols.name.size();
     std::vector<int32_t> pos(rows.name.size(), -1);
     for (int32_t i = 0; i < m; ++i) pos[row_order[i]] = i;
-    out.row.clear(); out.col.clear(); out.val.clear();
-    out.row.reserve(entries.size()); out.col.reserve(entries.size()); out.val.reserve(entries.size());
-    for (const Entry& en : entries) {
-        if (en.val == 0.0) continue;
-        const int32_t i = pos[en.row];
-        if (i < 0) continue;   // a row never declared in ROWS
-        out.row.push_back(i);
-        out.col.push_back(en.col);
-        out.val.push_back(en.val);
+    // explicit zeros and rows never declared in ROWS are dropped, order kept
+    {
+        const int T = worker_count(entries.size(), 1 << 17);
+        std::vector<size_t> kept((size_t)T + 1, 0);
+        for_ranges(entries.size(), T, [&](int t, size_t lo, size_t hi) {
+            size_t k = 0;
+            for (size_t e = lo; e < hi; ++e) k += entries[e].val != 0.0 && pos[entries[e].row] >= 0;
+            kept[t + 1] = k;
+        });
+        for (int t = 0; t < T; ++t) kept[t + 1] += kept[t];
+        out.row.resize(kept[T]); out.col.resize(kept[T]); out.val.resize(kept[T]);
+        for_ranges(entries.size(), T, [&](int t, size_t lo, size_t hi) {
+            size_t k = kept[t];
+            for (size_t e = lo; e < hi; ++e) {
+                const Entry& en = entries[e];
+                const int32_t i = pos[en.row];
+                if (en.val == 0.0 || i < 0) continue;
+                out.row[k] = i; out.col[k] = en.col; out.val[k] = en.val;
+                ++k;
+            }
+        });
     }
     // to_ranged_lp (include/ranged_lp.hpp:27-62)
     out.maximize = sense == "max";
@@ -502,20 +820,47 @@ void parse(const std::string& text, bool fixed_format, Parsed& out) {
     out.name = have_name && !pname.empty() ? std::string(pname) : std::string("UNNAMED");
     out.obj_name = std::string(obj_name);
     out.col_names.resize(n);
-    for (int32_t j = 0; j < n; ++j) out.col_names[j] = std::string(cols.name[j]);
+    for_ranges((size_t)n, worker_count((size_t)n, 1 << 17), [&](int, size_t lo, size_t hi) {
+        for (size_t j = lo; j < hi; ++j) out.col_names[j].assign(cols.name[j].data(), cols.name[j].size());
+    });
     out.row_names.resize(m);
     for (int32_t i = 0; i < m; ++i) out.row_names[i] = std::string(rows.name[row_order[i]]);
 }
 
-std::string read_file(const char* path) {
+// The whole file in memory: one read of its size for a regular file, else
+// read in blocks (a pipe).  A read error ends the text where it stopped, as
+// the stream the reference reads from ends.
+struct FileText {
+    std::unique_ptr<char[]> buf;
+    size_t size = 0;
+    std::string_view view() const { return std::string_view(buf.get(), size); }
+};
+
+FileText read_file(const char* path) {
     std::FILE* f = std::fopen(path, "rb");
     if (!f) fail(std::string("cannot open MPS file: ") + path);
-    std::string s;
-    char buf[1 << 16];
-    size_t k;
-    while ((k = std::fread(buf, 1, sizeof(buf), f)) > 0) s.append(buf, k);
+    struct stat st;
+    size_t cap = 1 << 16;
+    if (fstat(fileno(f), &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0)
+        cap = (size_t)st.st_size + 1;   // one byte more: the end shows as a short read
+    FileText t;
+    try {
+        t.buf.reset(new char[cap]);
+        for (;;) {
+            const size_t k = std::fread(t.buf.get() + t.size, 1, cap - t.size, f);
+            t.size += k;
+            if (t.size < cap) break;   // end of file or a read error
+            std::unique_ptr<char[]> nb(new char[2 * cap]);
+            std::memcpy(nb.get(), t.buf.get(), t.size);
+            t.buf = std::move(nb);
+            cap *= 2;
+        }
+    } catch (...) {
+        std::fclose(f);
+        throw;
+    }
     std::fclose(f);
-    return s;
+    return t;
 }
 
 }  // namespace
@@ -536,7 +881,8 @@ int supr_mps_read(const char* path, supr_mps** out) {
         auto h = std::make_unique<supr_mps>();
         std::string free_err;
         try {
-            const std::string text = read_file(path);
+            const FileText file = read_file(path);
+            const std::string_view text = file.view();
             try {
                 parse(text, false, h->p);
             } catch (const std::exception& e) {

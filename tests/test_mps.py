@@ -367,3 +367,145 @@ def test_differential_against_the_reference_reader(tmp_path):
             parsed += 1
         agreed += 1
     assert agreed == len(texts) and parsed > 200
+
+
+# ------------------------------------- the threaded COLUMNS pass (free format)
+
+def read_threads(monkeypatch, path, threads, min_chunk=1):
+    """read_mps with the COLUMNS body cut into `threads` chunks of at least
+    min_chunk bytes (threads=1: the one-thread parse); the LP or the error."""
+    monkeypatch.setenv("SUPR_MPS_THREADS", str(threads))
+    monkeypatch.setenv("SUPR_MPS_MIN_CHUNK", str(min_chunk))
+    try:
+        return P.read_mps(path)
+    except P.SupremaError as e:
+        return str(e)
+    finally:
+        monkeypatch.delenv("SUPR_MPS_THREADS")
+        monkeypatch.delenv("SUPR_MPS_MIN_CHUNK")
+
+
+def same_parse(a, b):
+    if isinstance(a, str) or isinstance(b, str):
+        return a == b
+    return (all(bits(getattr(a.lp, k), getattr(b.lp, k)) for k in ARRAYS)
+            and bits(np.float64(a.lp.obj_offset), np.float64(b.lp.obj_offset))
+            and bits(a.integer, b.integer) and bits(a.q_row, b.q_row) and bits(a.q_col, b.q_col)
+            and bits(a.q_val, b.q_val) and a.maximize == b.maximize and a.col_names == b.col_names
+            and a.row_names == b.row_names and a.obj_name == b.obj_name and a.name == b.name)
+
+
+@pytest.mark.parametrize("threads", [3, 8])
+def test_threaded_columns_on_the_fixtures(tmp_path, monkeypatch, threads):
+    """Every fixture, its COLUMNS body cut into many chunks, still gives the
+    reference reader's LP bit for bit (files with markers or in fixed format
+    take the one-thread path inside the reader)."""
+    for name in FIXTURES:
+        with gzip.open(os.path.join(MPS_DIR, name + ".mps.gz"), "rb") as f:
+            path = write(tmp_path, name + ".mps", f.read().decode("latin-1"))
+        d = dict(np.load(os.path.join(MPS_DIR, name + ".npz")))
+        mp = read_threads(monkeypatch, path, threads)
+        assert not isinstance(mp, str), (name, mp)
+        same_as_dict(mp, d)
+
+
+EDGE_HEAD = "NAME  EDGE\nROWS\n N  obj\n L  r1\n G  r2\n E  r3\n N  free\n L  r1\nCOLUMNS\n"
+EDGE_TAIL = "RHS\n    RHS  r1  4  r2  1\n    RHS  obj  2.5\nBOUNDS\n UP BND  a  3\n LO BND  z  -1\nENDATA\n"
+EDGE_BODIES = {
+    # a column's lines far apart, its objective split over them (summed in file order)
+    "repeats": "    a  obj  0.1  r1  1\n    b  r2  2\n    c  r3  3\n    a  obj  0.2  r2  4\n"
+               "    b  obj  1e-17\n    a  obj  0.3\n    d  r1  -1\n    b  r3  5\n",
+    "crlf_comments_blanks": "    a  r1  1\r\n* comment\r\n\r\n   \t \n    b  obj  2\r\n    a  r2  3\r\n",
+    "odd_tokens_and_zeros": "    a  r1  1  r2\n    b  r2  0  r3  7\n    c  free  9  r1  0.0\n",
+    "keyword_names_not_in_column_1": "    RHS  r1  1\n    BOUNDS  r2  2\n    a  r3  3\n",
+    "endata_indented": "    a  r1  1\n    b  r2  2\n    ENDATA\n    c  r3  3\n",
+    "name_line_inside": "    a  r1  1\nNAME  SECOND\n    b  r2  2\n",
+    "second_columns_section": "    a  r1  1\n    b  r2  2\nRHS\n    RHS  r1  7\nCOLUMNS\n    b  r3  3\n    c  r1  4\n",
+    "undeclared_row": "    a  r1  1\n    b  nowhere  2\n    c  r2  3\n",
+    "marker_late": "    a  r1  1\n    b  r2  2\n    MARKER  'MARKER'  'INTORG'\n    c  r3  3\n"
+                   "    MARKER  'MARKER'  'INTEND'\n    d  r1  4\n",
+    "bad_number_late": "    a  r1  1\n    b  r2  2\n    c  r3  x7\n",
+    "trailing_text_number": "    a  r1  1.5abc\n    b  r2  0x10\n",
+    "out_of_range_number": "    a  r1  1\n    b  r2  1e999\n",
+    "vertical_tab_line": "    a  r1  1\n\v\n    b  r2  2\n",
+    "no_endata": None,
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_BODIES))
+def test_threaded_columns_edge_cases(tmp_path, monkeypatch, case):
+    """Each case gives the same LP, names and errors on 2 .. 9 chunks as on
+    one thread (cuts land inside a column's run, on comments and on the
+    section line), and, where oracle/_ref is built, the reference reader's."""
+    body = EDGE_BODIES[case]
+    text = EDGE_HEAD + (body if body is not None else "    a  r1  1\n    b  r2  2\n    a  r3  3") \
+        + (EDGE_TAIL if body is not None else "")
+    path = write(tmp_path, case + ".mps", text)
+    one = read_threads(monkeypatch, path, 1)
+    for threads in range(2, 10):
+        assert same_parse(read_threads(monkeypatch, path, threads), one), (case, threads)
+    # the reference reader indexes the first token of a line that has none
+    # (src/mps_reader.cpp:175) and crashes on the vertical-tab line: compared
+    # with the one-thread parse only
+    if available("ref") and case != "vertical_tab_line":
+        from oracle import read_mps_ref, read_mps_ref_error
+        err = read_mps_ref_error(path)
+        if err:
+            assert isinstance(one, str) and err in one
+        else:
+            same_as_dict(one, read_mps_ref(path, full=True))
+
+
+def random_big_mps(rng):
+    """A free-format file with a few thousand COLUMNS lines: columns in
+    shuffled runs (a name comes back after others), objective entries in
+    several runs of one column, free and twice-declared rows, explicit zeros,
+    comments, and now and then a marker or a stray row (both send the
+    reader to its one-thread path)."""
+    rows = [f"R{i}" for i in range(rng.randint(5, 40))]
+    kinds = [rng.choice("LGE") for _ in rows]
+    out = ["NAME  BIG", "ROWS", " N  COST"] + [f" {k}  {r}" for k, r in zip(kinds, rows)]
+    out += [" N  FREE", f" G  {rows[0]}", "COLUMNS"]
+    ncols = rng.randint(50, 400)
+    lines = []
+    for _ in range(rng.randint(1500, 4000)):
+        j = rng.randrange(ncols)
+        pairs = []
+        for _ in range(rng.choice((1, 2))):
+            r = rng.random()
+            tgt = "COST" if r < 0.25 else "FREE" if r < 0.3 else rng.choice(rows)
+            v = "0" if rng.random() < 0.05 else f"{rng.uniform(-50, 50):.{rng.randint(1, 12)}g}"
+            pairs += [tgt, v]
+        lines.append(f"    C{j}  " + "  ".join(pairs))
+        if rng.random() < 0.01:
+            lines.append("* a comment")
+    if rng.random() < 0.15:
+        lines.insert(rng.randrange(len(lines)), "    MARKER  'MARKER'  'INTORG'")
+    if rng.random() < 0.15:
+        lines.insert(rng.randrange(len(lines)), "    C1  STRAY  1")
+    out += lines + ["RHS"] + [f"    RHS  {r}  {rng.randint(-5, 5)}" for r in rows[:5]]
+    out += ["BOUNDS", " UP BND  C0  4", " FR BND  C3", "ENDATA"]
+    return "\n".join(out) + "\n"
+
+
+def test_threaded_columns_random_files(tmp_path, monkeypatch):
+    """Random files of a few thousand lines: 2 .. 16 chunks give the
+    one-thread parse bit for bit."""
+    rng = random.Random(5)
+    for k in range(40):
+        path = write(tmp_path, f"big{k}.mps", random_big_mps(rng))
+        one = read_threads(monkeypatch, path, 1)
+        assert not isinstance(one, str), one
+        for threads in (2, 5, 16):
+            assert same_parse(read_threads(monkeypatch, path, threads, min_chunk=64), one), (k, threads)
+
+
+def test_file_edge_cases(tmp_path):
+    """An empty file, a directory and a file whose last line has no newline
+    read as the reference's stream reads them."""
+    with pytest.raises(P.SupremaError, match="No objective"):
+        P.read_mps(write(tmp_path, "empty.mps", ""))
+    with pytest.raises(P.SupremaError, match="No objective"):
+        P.read_mps(str(tmp_path))
+    mp = P.read_mps(write(tmp_path, "nonl.mps", "ROWS\n N  o\n L  r\nCOLUMNS\n    x  o  1  r  2"))
+    assert (mp.lp.m, mp.lp.n, mp.lp.nnz) == (1, 1, 1) and mp.lp.c[0] == 1.0
