@@ -392,6 +392,12 @@ int supr_mps_get(const supr_mps* h, int32_t* row_idx, int32_t* col_idx, double* 
 /* Names held by the handle: kind 0 the problem (NAME), 1 column index, 2 row
  * index, 3 the objective row; NULL when out of range. */
 const char* supr_mps_name(const supr_mps* h, int32_t kind, int32_t index);
+/* All column (kind 1) or row (kind 2) names at once, each followed by a NUL,
+ * in index order: *bytes gets the total size; with buf == NULL nothing is
+ * copied (ask for the size first), else buf must hold cap >= *bytes bytes.
+ * (No reference counterpart: the reference's RangedLP carries the names;
+ * this fills a caller's name table with one call instead of one per name.) */
+int supr_mps_names(const supr_mps* h, int32_t kind, char* buf, int64_t cap, int64_t* bytes);
 void supr_mps_free(supr_mps* h);
 /* Writes lp as a free-format MPS file that supr_mps_read (and the
  * reference's read_mps) reads back to the same LP: row kinds E / L / G from

@@ -951,6 +951,30 @@ const char* supr_mps_name(const supr_mps* h, int32_t kind, int32_t index) {
     }
 }
 
+int supr_mps_names(const supr_mps* h, int32_t kind, char* buf, int64_t cap, int64_t* bytes) {
+    if (!h || !bytes || (kind != 1 && kind != 2)) {
+        supr::set_error("supr_mps_names: null handle / bytes, or kind not 1 (columns) or 2 (rows)");
+        return SUPR_EINVAL;
+    }
+    const std::vector<std::string>& v = kind == 1 ? h->p.col_names : h->p.row_names;
+    int64_t total = 0;
+    for (const std::string& x : v) total += (int64_t)x.size() + 1;
+    *bytes = total;
+    if (!buf) return SUPR_OK;
+    if (cap < total) {
+        supr::set_error("supr_mps_names: buffer of " + std::to_string(cap) + " bytes, " +
+                        std::to_string(total) + " needed");
+        return SUPR_EINVAL;
+    }
+    char* o = buf;
+    for (const std::string& x : v) {
+        std::memcpy(o, x.data(), x.size());
+        o += x.size();
+        *o++ = '\0';
+    }
+    return SUPR_OK;
+}
+
 void supr_mps_free(supr_mps* h) { delete h; }
 
 int supr_mps_write(const char* path, const supr_lp* lp, const char* name,

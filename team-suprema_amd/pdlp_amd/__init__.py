@@ -611,6 +611,17 @@ class MpsProblem:
         return self.lp.name
 
 
+def _mps_names(h, kind):
+    """Every column (kind 1) or row (kind 2) name of an MPS handle, one call."""
+    nb = ctypes.c_int64()
+    check(lib().supr_mps_names(h, kind, None, 0, ctypes.byref(nb)))
+    if nb.value == 0:
+        return []
+    buf = ctypes.create_string_buffer(nb.value)
+    check(lib().supr_mps_names(h, kind, buf, nb.value, ctypes.byref(nb)))
+    return buf.raw[:nb.value - 1].decode().split("\0")
+
+
 def read_mps(path, names=True):
     """Parses an MPS file as the reference's read_mps + to_ranged_lp do.
     names=False skips building the Python name lists (large files)."""
@@ -628,8 +639,8 @@ def read_mps(path, names=True):
         check(lib().supr_mps_get(h, _ip(row), _ip(col), _dp(val), _dp(c), _dp(rL), _dp(rU), _dp(l),
                                  _dp(u), ctypes.byref(off), integ, _ip(qr), _ip(qc), _dp(qv)))
         nm = lib().supr_mps_name(h, 0, 0).decode()
-        cols = [lib().supr_mps_name(h, 1, j).decode() for j in range(n)] if names else []
-        rows = [lib().supr_mps_name(h, 2, i).decode() for i in range(m)] if names else []
+        cols = _mps_names(h, 1) if names else []
+        rows = _mps_names(h, 2) if names else []
         obj = lib().supr_mps_name(h, 3, 0).decode()
     finally:
         lib().supr_mps_free(h)

@@ -152,6 +152,8 @@ SIGNATURES = [
      [ctypes.c_void_p, c_int32_p, c_int32_p] + [c_double_p] * 7 + [ctypes.c_char_p, c_int32_p,
                                                                     c_int32_p, c_double_p]),
     ("supr_mps_name", ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("supr_mps_names", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64,
+                                       ctypes.POINTER(ctypes.c_int64)]),
     ("supr_mps_free", None, [ctypes.c_void_p]),
     ("supr_mps_write", ctypes.c_int,
      [ctypes.c_char_p, _LP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),

@@ -509,3 +509,31 @@ def test_file_edge_cases(tmp_path):
         P.read_mps(str(tmp_path))
     mp = P.read_mps(write(tmp_path, "nonl.mps", "ROWS\n N  o\n L  r\nCOLUMNS\n    x  o  1  r  2"))
     assert (mp.lp.m, mp.lp.n, mp.lp.nnz) == (1, 1, 1) and mp.lp.c[0] == 1.0
+
+
+def test_bulk_names_match_the_per_index_names(tmp_path):
+    """supr_mps_names (one call per kind) gives the names supr_mps_name gives
+    one index at a time; a short buffer and a bad kind are errors."""
+    import ctypes
+    from pdlp_amd._lib import lib
+    with gzip.open(os.path.join(MPS_DIR, "israel.mps.gz"), "rb") as f:
+        path = write(tmp_path, "israel.mps", f.read().decode("latin-1"))
+    h = ctypes.c_void_p()
+    P.check(lib().supr_mps_read(os.fsencode(path), ctypes.byref(h)))
+    try:
+        dims = [ctypes.c_int32() for _ in range(5)]
+        P.check(lib().supr_mps_sizes(h, *[ctypes.byref(d) for d in dims]))
+        m, n = dims[0].value, dims[1].value
+        for kind, k in ((1, n), (2, m)):
+            one = [lib().supr_mps_name(h, kind, j).decode() for j in range(k)]
+            assert P._mps_names(h, kind) == one
+            nb = ctypes.c_int64()
+            P.check(lib().supr_mps_names(h, kind, None, 0, ctypes.byref(nb)))
+            assert nb.value == sum(len(x) + 1 for x in one)
+            short = ctypes.create_string_buffer(nb.value - 1)
+            assert lib().supr_mps_names(h, kind, short, nb.value - 1, ctypes.byref(nb)) == -1
+        assert lib().supr_mps_names(h, 3, None, 0, ctypes.byref(nb)) == -1
+    finally:
+        lib().supr_mps_free(h)
+    d = dict(np.load(os.path.join(MPS_DIR, "israel.npz")))
+    assert P.read_mps(path).col_names == [str(v) for v in d["col_names"]]
