@@ -631,8 +631,9 @@ def read_mps(path, names=True):
         dims = [ctypes.c_int32() for _ in range(5)]
         check(lib().supr_mps_sizes(h, *[ctypes.byref(d) for d in dims]))
         m, n, nnz, qn, mx = (d.value for d in dims)
-        row, col = np.zeros(nnz, dtype=_i32), np.zeros(nnz, dtype=_i32)
-        val, c, rL, rU, l, u = (np.zeros(k) for k in (nnz, n, m, m, n, n))
+        # every array is filled by supr_mps_get
+        row, col = np.empty(nnz, dtype=_i32), np.empty(nnz, dtype=_i32)
+        val, c, rL, rU, l, u = (np.empty(k) for k in (nnz, n, m, m, n, n))
         integ = ctypes.create_string_buffer(max(1, n))
         qr, qc, qv = np.zeros(qn, dtype=_i32), np.zeros(qn, dtype=_i32), np.zeros(qn)
         off = ctypes.c_double(0.0)
