@@ -58,20 +58,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(lp, s, ell_AT, ell_A):
+def nonneg_bounds(lp, s):
+    """Every l_j is +0 and every u_j +inf in the loop's precision (size s):
+    the single-LP primal kernel then reads neither l nor u."""
+    dt = np.float32 if s == 4 else np.float64
+    l, u = np.asarray(lp.l).astype(dt), np.asarray(lp.u).astype(dt)
+    return bool(np.all((l == 0) & ~np.signbit(l)) and np.all(np.isposinf(u)))
+
+
+def kernel_bytes(lp, s, ell_AT, ell_A, nonneg=False):
     """Algorithmic bytes of one launch of each iteration kernel (every array
     read or written once), value/vector size s (4: FP32 loop, 8: FP64 loop).
       primal half (rows of A^T): A^T indices + values, row pointers unless ELL,
-        y once, then per column x, c, l, u read, xsum read+write, aty and xn
-        written, and in the FP32 loop the (xn, xn - x) pair written
-        -> nnz (4 + s) + [4 (n + 1)] + s m + 8 s n  [+ 8 n]
+        y once, then per column x, c, l, u read (l, u not when nonneg: every
+        l = +0, u = +inf), xsum read+write, aty and xn written, and in the
+        FP32 loop the (xn, xn - x) pair written
+        -> nnz (4 + s) + [4 (n + 1)] + s m + (8 | 6) s n  [+ 8 n]
       dual half (rows of A): A indices + values, row pointers unless ELL, the
         gathered vector once (xn; the (xn, xn - x) pairs in the FP32 loop),
         then per row y, ax, b read, ysum read+write, axn and yn written, the
         inequality flag -> nnz (4 + s) + [4 (m + 1)] + (s | 8) n + 7 s m + m"""
     n, m, nnz = lp.n, lp.m, lp.nnz
     pair = s == 4
-    primal = nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + 8 * s * n + (8 * n if pair else 0)
+    primal = (nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + (6 if nonneg else 8) * s * n
+              + (8 * n if pair else 0))
     dual = nnz * (4 + s) + (0 if ell_A else 4 * (m + 1)) + (8 if pair else s) * n + 7 * s * m + m
     return primal, dual
 
@@ -559,7 +569,7 @@ def bench_large_kernels(reps=50):
            "lanes_AT": kb["lanes_AT"], "kernels": {}}
     for name, key, sz in (("k_primal_f32", "primal_ms_fp32", 4), ("k_dual_f32", "dual_ms_fp32", 4),
                           ("k_primal_f64", "primal_ms_fp64", 8), ("k_dual_f64", "dual_ms_fp64", 8)):
-        prim, dual = kernel_bytes(lp, sz, False, False)
+        prim, dual = kernel_bytes(lp, sz, False, False, nonneg_bounds(s, sz))
         b = prim if "primal" in name else dual
         out["kernels"][name] = dict(ms=kb[key], bytes=b, GBps=b / (kb[key] * 1e-3) / 1e9,
                                     frac=b / (kb[key] * 1e-3) / 1e9 / HBM_PEAK_GBS)
@@ -695,8 +705,8 @@ def main():
 
     log("timing the iteration kernels (HIP events, 200 launches each)")
     kb = P.bench_kernels(lp, scaled, Dr, Dc, reps=200)
-    prim32, dual32 = kernel_bytes(lp, 4, kb["ell_AT"] > 0, kb["ell_A"] > 0)
-    prim64, dual64 = kernel_bytes(lp, 8, kb["ell_AT"] > 0, kb["ell_A"] > 0)
+    prim32, dual32 = kernel_bytes(lp, 4, kb["ell_AT"] > 0, kb["ell_A"] > 0, nonneg_bounds(scaled, 4))
+    prim64, dual64 = kernel_bytes(lp, 8, kb["ell_AT"] > 0, kb["ell_A"] > 0, nonneg_bounds(scaled, 8))
     kernels = {
         "k_primal_f32": dict(ms=kb["primal_ms_fp32"], bytes=prim32),
         "k_dual_f32": dict(ms=kb["dual_ms_fp32"], bytes=dual32),

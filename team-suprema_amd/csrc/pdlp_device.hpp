@@ -93,7 +93,7 @@ template <typename T> struct IterBufs {
     T* ylast;
     T* xd;        // FP32 loop: interleaved pairs (xn, xn - x) for k_dual's gather
     const T* c;
-    const T* l;
+    const T* l;   // l, u null: every l_j = +0 and u_j = +inf (x >= 0)
     const T* u;
     const T* b;
     const uint8_t* ineq;

@@ -338,6 +338,9 @@ class Solver {
     // device-resident modes
     template <typename T> struct Iter {
         DevBuf x[2], y[2], ax[2], aty, xsum, ysum, xlast, ylast, xd, c, l, u, b;
+        // every l_j is +0 and every u_j is +inf in this precision: l and u are
+        // not stored and the kernels project onto x >= 0
+        bool nonneg = false;
         IterBufs<T> view(const uint8_t* ineq) const {
             IterBufs<T> B;
             for (int k = 0; k < 2; ++k) {
@@ -352,8 +355,8 @@ class Solver {
             B.ylast = ylast.template as<T>();
             B.xd = xd.template as<T>();
             B.c = c.template as<T>();
-            B.l = l.template as<T>();
-            B.u = u.template as<T>();
+            B.l = nonneg ? nullptr : l.template as<T>();
+            B.u = nonneg ? nullptr : u.template as<T>();
             B.b = b.template as<T>();
             B.ineq = ineq;
             return B;
@@ -687,7 +690,11 @@ void Solver::alloc_iter(Iter<T>& I) {
     std::vector<T> c(n), l(n), u(n), b(m);
     for (int j = 0; j < n; ++j) { c[j] = (T)S_.c[j]; l[j] = (T)S_.l[j]; u[j] = (T)S_.u[j]; }
     for (int i = 0; i < m; ++i) b[i] = (T)S_.rL[i];
-    I.c = upload(c, s); I.l = upload(l, s); I.u = upload(u, s); I.b = upload(b, s);
+    I.nonneg = true;
+    for (int j = 0; j < n && I.nonneg; ++j)
+        I.nonneg = l[j] == T(0) && !std::signbit(l[j]) && u[j] == std::numeric_limits<T>::infinity();
+    I.c = upload(c, s); I.b = upload(b, s);
+    if (!I.nonneg) { I.l = upload(l, s); I.u = upload(u, s); }
     for (int k = 0; k < 2; ++k) {
         I.x[k] = device_vec<T>(n);
         I.y[k] = device_vec<T>(m);

@@ -148,7 +148,10 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
     auto epilogue = [&](int row, T a) {
         const T xj = x[row];
         if (pending) B.xsum[row] += eta_acc * xj;
-        const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
+        const T z = xj - tau * (B.c[row] - a);
+        // B.l null: l = +0 and u = +inf everywhere, and smin(smax(z, +0), +inf)
+        // is smax(z, +0) bit for bit -- 2 of the 7 per-column streams skipped
+        const T v = B.l ? smin(smax(z, B.l[row]), B.u[row]) : smax(z, T(0));
         xn[row] = v;
         if constexpr (sizeof(T) == 4)
             reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
