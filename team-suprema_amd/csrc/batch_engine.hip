@@ -445,6 +445,9 @@ struct Pool {
         I = BatchInst{};
         I.m = p.m; I.n = p.n;
         I.ro = ro; I.co = co; I.pa = ro; I.pat = co; I.za = za;
+        I.nonneg = 1;
+        for (int32_t j = 0; j < p.n && I.nonneg; ++j)
+            I.nonneg = p.l[j] == 0.0 && !std::signbit(p.l[j]) && p.u[j] == std::numeric_limits<double>::infinity();
         nnz_slot[k] = p.nnz;
         meanA[k] = p.meanA;
         meanAT[k] = p.meanAT;

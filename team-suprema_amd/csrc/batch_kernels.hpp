@@ -29,6 +29,8 @@ struct BatchInst {
     int32_t bn, bm;      // blocks of this instance in column-side / row-side launches
     int32_t on, om;      // its first block in each (also its first partial slot)
     int32_t sn, sm;      // column / row side uses the LDS-staged tiles (for_row_tiles)
+    int32_t nonneg;      // every l_j = +0 and u_j = +inf: the primal projects onto x >= 0
+    int32_t pad0;
     int64_t ro, co;      // offsets into the concatenated row / column vectors
     int64_t pa, pat;     // offsets of its row-pointer arrays (m + 1 and n + 1 entries)
     int64_t za;          // offset into the nonzero arrays (A and A^T hold nnz each)
@@ -169,10 +171,13 @@ __device__ __forceinline__ double primal_rows(const BatchGeo& g, const BatchInst
     T* __restrict__ xn = sel(B.x, cur ^ 1);
     const T* __restrict__ y = sel(B.y, cur);
     double acc = 0.0;
+    const bool nonneg = I.nonneg != 0;
     auto epilogue = [&](int row, T a) {
         const T xj = x[row];
         if (pending) B.xsum[row] += eta_acc * xj;
-        const T v = smin(smax(xj - tau * (B.c[row] - a), B.l[row]), B.u[row]);
+        const T z = xj - tau * (B.c[row] - a);
+        // smin(smax(z, +0), +inf) is smax(z, +0) bit for bit: no l / u reads
+        const T v = nonneg ? smax(z, T(0)) : smin(smax(z, B.l[row]), B.u[row]);
         xn[row] = v;
         if constexpr (sizeof(T) == 4)
             reinterpret_cast<float2*>(B.xd)[row] = make_float2(v, v - xj);
