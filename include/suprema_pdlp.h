@@ -148,7 +148,11 @@ typedef int (*supr_stop_fn)(void* ctx);
 /* solve_pdhg_gpu(): unscaled and scaled PDLP forms share one COO pattern
  * (scaled = diag(Dr) A diag(Dc) with scaled c, bounds, rows as produced by
  * supr_scale); x_out (n) and y_out (m) receive the returned point in
- * UNSCALED PDLP-form coordinates, exactly like PdlpResult::x / y. */
+ * UNSCALED PDLP-form coordinates, exactly like PdlpResult::x / y.  Forms of
+ * 2^16 or more nonzeros are laid out (CSR / ELL of A and A^T) on the device,
+ * smaller ones on the host, with the same layout either way; the environment
+ * variable SUPR_HOST_LAYOUT=1 forces the host build (also for
+ * supr_engine_create). */
 int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                         const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,

@@ -114,14 +114,6 @@ void build_pattern(DevPattern& P, const int32_t* keys, const int32_t* other, int
     }
 }
 
-void gather_values(const DevPattern& P, const double* coo_val, DevBuf& out, hipStream_t s) {
-    out = device_vec<double>(P.nnz);
-    if (P.nnz == 0) return;
-    hipLaunchKernelGGL(kp_gather_val, dim3(grid_for(P.nnz)), dim3(kThreads), 0, s,
-                       (const int32_t*)P.perm.as<int32_t>(), P.nnz, coo_val, out.as<double>());
-    SUPR_HIP(hipGetLastError());
-}
-
 DevBuf copy_of(const DevBuf& src, size_t bytes, hipStream_t s) {
     DevBuf d(bytes);
     if (bytes > 0)
@@ -130,6 +122,14 @@ DevBuf copy_of(const DevBuf& src, size_t bytes, hipStream_t s) {
 }
 
 }  // namespace
+
+void gather_values(const DevPattern& P, const double* coo_val, DevBuf& out, hipStream_t s) {
+    out = device_vec<double>(P.nnz);
+    if (P.nnz == 0) return;
+    hipLaunchKernelGGL(kp_gather_val, dim3(grid_for(P.nnz)), dim3(kThreads), 0, s,
+                       (const int32_t*)P.perm.as<int32_t>(), P.nnz, coo_val, out.as<double>());
+    SUPR_HIP(hipGetLastError());
+}
 
 DevLp::DevLp(const supr_lp& lp, hipStream_t s, bool with_layout) : m(lp.m), n(lp.n), nnz(lp.nnz) {
     row = upload(lp.row_idx, nnz, s);

@@ -62,4 +62,8 @@ double device_absmax(const double* v, int32_t len, hipStream_t s);
 // out[perm[p]] = v[p] for the pattern's entries (device arrays).
 void scatter_to_coo(const DevPattern& P, const double* v, double* out, hipStream_t s);
 
+// COO-order values of the pattern's LP in the pattern's order (CSR for A,
+// CSC for A^T): out = a new device array of P.nnz values.
+void gather_values(const DevPattern& P, const double* coo_val, DevBuf& out, hipStream_t s);
+
 }  // namespace supr

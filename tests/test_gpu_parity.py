@@ -322,6 +322,30 @@ def test_direct_launches_match_graph_replay(mode):
         assert np.array_equal(r.x, runs[0].x) and np.array_equal(r.y, runs[0].y)
 
 
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_device_layout_matches_host_layout(monkeypatch, mode):
+    """Caller-scaled forms of 2^16 or more nonzeros are laid out on the device
+    (a stable counting sort, as the host's build_csr): with the lanes pinned,
+    the solve gives the same bits as with the host layout (SUPR_HOST_LAYOUT=1),
+    for CSR rows (power-law LP) and ELL rows (transportation LP's A^T)."""
+    cases = [(P.gen_power_law_lp(3000, 12000, seed=8), 16, 4), (P.gen_transport(200, 400, seed=3), 32, 1)]
+    for lp, la, lat in cases:
+        assert lp.nnz >= 1 << 16
+        s, Dr, Dc = P.scale(lp)
+        def run():
+            return P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, max_iterations=4000,
+                                    spmv_lanes_A=la, spmv_lanes_AT=lat)
+        dev = run()
+        monkeypatch.setenv("SUPR_HOST_LAYOUT", "1")
+        try:
+            host = run()
+        finally:
+            monkeypatch.delenv("SUPR_HOST_LAYOUT")
+        assert dev.iterations == host.iterations and dev.restarts == host.restarts
+        assert np.array_equal(dev.x, host.x) and np.array_equal(dev.y, host.y)
+        assert (dev.info["lanes_A"], dev.info["lanes_AT"]) == (host.info["lanes_A"], host.info["lanes_AT"])
+
+
 def test_cpu_gpu_race_through_the_drop_in():
     """The reference's CPU / GPU race (src/solve.cpp:76-148): the reference's
     own CPU solve_pdhg (oracle/_ref) and this engine on two threads sharing one
