@@ -27,7 +27,7 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 
 void set_error(const std::string& msg) { t_error = msg; }
 
-bool check_lp(const supr_lp* lp, const char* what) {
+bool check_lp(const supr_lp* lp, const char* what, bool indices) {
     if (!lp) { set_error(std::string(what) + ": null LP"); return false; }
     if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) {
         set_error(std::string(what) + ": negative size");
@@ -38,7 +38,7 @@ bool check_lp(const supr_lp* lp, const char* what) {
         set_error(std::string(what) + ": null array");
         return false;
     }
-    for (int32_t k = 0; k < lp->nnz; ++k) {
+    for (int32_t k = 0; indices && k < lp->nnz; ++k) {
         if (lp->row_idx[k] < 0 || lp->row_idx[k] >= lp->m || lp->col_idx[k] < 0 ||
             lp->col_idx[k] >= lp->n) {
             set_error(std::string(what) + ": COO index out of range at entry " + std::to_string(k));

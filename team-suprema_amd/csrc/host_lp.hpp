@@ -35,7 +35,7 @@ void csr_matvec(const Csr& M, const double* x, double* out);
 
 // Validates shapes and index ranges; sets the error text and returns false
 // when the LP is malformed.
-bool check_lp(const supr_lp* lp, const char* what);
+bool check_lp(const supr_lp* lp, const char* what, bool indices = true);
 
 // verify() with prebuilt CSR(A) / CSR(A^T) of lp's values (src/verify.cpp:24-103).
 void verify_csr(const supr_lp& lp, const Csr& A, const Csr& AT, const double* x,

@@ -244,6 +244,9 @@ struct OwnedLp {
     std::vector<int32_t> row, col;
     std::vector<double> val, c, rL, rU, l, u;
     supr_lp v;
+    struct Borrow {};
+    // v aliases the caller's arrays (a one-shot solve: they outlive the solver)
+    OwnedLp(const supr_lp& s, Borrow) : v(s) {}
     explicit OwnedLp(const supr_lp& s, bool with_matrix = true)
         : row(with_matrix ? s.row_idx : nullptr, with_matrix ? s.row_idx + s.nnz : nullptr),
           col(with_matrix ? s.col_idx : nullptr, with_matrix ? s.col_idx + s.nnz : nullptr),
@@ -265,6 +268,12 @@ class Solver {
            const supr_pdlp_options& o)
         : Uown_(U), Sown_(S), Drv_(Dr, Dr + S.m), Dcv_(Dc, Dc + S.n), U_(Uown_.v), S_(Sown_.v),
           Dr_(Drv_.data()), Dc_(Dcv_.data()), o_(o), m_(S.m), n_(S.n) {}
+    // A one-shot solve: the caller's forms and factors are read in place, not
+    // copied (they are const and outlive the call).
+    Solver(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc,
+           const supr_pdlp_options& o, OwnedLp::Borrow b)
+        : Uown_(U, b), Sown_(S, b), U_(Uown_.v), S_(Sown_.v), Dr_(Dr), Dc_(Dc), o_(o), m_(S.m),
+          n_(S.n) {}
     // From the unscaled form alone: the matrix is laid out and scaled on the
     // device (scale(), src/scaling.cpp:25-84, bit-identical to the host's);
     // the scaled vectors and Dr, Dc come back to the host (n + m values).
@@ -1143,7 +1152,8 @@ int check_options(const supr_lp* S, const supr_pdlp_options* opt);
 
 int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const double* Dc,
                      const supr_pdlp_options* opt) {
-    if (!check_lp(U, "unscaled_form") || !check_lp(S, "scaled_form")) return SUPR_EINVAL;
+    // U's indices equal S's (checked below), so only S's are range-checked
+    if (!check_lp(U, "unscaled_form", false) || !check_lp(S, "scaled_form")) return SUPR_EINVAL;
     if (U->m != S->m || U->n != S->n || U->nnz != S->nnz) {
         set_error("unscaled and scaled forms differ in shape");
         return SUPR_EINVAL;
@@ -1222,7 +1232,7 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
     if (!x_out || !y_out || !info) { set_error("null output"); return SUPR_EINVAL; }
     try {
         SUPR_HIP(hipSetDevice(opt->device));
-        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt);
+        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt, OwnedLp::Borrow{});
         solver.setup();
         solver.run(stop, stop_ctx, x_out, y_out, info);
         return SUPR_OK;
