@@ -655,6 +655,12 @@ def main():
         print(json.dumps({"kernels_ms": kb}))
         return
 
+    # the process's first device use (runtime start, code-object loading) is
+    # paid here, so that setup_seconds below is the engine's own setup
+    warm = P.gen_random_lp(50, 120, 4, seed=1)
+    ws, wdr, wdc = P.scale(warm)
+    P.solve_pdhg_gpu(warm, ws, wdr, wdc, tol=TOL, mode=P.MODE_MIXED)
+
     results = {}
     for name, mode in (("mixed", P.MODE_MIXED), ("fast_fp64", P.MODE_FAST_FP64)):
         t0 = time.perf_counter()

@@ -533,7 +533,7 @@ void Solver::setup_plain() {
     // host CSR of the unscaled form for the host verifier (pdlp_algo.hpp:57)
     Au_ = std::make_unique<Csr>(build_csr(U_, U_.val, false));
     AuT_ = std::make_unique<Csr>(build_csr(U_, U_.val, true));
-    auto vec = [&](const double* p, int len) { return upload(std::vector<double>(p, p + len), s); };
+    auto vec = [&](const double* p, int len) { return upload(p, (size_t)len, s); };
     pc_ = vec(S_.c, n); pl_ = vec(S_.l, n); pu_ = vec(S_.u, n); pb_ = vec(S_.rL, m);
     std::vector<uint8_t> hineq(m);
     for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
@@ -771,7 +771,7 @@ void Solver::setup_device_resident(bool mixed) {
     ineq_ = upload(hineq, s);
     if (mixed) alloc_iter(F_);
     alloc_iter(D_);
-    auto vec = [&](const double* p, int len) { return upload(std::vector<double>(p, p + len), s); };
+    auto vec = [&](const double* p, int len) { return upload(p, (size_t)len, s); };
     kc_ = vec(U_.c, n); kl_ = vec(U_.l, n); ku_ = vec(U_.u, n);
     krL_ = vec(U_.rL, m); krU_ = vec(U_.rU, m);
     kDr_ = vec(Dr_, m); kDc_ = vec(Dc_, n);
