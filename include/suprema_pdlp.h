@@ -152,7 +152,12 @@ typedef int (*supr_stop_fn)(void* ctx);
  * 2^16 or more nonzeros are laid out (CSR / ELL of A and A^T) on the device,
  * smaller ones on the host, with the same layout either way; the environment
  * variable SUPR_HOST_LAYOUT=1 forces the host build (also for
- * supr_engine_create). */
+ * supr_engine_create).  An LP with 2^16 or more columns whose largest column
+ * degree is at least 4x the mean is solved with its columns relabelled
+ * (sorted by degree, in groups of 16 dealt over 256-column tiles: the dual
+ * half's gathers of x then share cache lines); x_out and the objective come
+ * back in the caller's column order.  SUPR_COLUMN_ORDER=given turns this off
+ * (also for supr_engine_create and supr_bench_kernels). */
 int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                         const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
