@@ -114,8 +114,11 @@ typedef struct supr_pdlp_options {
     int32_t stall_checks;      /* MIXED: ... or after this many checks without a new best (0: auto) */
     int32_t graph_attempts;    /* device-resident modes: attempts per graph replay (0: auto = 16); < 0: the same
                                   kernel sequence launched on the stream directly, -graph_attempts attempts per poll */
-    int32_t spmv_lanes_A;      /* lanes per row for A (0: autotune; 1..64 powers of two, or 256 = one block per row) */
-    int32_t spmv_lanes_AT;     /* lanes per row for A^T (0: autotune) */
+    int32_t spmv_lanes_A;      /* lanes per row for A: 0 = a rule of the matrix shape (deterministic: every
+                                  run sums in the same order); -1 = timed autotune (may differ run to run);
+                                  1..64 powers of two, or 256 = one block per row.  The batched engine:
+                                  0 tunes its lanes rule by timing, see supr_batch_create */
+    int32_t spmv_lanes_AT;     /* lanes per row for A^T (as spmv_lanes_A) */
     int32_t device;            /* HIP device ordinal */
     int32_t verbose;           /* 0 quiet, 1 phase summary on stderr, 2 also a progress line every 5 s, 3 also per-kernel timing (batched) */
 } supr_pdlp_options;
@@ -157,7 +160,8 @@ typedef int (*supr_stop_fn)(void* ctx);
  * (sorted by degree, in groups of 16 dealt over 256-column tiles: the dual
  * half's gathers of x then share cache lines); x_out and the objective come
  * back in the caller's column order.  SUPR_COLUMN_ORDER=given turns this off
- * (also for supr_engine_create and supr_bench_kernels). */
+ * (also for supr_engine_create and supr_bench_kernels); SUPR_MODE_PLAIN_FP64,
+ * the reference engine's design, always solves the columns as given. */
 int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                         const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
@@ -205,7 +209,8 @@ int supr_engine_create_form(const supr_lp* form, const char* integer, int32_t ru
                             const supr_pdlp_options* opt, supr_engine** engine);
 
 /* y = A x (transposed = 0; y has m entries) or A^T x (transposed = 1; y has
- * n entries) on the device, with lanes-per-row `lanes` (0: autotuned) and
+ * n entries) on the device, with lanes-per-row `lanes` (0: the shape rule,
+ * -1: timed autotune) and
  * FP32 (fp32 = 1: values, x and the accumulation in FP32) or FP64 arithmetic. */
 int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32,
               const double* x, double* y);

@@ -63,6 +63,9 @@ class Checker:
         self._scale.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip] + [_dp] * 6
                                 + ([ctypes.c_char_p] if kind == "oracle" else [])
                                 + [ctypes.c_int] + [_dp] * 8)
+        self._extract = getattr(self.lib, p + "extract_solution")
+        self._extract.restype = ctypes.c_int
+        self._extract.argtypes = [ctypes.c_int] * 3 + [_ip] + [_dp] * 5
         if kind == "oracle":
             self._spmv = self.lib.orc_csr_spmv
             self._spmv.restype = ctypes.c_int
@@ -70,6 +73,17 @@ class Checker:
             self._csr = self.lib.orc_csr_arrays
             self._csr.restype = ctypes.c_int
             self._csr.argtypes = [ctypes.c_int] * 3 + [_ip, _ip, _dp, _ip, _ip, _dp]
+            self._step_rule = self.lib.orc_step_rule
+            self._step_rule.restype = ctypes.c_int
+            self._step_rule.argtypes = ([ctypes.c_double, ctypes.c_double, ctypes.c_int]
+                                        + [ctypes.c_double] * 3 + [_dp, _dp])
+            self._check_rule = self.lib.orc_check_rule
+            self._check_rule.restype = ctypes.c_int
+            self._check_rule.argtypes = ([_dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, _dp, _dp, _ip, _ip, _ip, _dp])
+            self._primal_weight = self.lib.orc_primal_weight
+            self._primal_weight.restype = ctypes.c_double
+            self._primal_weight.argtypes = [ctypes.c_double] * 3
 
     @staticmethod
     def _lpargs(lp):
@@ -110,6 +124,19 @@ class Checker:
         assert rc == 0
         return dims, ri, ci, arrs, perm, signs
 
+    def extract_solution(self, n_orig, m_orig, row_perm, row_signs, x2, y2):
+        """extract_solution(meta, x2, y2): x = the first n_orig entries of x2,
+        y[row_perm[p]] = row_signs[p] * y2[p] (y = 0 elsewhere)."""
+        row_perm = np.ascontiguousarray(row_perm, dtype=np.int32)
+        row_signs = np.ascontiguousarray(row_signs, dtype=np.float64)
+        x2 = np.ascontiguousarray(x2, dtype=np.float64)
+        y2 = np.ascontiguousarray(y2, dtype=np.float64)
+        x, y = np.zeros(n_orig), np.zeros(m_orig)
+        rc = self._extract(int(n_orig), int(m_orig), len(row_perm), _i(row_perm), _d(row_signs),
+                           _d(x2), _d(y2), _d(x), _d(y))
+        assert rc == 0
+        return x, y
+
     def pdlp(self, unscaled, scaled, Dr, Dc, max_iterations=200000, check_every=64, tol=1e-6):
         n, m = scaled.n, scaled.m
         x, y = np.zeros(n), np.zeros(m)
@@ -125,6 +152,35 @@ class Checker:
                         int(check_every), float(tol), None, _d(x), _d(y), _i(info_i), _d(info_d))
         assert rc == 0
         return PdlpOut(x, y, int(info_i[0]), int(info_i[1]), info_i[2], info_i[3], info_d)
+
+    # The scalar rules orc_pdlp takes its decisions through (oracle only).
+    def step_rule(self, eta, omega, accepted, dx2, dy2, inter):
+        """include/pdlp_algo.hpp:113-131 -> (accepted?, eta_bar, eta_next)."""
+        eb, en = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        acc = self._step_rule(float(eta), float(omega), int(accepted), float(dx2), float(dy2),
+                              float(inter), ctypes.byref(eb), ctypes.byref(en))
+        return bool(acc), eb.value, en.value
+
+    def check_rule(self, rc, ra, tol, since_restart, accepted, merit_last_restart, best_merit,
+                   merit_prev_cand):
+        """include/pdlp_algo.hpp:94-100, 135-163 from the eps of the current
+        point (rc) and the average (ra).  Returns a dict: converged,
+        best_which, use_avg, restart, cand_merit, best_merit, merit_prev_cand."""
+        rc = np.ascontiguousarray(rc, dtype=np.float64)
+        ra = np.ascontiguousarray(ra, dtype=np.float64)
+        bm, mp, cm = (ctypes.c_double(float(best_merit)), ctypes.c_double(float(merit_prev_cand)),
+                      ctypes.c_double(0.0))
+        bw, ua, rs = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        done = self._check_rule(_d(rc), _d(ra), float(tol), int(since_restart), int(accepted),
+                                float(merit_last_restart), ctypes.byref(bm), ctypes.byref(mp),
+                                ctypes.byref(bw), ctypes.byref(ua), ctypes.byref(rs), ctypes.byref(cm))
+        return dict(converged=bool(done), best_which=bw.value, use_avg=bool(ua.value),
+                    restart=bool(rs.value), cand_merit=cm.value, best_merit=bm.value,
+                    merit_prev_cand=mp.value)
+
+    def primal_weight(self, omega, ddx, ddy):
+        """include/pdlp_algo.hpp:165-167 (ddx, ddy: restart distances)."""
+        return self._primal_weight(float(omega), float(ddx), float(ddy))
 
     def spmv(self, lp, x, transposed=False):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -15,7 +15,11 @@
  *   orc_pdlp_form_*       src/pdlp.cpp:8-82
  *   orc_extract_solution  src/pdlp.cpp:84-90
  *   orc_pdlp              include/pdlp_algo.hpp:49-185 with CpuBackend,
- *                         src/pdlp.cpp:97-152
+ *                         src/pdlp.cpp:97-152; its scalar rules as
+ *   orc_step_rule         include/pdlp_algo.hpp:113-131 (adaptive step size)
+ *   orc_check_rule        include/pdlp_algo.hpp:94-100, 135-163 (keep-best,
+ *                         candidate, convergence, restart predicates)
+ *   orc_primal_weight     include/pdlp_algo.hpp:165-167
  *
  * The arithmetic is written in the reference's evaluation order (left-to-
  * right row sums, COO-order accumulations, (v*a)*b products) and must be
@@ -26,6 +30,10 @@
  *   - bit-for-bit against the reference CPU engine compiled from its own
  *     sources into oracle/_ref (scale, form, verify, full PDLP trajectories:
  *     iterations, restarts, eps, x, y) via committed golden fixtures;
+ *   - extract_solution: the reference's own extract_solution() output on
+ *     every MPS golden case (tests/golden/extract.json), a hand-built closed
+ *     form (permuted, sign-flipped and dropped rows) and, where _ref is
+ *     built, the live reference on random inputs;
  *   - against published Netlib optima (tools/audit_lp.cpp:148-160);
  *   - against closed forms: dense brute-force matvec, a row off by delta
  *     giving eps_P = delta/(1+|b|inf), infinite-bound sign cases, Ruiz
@@ -385,6 +393,64 @@ static double merit3(const double* r) {
     return sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
 }
 
+/* The adaptive step-size rule, include/pdlp_algo.hpp:113-131: from the step
+ * statistics of one attempt, *eta_bar = (omega dx2 + dy2 / omega) /
+ * (2 |inter|) (+inf when inter = 0) and *eta_next = min((1 - k1^-0.3)
+ * eta_bar, (1 + k1^-0.6) eta) with k1 = accepted + 2 (the growth term alone
+ * when eta_bar is infinite).  Returns 1 when the step is accepted
+ * (eta <= eta_bar).  orc_pdlp takes every step decision through it. */
+int orc_step_rule(double eta, double omega, int accepted, double dx2, double dy2, double inter,
+                  double* eta_bar_out, double* eta_next_out) {
+    const double num = omega * dx2 + dy2 / omega;
+    const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : ORC_INF;
+    const double k1 = accepted + 2.0;
+    const double grow = (1.0 + pow(k1, -0.6)) * eta;
+    const double eta_next = isfinite(eta_bar) ? SMIN((1.0 - pow(k1, -0.3)) * eta_bar, grow) : grow;
+    *eta_bar_out = eta_bar;
+    *eta_next_out = eta_next;
+    return eta <= eta_bar;
+}
+
+/* The decisions of one KKT check, include/pdlp_algo.hpp:135-163, from the
+ * verifier's eps of the current point (rc) and of the average (ra):
+ * keep_best of the current then the average (:94-100, :142-143) on
+ * *best_merit (*best_which = 0 none improved, 1 current, 2 average), the
+ * candidate (*use_avg: the average's merit is strictly smaller), and either
+ * convergence (returns 1: max eps of the candidate <= tol; nothing else is
+ * updated) or the restart predicates (*restart = sufficient | necessary |
+ * artificial) with *merit_prev_cand moved to the candidate's merit and
+ * *cand_merit set.  orc_pdlp takes every check's decisions through it. */
+int orc_check_rule(const double* rc, const double* ra, double tol, int since_restart,
+                   int accepted, double merit_last_restart, double* best_merit,
+                   double* merit_prev_cand, int* best_which, int* use_avg, int* restart,
+                   double* cand_merit) {
+    *best_which = 0;
+    if (merit3(rc) < *best_merit) { *best_merit = merit3(rc); *best_which = 1; }
+    if (merit3(ra) < *best_merit) { *best_merit = merit3(ra); *best_which = 2; }
+    *use_avg = merit3(ra) < merit3(rc);
+    const double* rep = *use_avg ? ra : rc;
+    const double mc = merit3(rep);
+    *cand_merit = mc;
+    *restart = 0;
+    const double worst = SMAX(SMAX(rep[0], rep[1]), rep[2]);   /* std::max({...}) */
+    if (worst <= tol) return 1;
+    const int sufficient = mc <= 0.2 * merit_last_restart;
+    const int necessary = mc <= 0.8 * merit_last_restart && mc > *merit_prev_cand;
+    const int artificial = since_restart >= 0.36 * accepted;
+    *merit_prev_cand = mc;
+    *restart = sufficient || necessary || artificial;
+    return 0;
+}
+
+/* The primal-weight update on a restart, include/pdlp_algo.hpp:165-167:
+ * omega' = exp(0.5 log(ddy / ddx) + 0.5 log omega) when both distances
+ * exceed 1e-10, else omega (ddx, ddy: 2-norms of the moves since the last
+ * restart point, in scaled space). */
+double orc_primal_weight(double omega, double ddx, double ddy) {
+    if (ddx > 1e-10 && ddy > 1e-10) return exp(0.5 * log(ddy / ddx) + 0.5 * log(omega));
+    return omega;
+}
+
 /* include/pdlp_algo.hpp:49-185 with the CpuBackend arithmetic of
  * src/pdlp.cpp:97-142.  The unscaled and scaled forms share one COO pattern
  * (scale() only rescales values).  stop may be NULL; it is polled once per
@@ -456,12 +522,8 @@ int orc_pdlp(int m, int n, int nnz, const int* ri, const int* ci,
             double dx2 = 0.0, dy2 = 0.0, inter = 0.0;   /* step_stats */
             for (int j = 0; j < n; ++j) { double d = xn[j] - x[j]; dx2 += d * d; }
             for (int i = 0; i < m; ++i) { double d = yn[i] - y[i]; dy2 += d * d; inter += d * (axn[i] - ax[i]); }
-            const double num = omega * dx2 + dy2 / omega;
-            const double eta_bar = fabs(inter) > 0 ? num / (2.0 * fabs(inter)) : ORC_INF;
-            const double k1 = accepted + 2.0;
-            const double grow = (1.0 + pow(k1, -0.6)) * eta;
-            const double eta_next = isfinite(eta_bar) ? SMIN((1.0 - pow(k1, -0.3)) * eta_bar, grow) : grow;
-            if (eta <= eta_bar) {
+            double eta_bar, eta_next;
+            if (orc_step_rule(eta, omega, accepted, dx2, dy2, inter, &eta_bar, &eta_next)) {
                 double* t;
                 t = x; x = xn; xn = t;
                 t = y; y = yn; yn = t;
@@ -488,20 +550,20 @@ int orc_pdlp(int m, int n, int nnz, const int* ri, const int* ci,
         for (int i = 0; i < m; ++i) { yu[i] = Dr[i] * y[i]; yu2[i] = Dr[i] * hys[i]; }
         verify_csr(&PU, &Au, &AuT, xu, yu, rc);
         verify_csr(&PU, &Au, &AuT, xu2, yu2, ra);
-        /* keep_best, current then average (:94-100, :142-143) */
-        if (merit3(rc) < best_merit) {
-            best_merit = merit3(rc); memcpy(best_x, xu, bn); memcpy(best_y, yu, bm);
-            best_eps[0] = rc[0]; best_eps[1] = rc[1]; best_eps[2] = rc[2]; have_best = 1;
+        /* keep_best, candidate, convergence, restart decision (:94-100, :135-163) */
+        int best_which, use_avg, restart;
+        double mc;
+        const int done = orc_check_rule(rc, ra, tol, since_restart, accepted, merit_last_restart,
+                                        &best_merit, &merit_prev_cand, &best_which, &use_avg,
+                                        &restart, &mc);
+        if (best_which) {
+            const double* r = best_which == 1 ? rc : ra;
+            memcpy(best_x, best_which == 1 ? xu : xu2, bn);
+            memcpy(best_y, best_which == 1 ? yu : yu2, bm);
+            best_eps[0] = r[0]; best_eps[1] = r[1]; best_eps[2] = r[2]; have_best = 1;
         }
-        if (merit3(ra) < best_merit) {
-            best_merit = merit3(ra); memcpy(best_x, xu2, bn); memcpy(best_y, yu2, bm);
-            best_eps[0] = ra[0]; best_eps[1] = ra[1]; best_eps[2] = ra[2]; have_best = 1;
-        }
-        const int use_avg = merit3(ra) < merit3(rc);
         const double* rep = use_avg ? ra : rc;
-        const double mc = merit3(rep);
-        const double worst = SMAX(SMAX(rep[0], rep[1]), rep[2]);   /* std::max({...}) */
-        if (worst <= tol) {
+        if (done) {
             memcpy(x_out, use_avg ? xu2 : xu, sizeof(double) * (size_t)n);
             memcpy(y_out, use_avg ? yu2 : yu, sizeof(double) * (size_t)m);
             info_d[0] = rep[0]; info_d[1] = rep[1]; info_d[2] = rep[2];
@@ -509,19 +571,15 @@ int orc_pdlp(int m, int n, int nnz, const int* ri, const int* ci,
             break;
         }
 
-        /* restart decision and primal weight (:155-174) */
-        const int sufficient = mc <= 0.2 * merit_last_restart;
-        const int necessary = mc <= 0.8 * merit_last_restart && mc > merit_prev_cand;
-        const int artificial = since_restart >= 0.36 * accepted;
-        merit_prev_cand = mc;
-        if (sufficient || necessary || artificial) {
+        /* restart and primal weight (:164-174) */
+        if (restart) {
             const double* cx = use_avg ? hxs : x;
             const double* cy = use_avg ? hys : y;
             double ddx = 0.0, ddy = 0.0;
             for (int j = 0; j < n; ++j) ddx += (cx[j] - hx_last[j]) * (cx[j] - hx_last[j]);
             for (int i = 0; i < m; ++i) ddy += (cy[i] - hy_last[i]) * (cy[i] - hy_last[i]);
             ddx = sqrt(ddx); ddy = sqrt(ddy);
-            if (ddx > 1e-10 && ddy > 1e-10) omega = exp(0.5 * log(ddy / ddx) + 0.5 * log(omega));
+            omega = orc_primal_weight(omega, ddx, ddy);
             memcpy(hx_last, cx, bn); memcpy(hy_last, cy, bm);
             if (use_avg) {
                 memcpy(x, hxs, bn); memcpy(y, hys, bm);

@@ -8,6 +8,7 @@
 //   ref_scale        -> scale()              src/scaling.cpp:25-84
 //   ref_verify       -> verify()             src/verify.cpp:18-103
 //   ref_pdlp_form_*  -> build_pdlp_form()    src/pdlp.cpp:8-82
+//   ref_extract_solution -> extract_solution() src/pdlp.cpp:84-90
 //   ref_pdlp[_live]  -> solve_pdhg()         src/pdlp.cpp:146-152 (pdlp_run)
 //   ref_mps_*        -> read_mps() + to_ranged_lp() (arrays, integer flags,
 //                       QP triangle, names, error text)
@@ -24,6 +25,7 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -119,6 +121,29 @@ int ref_pdlp_form(int m, int n, int nnz, const int* ri, const int* ci, const dou
         std::memcpy(u2, p.u.data(), sizeof(double) * n2);
         std::memcpy(row_perm, f.second.row_perm.data(), sizeof(int) * m2);
         std::memcpy(row_signs, f.second.row_signs.data(), sizeof(double) * m2);
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+// Same argument list as the oracle's orc_extract_solution: meta is rebuilt
+// from its three fields (extract_solution reads n_orig, m_orig, row_perm and
+// row_signs only).
+int ref_extract_solution(int n_orig, int m_orig, int m2, const int* row_perm,
+                         const double* row_signs, const double* x2, const double* y2,
+                         double* x, double* y) {
+    try {
+        PdlpFormMeta meta;
+        meta.n_orig = n_orig;
+        meta.m_orig = m_orig;
+        meta.row_perm.assign(row_perm, row_perm + m2);
+        meta.row_signs.assign(row_signs, row_signs + m2);
+        int n2 = n_orig;
+        // x2 holds the form's n2 >= n_orig columns; only the first n_orig are read
+        std::vector<double> vx2(x2, x2 + n2), vy2(y2, y2 + m2), vx, vy;
+        extract_solution(meta, vx2, vy2, vx, vy);
+        if ((int)vx.size() != n_orig || (int)vy.size() != m_orig) return -1;
+        std::memcpy(x, vx.data(), sizeof(double) * vx.size());
+        std::memcpy(y, vy.data(), sizeof(double) * vy.size());
         return 0;
     } catch (const std::exception&) { return -1; }
 }

@@ -1,10 +1,12 @@
 // pdlp_gpu_hip.cpp -- drop-in replacement for the reference's
 // gpu/pdlp_gpu_solve.cu: defines solve_pdhg_gpu() (reference:
 // include/pdlp_gpu.hpp:22-25) on top of the MI355X engine's C ABI.  Compile it
-// against the reference's own headers with its GPU gate defined, e.g.
-//   g++ -std=c++17 -DSOVEREIGN_WITH_CUDA -I<reference>/include -I<this repo>/include \
+// against the reference's own headers with the GPU gate renamed to
+// SUPREMA_WITH_HIP and defined, e.g.
+//   g++ -std=c++17 -DSUPREMA_WITH_HIP=1 -I<reference>/include -I<this repo>/include \
 //       -c pdlp_gpu_hip.cpp
-// and link libsuprema_pdlp.so.  The reference's types are used as they are.
+// (unrenamed headers: add -DSOVEREIGN_WITH_CUDA=SUPREMA_WITH_HIP) and link
+// libsuprema_pdlp.so.  The reference's types are used as they are.
 #include "pdlp_gpu.hpp"
 #include "suprema_pdlp.hpp"
 

@@ -455,11 +455,25 @@ struct Pool {
         init[k] = p.init;
         c_unscaled[k] = p.cu;
         obj_offset[k] = p.init.obj_offset;
-        hipStream_t st = stream;
-        auto put = [&](auto* dst, const auto& v) {
+        const hipStream_t st = stream;
+        auto put = [st](auto* dst, const auto& v) {
             if (!v.empty())
                 BATCH_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st));
         };
+        try {
+            upload(p, ro, co, za, put);
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            remove(k);   // ranges and slot back; the caller keeps the request
+            throw;
+        }
+        return k;
+    }
+
+    // Copies p's arrays into the arenas at its ranges.
+    template <typename Put>
+    void upload(const Prepared& p, int64_t ro, int64_t co, int64_t za, Put&& put) {
+        hipStream_t st = stream;
         put(ptrA.get<int32_t>() + ro, p.ptrA);
         put(ptrAT.get<int32_t>() + co, p.ptrAT);
         put(idxA.get<int32_t>() + za, p.idxA);
@@ -489,7 +503,6 @@ struct Pool {
             put(fb.get<float>() + ro, p.fb);
         }
         BATCH_HIP(hipStreamSynchronize(st));   // p's vectors may go away after this
-        return k;
     }
 
     void remove(int k) {
@@ -839,9 +852,26 @@ struct Server {
     std::unordered_map<int32_t, std::unique_ptr<Request>> running;   // slot -> request
     std::unordered_map<int32_t, Clock::time_point> admitted;
     hipGraphExec_t exec = nullptr;
+    bool recapture = false;             // the running set changed since the graph was captured
     int32_t finished_total = 0;
 
     ~Server() { if (exec) (void)hipGraphExecDestroy(exec); }
+
+    // Undoes the admission of `fresh` (not yet running): slots back to the
+    // pool, requests back to the head of the queue in their order.
+    void unadmit(std::vector<int32_t>& fresh) {
+        std::lock_guard<std::mutex> lock(mu);
+        for (auto k = fresh.rbegin(); k != fresh.rend(); ++k) {
+            auto it = running.find(*k);
+            if (it != running.end()) {
+                queue.push_front(std::move(it->second));
+                running.erase(it);
+            }
+            admitted.erase(*k);
+            pool.remove(*k);
+        }
+        fresh.clear();
+    }
 
     void create(const supr_server_options& o) {
         so = o;
@@ -864,18 +894,22 @@ struct Server {
 
     // Retires finished instances, admits queued requests that fit (first
     // come, first served: a request that does not fit waits at the head),
-    // then runs the configured number of graph replays.
+    // then runs the configured number of graph replays.  State stays
+    // consistent when a step throws: a slot leaves `live`, `running` and the
+    // pool together, after its result is stored; a request whose admission
+    // fails goes back to the head of the queue with its host data; a failed
+    // capture is retried at the next step.
     void step(int32_t* n_running, int32_t* n_queued, int32_t* n_finished) {
-        bool changed = false;
         if (!live.empty()) {
-            std::vector<int32_t> still;
             std::atomic_thread_fence(std::memory_order_acquire);
-            for (int32_t k : live) {
-                if (!((volatile int32_t*)pool.done_h)[k]) { still.push_back(k); continue; }
+            for (size_t q = 0; q < live.size();) {
+                const int32_t k = live[q];
+                if (!((volatile int32_t*)pool.done_h)[k]) { ++q; continue; }
+                auto it = running.find(k);
+                if (it == running.end()) throw std::logic_error("server: running slot without a request");
                 DevState s;
                 BATCH_HIP(hipMemcpy(&s, pool.dstate.get<DevState>() + 2 * k, sizeof(DevState),
                                     hipMemcpyDeviceToHost));
-                auto it = running.find(k);
                 Finished f;
                 f.m = pool.inst[k].m;
                 f.n = pool.inst[k].n;
@@ -887,17 +921,17 @@ struct Server {
                 f.info.seconds_total = std::chrono::duration<double>(now - it->second->submitted).count();
                 f.info.seconds_setup = f.info.seconds_total - f.info.seconds_solve;
                 const int64_t t = it->second->ticket;
-                running.erase(it);
-                admitted.erase(k);
-                pool.remove(k);
                 {
                     std::lock_guard<std::mutex> lock(mu);
                     done.emplace(t, std::move(f));
                 }
+                live.erase(live.begin() + (std::ptrdiff_t)q);
+                running.erase(it);
+                admitted.erase(k);
+                pool.remove(k);
                 ++finished_total;
-                changed = true;
+                recapture = true;
             }
-            live.swap(still);
         }
         std::vector<int32_t> fresh;
         while (true) {
@@ -908,23 +942,36 @@ struct Server {
                 r = std::move(queue.front());
                 queue.pop_front();
             }
-            const int k = pool.add(r->prep);
+            int k = -1;
+            try {
+                k = pool.add(r->prep);
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(mu);
+                queue.push_front(std::move(r));
+                unadmit(fresh);
+                throw;
+            }
             if (k < 0) {
                 std::lock_guard<std::mutex> lock(mu);
                 queue.push_front(std::move(r));
                 break;
             }
-            r->prep = Prepared{};   // uploaded; free the host copy
             admitted[k] = Clock::now();
             running[k] = std::move(r);
             fresh.push_back(k);
         }
         if (!fresh.empty()) {
-            pool.cold_start(fresh, pool.opt.max_iterations);
+            try {
+                pool.cold_start(fresh, pool.opt.max_iterations);
+            } catch (...) {
+                unadmit(fresh);
+                throw;
+            }
+            for (int32_t k : fresh) running[k]->prep = Prepared{};   // uploaded; free the host copy
             live.insert(live.end(), fresh.begin(), fresh.end());
-            changed = true;
+            recapture = true;
         }
-        if (changed) {
+        if (recapture) {
             if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
             if (!live.empty()) {
                 const BatchGeo g = pool.layout(live);
@@ -932,6 +979,7 @@ struct Server {
                 if (pool.opt.verbose)
                     std::fprintf(stderr, "[suprema] server: %zu running, graph captured\n", live.size());
             }
+            recapture = false;
         }
         if (!live.empty()) {
             const int reps = so.replays_per_step > 0 ? so.replays_per_step : 1;

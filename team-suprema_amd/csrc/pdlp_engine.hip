@@ -38,6 +38,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hip_util.hpp"
@@ -192,7 +193,26 @@ void launch_spmv(const DevMatrix& M, const MatView<V>& view, const V* x, V* y, h
     SUPR_HIP(hipGetLastError());
 }
 
-// Picks lanes-per-row for M by timing every candidate on the device.
+// Lanes per row for M from its shape alone (the default: deterministic, so
+// every run of an LP sums in the same order and gives the same bits).  ELL
+// rows take one lane.  Otherwise the largest power of two not above a
+// quarter (FP32 values) or half (FP64) of the mean row length, at most 32:
+// the choices the timed autotuner makes on the transportation, power-law and
+// random LPs measured (4 nonzeros or more per FP32 lane, 2 per FP64 lane).
+// Long rows that would leave the GPU underfilled at 32 lanes (fewer than 64
+// blocks' worth of lanes in all, rows of 512 or more) take a block per row.
+int rule_lanes(const DevMatrix& M, bool fp64) {
+    if (M.ell_w > 0 || M.rows <= 0) return 1;
+    const double mean = (double)M.nnz / M.rows;
+    const double t = fp64 ? mean / 2.0 : mean / 4.0;
+    int L = 1;
+    while (L < 32 && 2.0 * L <= t) L *= 2;
+    if (L == 32 && mean >= 512.0 && (double)M.rows * L < 64.0 * kBlock) L = kBlock;
+    return L;
+}
+
+// Picks lanes-per-row for M by timing every candidate on the device (the
+// option value -1); the choice can change from run to run with the timing.
 template <typename V>
 int autotune_lanes(DevMatrix& M, const MatView<V>& view, const V* x, V* y, hipStream_t s) {
     if (M.ell_w > 0) return 1;
@@ -272,8 +292,8 @@ struct OwnedLp {
 // 500,000 x 2,000,000: dual kernel 0.49 -> 0.30 ms, 804 -> 634 us per mixed
 // attempt).  Applied when there are at least 2^16 columns and the largest
 // degree is at least 4x the mean (SUPR_COLUMN_ORDER=given in the environment
-// turns it off); the solver sees the relabelled forms and x is mapped back,
-// so callers see the LP as given.
+// turns it off; the plain FP64 mode never relabels); the solver sees the
+// relabelled forms and x is mapped back, so callers see the LP as given.
 constexpr int32_t kRelabelMinCols = 1 << 16;
 constexpr int kRelabelGroup = 16;
 
@@ -284,9 +304,12 @@ struct ColumnRelabel {
     supr_lp U{}, S{};              // views of the relabelled forms
 };
 
-bool relabel_columns(const supr_lp& U, const supr_lp& S, const double* Dc, ColumnRelabel& R) {
+bool relabel_columns(const supr_lp& U, const supr_lp& S, const double* Dc, int mode,
+                     ColumnRelabel& R) {
     const int32_t n = S.n;
     const int64_t nnz = S.nnz;
+    // the plain FP64 mode is the reference GPU engine's design, columns as given
+    if (mode == SUPR_MODE_PLAIN_FP64) return false;
     if (n < kRelabelMinCols || nnz == 0) return false;
     if (const char* e = std::getenv("SUPR_COLUMN_ORDER"); e && std::strcmp(e, "given") == 0) return false;
     std::vector<int32_t> deg(n, 0);
@@ -620,8 +643,10 @@ void Solver::setup_plain() {
     pax_ = device_vec<double>(m); paxn_ = device_vec<double>(m); pysum_ = device_vec<double>(m);
     pstats_ = device_vec<double>(3 * kMaxPlain);
     if (o_.spmv_lanes_A > 0) A_.lanes = o_.spmv_lanes_A;
+    else if (o_.spmv_lanes_A == 0) A_.lanes = rule_lanes(A_, true);
     else autotune_lanes(A_, A_.f64(), px_.as<double>(), pax_.as<double>(), s);
     if (o_.spmv_lanes_AT > 0) AT_.lanes = o_.spmv_lanes_AT;
+    else if (o_.spmv_lanes_AT == 0) AT_.lanes = rule_lanes(AT_, true);
     else autotune_lanes(AT_, AT_.f64(), py_.as<double>(), paty_.as<double>(), s);
     if (A_.ell_w > 0 && o_.spmv_lanes_A <= 0) A_.lanes = 1;
     if (AT_.ell_w > 0 && o_.spmv_lanes_AT <= 0) AT_.lanes = 1;
@@ -841,13 +866,16 @@ void Solver::reset_iter(Iter<T>& I) {
 void Solver::setup_device_resident(bool mixed) {
     hipStream_t s = stream_.s;
     const int n = n_, m = m_;
+    const auto t0 = Clock::now();
     setup_matrices(mixed, true, true);
+    const double t_mat = seconds_since(t0);
     initial_point();
     std::vector<uint8_t> hineq(m);
     for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
     ineq_ = upload(hineq, s);
     if (mixed) alloc_iter(F_);
     alloc_iter(D_);
+    const double t_iter = seconds_since(t0);
     auto vec = [&](const double* p, int len) { return upload(p, (size_t)len, s); };
     kc_ = vec(U_.c, n); kl_ = vec(U_.l, n); ku_ = vec(U_.u, n);
     krL_ = vec(U_.rL, m); krU_ = vec(U_.rU, m);
@@ -879,16 +907,20 @@ void Solver::setup_device_resident(bool mixed) {
     SUPR_HIP(hipHostMalloc((void**)&mirror_, sizeof(DevState), hipHostMallocCoherent | hipHostMallocMapped));
     std::memset(mirror_, 0, sizeof(DevState));
     SUPR_HIP(hipHostGetDevicePointer((void**)&mirror_dev_, mirror_, 0));
+    const double t_kkt = seconds_since(t0);
 
-    // SpMV schedules per iterate precision, timed on scratch vectors
+    // SpMV schedules per iterate precision: given, from the shape rule (0),
+    // or timed on scratch vectors (-1)
     auto tune = [&](DevMatrix& M, int fixed, auto view, auto* xv, auto* yv) {
+        using V = std::remove_pointer_t<decltype(xv)>;
         if (fixed > 0) M.lanes = fixed;
         else if (M.ell_w > 0) M.lanes = 1;
+        else if (fixed == 0) M.lanes = rule_lanes(M, sizeof(V) == 8);
         else autotune_lanes(M, view, xv, yv, s);
         return M.lanes;
     };
-    if (!valid_lanes(o_.spmv_lanes_A < 0 ? 1 : std::max(1, o_.spmv_lanes_A)) ||
-        !valid_lanes(o_.spmv_lanes_AT < 0 ? 1 : std::max(1, o_.spmv_lanes_AT)))
+    if (!valid_lanes(o_.spmv_lanes_A <= 0 ? 1 : o_.spmv_lanes_A) ||
+        !valid_lanes(o_.spmv_lanes_AT <= 0 ? 1 : o_.spmv_lanes_AT))
         throw BadInput("bad lanes-per-row option");
     if (mixed) {
         lanesA_[0] = tune(A_, o_.spmv_lanes_A, A_.f32(), F_.x[1].as<float>(), F_.ax[1].as<float>());
@@ -897,6 +929,11 @@ void Solver::setup_device_resident(bool mixed) {
     lanesA_[1] = tune(A_, o_.spmv_lanes_A, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>());
     lanesAT_[1] = tune(AT_, o_.spmv_lanes_AT, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>());
     SUPR_HIP(hipStreamSynchronize(s));
+    if (o_.verbose)
+        std::fprintf(stderr, "[suprema] setup: matrices %.2f ms, iterates %.2f ms, KKT vectors %.2f ms, "
+                     "lanes %.2f ms (A %d/%d, AT %d/%d)\n", 1e3 * t_mat, 1e3 * (t_iter - t_mat),
+                     1e3 * (t_kkt - t_iter), 1e3 * (seconds_since(t0) - t_kkt), lanesA_[0], lanesA_[1],
+                     lanesAT_[0], lanesAT_[1]);
 }
 
 DevState Solver::initial_state(bool mixed) const {
@@ -1255,9 +1292,10 @@ int check_options(const supr_lp* S, const supr_pdlp_options* opt) {
         set_error("bad options: max_iterations >= 0, check_every >= 1, tol >= 0 required");
         return SUPR_EINVAL;
     }
-    if ((opt->spmv_lanes_A != 0 && !valid_lanes(opt->spmv_lanes_A)) ||
-        (opt->spmv_lanes_AT != 0 && !valid_lanes(opt->spmv_lanes_AT))) {
-        set_error("spmv lanes must be 0 (auto), a power of two in [1, 64], or 256");
+    if ((opt->spmv_lanes_A > 0 && !valid_lanes(opt->spmv_lanes_A)) ||
+        (opt->spmv_lanes_AT > 0 && !valid_lanes(opt->spmv_lanes_AT)) || opt->spmv_lanes_A < -1 ||
+        opt->spmv_lanes_AT < -1) {
+        set_error("spmv lanes must be 0 (shape rule), -1 (timed autotune), a power of two in [1, 64], or 256");
         return SUPR_EINVAL;
     }
     int count = 0;
@@ -1310,7 +1348,7 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
     try {
         SUPR_HIP(hipSetDevice(opt->device));
         ColumnRelabel R;
-        if (relabel_columns(*unscaled_form, *scaled_form, Dc, R)) {
+        if (relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R)) {
             std::vector<double> xp(R.S.n, 0.0);
             Solver solver(R.U, R.S, Dr, R.dc.data(), *opt, OwnedLp::Borrow{});
             solver.setup();
@@ -1479,7 +1517,7 @@ int supr_engine_create(const supr_lp* unscaled_form, const supr_lp* scaled_form,
         auto e = std::make_unique<supr_engine>();
         e->device = opt->device;
         ColumnRelabel R;
-        if (relabel_columns(*unscaled_form, *scaled_form, Dc, R)) {
+        if (relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R)) {
             e->solver = std::make_unique<Solver>(R.U, R.S, Dr, R.dc.data(), *opt);
             e->newpos = std::move(R.newpos);
             e->c.assign(unscaled_form->c, unscaled_form->c + unscaled_form->n);
@@ -1529,7 +1567,7 @@ int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32
     using namespace supr;
     if (!check_lp(lp, "supr_spmv")) return SUPR_EINVAL;
     if (!x || !y) { set_error("supr_spmv: null x or y"); return SUPR_EINVAL; }
-    if (lanes != 0 && !valid_lanes(lanes)) { set_error("supr_spmv: bad lanes"); return SUPR_EINVAL; }
+    if (lanes != 0 && lanes != -1 && !valid_lanes(lanes)) { set_error("supr_spmv: bad lanes"); return SUPR_EINVAL; }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1) { set_error("no HIP device visible"); return SUPR_ENODEV; }
     try {
@@ -1543,7 +1581,8 @@ int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32
         if (fp32) {
             std::vector<float> xf(x, x + cols);
             DevBuf dx = upload(xf, s), dy = device_vec<float>(rows);
-            if (lanes == 0) autotune_lanes(M, M.f32(), dx.as<float>(), dy.as<float>(), s);
+            if (lanes == 0) M.lanes = rule_lanes(M, false);
+            else if (lanes < 0) autotune_lanes(M, M.f32(), dx.as<float>(), dy.as<float>(), s);
             else M.lanes = lanes;
             launch_spmv(M, M.f32(), dx.as<float>(), dy.as<float>(), s);
             std::vector<float> yf(rows);
@@ -1552,7 +1591,8 @@ int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32
             for (int i = 0; i < rows; ++i) y[i] = yf[i];
         } else {
             DevBuf dx = upload(std::vector<double>(x, x + cols), s), dy = device_vec<double>(rows);
-            if (lanes == 0) autotune_lanes(M, M.f64(), dx.as<double>(), dy.as<double>(), s);
+            if (lanes == 0) M.lanes = rule_lanes(M, true);
+            else if (lanes < 0) autotune_lanes(M, M.f64(), dx.as<double>(), dy.as<double>(), s);
             else M.lanes = lanes;
             launch_spmv(M, M.f64(), dx.as<double>(), dy.as<double>(), s);
             SUPR_HIP(hipMemcpyAsync(y, dy.as<void>(), sizeof(double) * rows, hipMemcpyDeviceToHost, s));
@@ -1577,7 +1617,7 @@ int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
         supr_pdlp_options o = *opt;
         o.mode = SUPR_MODE_MIXED;
         ColumnRelabel R;   // the engine's layout, relabelled columns included
-        const bool rl = relabel_columns(*unscaled_form, *scaled_form, Dc, R);
+        const bool rl = relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R);
         Solver solver(rl ? R.U : *unscaled_form, rl ? R.S : *scaled_form, Dr, rl ? R.dc.data() : Dc, o);
         solver.setup();
         solver.bench_kernels(reps, out8);

@@ -224,8 +224,16 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
         axn[row] = a;
         const T yi = y[row];
         if (pending) B.ysum[row] += eta_acc * yi;
-        if constexpr (!kDiff) ad = a - ax[row];
-        T v = yi + sigma * (B.b[row] - (a + ad));
+        // FP32: A(2 xn - x) = A xn + A (xn - x); FP64: 2 A xn - A x rounded
+        // once, as the reference forms it (pdlp_gpu_solve.cu:89)
+        T v;
+        if constexpr (kDiff) {
+            v = yi + sigma * (B.b[row] - (a + ad));
+        } else {
+            const T axr = ax[row];
+            ad = a - axr;
+            v = yi + sigma * (B.b[row] - (T(2) * a - axr));
+        }
         if (B.ineq[row] && v < T(0)) v = T(0);
         yn[row] = v;
         const double d = (double)v - (double)yi;

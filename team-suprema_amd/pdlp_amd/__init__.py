@@ -109,7 +109,7 @@ def device_count():
 
 def spmv(lp, x, transposed=False, lanes=0, fp32=False):
     """A x (or A^T x) on the device with one SpMV schedule (GpuBackend::spmv,
-    gpu/pdlp_gpu_solve.cu:211-222); lanes 0 = autotuned."""
+    gpu/pdlp_gpu_solve.cu:211-222); lanes 0 = the shape rule, -1 = timed autotune."""
     x = np.ascontiguousarray(x, dtype=_f64)
     if x.shape != ((lp.m,) if transposed else (lp.n,)):
         raise ValueError("x has the wrong length")

@@ -75,7 +75,8 @@ def test_adapter_compiles_against_the_reference_headers():
     solve_pdhg_gpu with the reference's own RangedLP / PdlpResult types."""
     import subprocess
     src = os.path.join(ROOT, "team-suprema_amd", "adapter", "pdlp_gpu_hip.cpp")
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-DSOVEREIGN_WITH_CUDA",
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-DSUPREMA_WITH_HIP=1",
+                        "-DSOVEREIGN_WITH_CUDA=SUPREMA_WITH_HIP",
                         "-I" + REF_INCLUDE, "-I" + os.path.join(ROOT, "include"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr

@@ -15,7 +15,7 @@ from conftest import golden_lp
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-OBJ_RTOL = 1e-5
+OBJ_RTOL = 1e-6
 MAX_IT = 60000
 
 

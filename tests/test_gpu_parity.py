@@ -5,10 +5,11 @@ Tolerances (floating point; the reference path is FP64 throughout):
     FP32 (values, x and accumulation in FP32): (nnz_row + 2) * 2^-23 * (|A||x|)_row.
   * Solves: converged, the ORACLE's verifier on the returned point gives
     max(eps_P, eps_D, eps_G) <= tol (1e-6), and the objective agrees with the
-    reference engine's to |d| / (1 + |obj|) <= 1e-5 (audit_lp warns above 1e-6 and
-    fails above 1e-4, tools/audit_lp.cpp:261-268; two points that both meet a
-    1e-6 KKT tolerance can differ by a few times that).
+    reference engine's to |d| / (1 + |obj|) <= 1e-6 (audit_lp's WARN level,
+    tools/audit_lp.cpp:261-268; the errors measured are 1e-9 to 1e-11).
 """
+import sys
+
 import numpy as np
 import pytest
 
@@ -18,7 +19,7 @@ from conftest import golden_lp, load_fixture
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-OBJ_RTOL = 1e-5
+OBJ_RTOL = 1e-6
 
 
 def _spmv_cases():
@@ -70,7 +71,7 @@ def test_spmv_schedules_match_oracle(oracle, name, transposed, fp32):
     cnt = np.bincount(rows, minlength=lp.n if transposed else lp.m)
     unit = 2.0 ** -23 if fp32 else 2.0 ** -52
     bound = (cnt + 2) * unit * mag + 1e-300
-    for lanes in (1, 2, 4, 8, 16, 32, 64, 256, 0):
+    for lanes in (1, 2, 4, 8, 16, 32, 64, 256, 0, -1):
         got = P.spmv(lp, x, transposed=transposed, lanes=lanes, fp32=fp32)
         err = np.abs(got - want)
         assert np.all(err <= bound), f"lanes={lanes}: worst row {np.argmax(err / bound)}"
@@ -91,7 +92,10 @@ def _check_against_reference(oracle, case, form, res):
     # the engine's own eps are the oracle verifier's within FP64 reduction order
     np.testing.assert_allclose([res.eps_P, res.eps_D, res.eps_G], rep[:3], rtol=1e-6, atol=1e-12)
     ref_obj = case["ref"]["objective_form"]
-    assert abs(rep[3] - ref_obj) / (1 + abs(ref_obj)) <= OBJ_RTOL, (rep[3], ref_obj)
+    err = abs(rep[3] - ref_obj) / (1 + abs(ref_obj))
+    print(f"[parity] {case['name']}: {res.iterations} attempts (reference {case['ref']['iterations']}), "
+          f"objective error {err:.1e}", file=sys.stderr, flush=True)
+    assert err <= OBJ_RTOL, (rep[3], ref_obj)
     return rep
 
 

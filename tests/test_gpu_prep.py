@@ -1,8 +1,11 @@
 """GPU tests of the device preprocessing: the stable CSR / CSC build
 (supr_csr_gpu) against the stable-sort definition of the reference's to_csr
 / to_csc_as_transposed_csr (include/sparse.hpp:31-57), and scale() on the
-device (supr_scale_gpu) against the host scale() (src/scaling.cpp:25-84),
-which the CPU tests pin to the reference; both bit for bit."""
+device (supr_scale_gpu) against the oracle's scale() (src/scaling.cpp:25-84,
+pinned to the reference by the golden SHA-256 digests) and against the
+product's host scale(); all bit for bit.  Cases that are golden fixtures
+are also checked against the reference's digest directly."""
+import hashlib
 import sys
 from dataclasses import replace
 
@@ -31,16 +34,31 @@ def bits_equal(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-def assert_same_scaling(lp, passes=10, integer=None):
-    hs, hDr, hDc = P.scale(lp, passes, integer)
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def assert_same_scaling(oracle, lp, passes=10, integer=None, golden_sha=None):
+    """Device scale() equals the oracle's (the checker) and the host
+    product's, bit for bit; golden_sha: the reference's digest of scale()'s
+    eight outputs for this LP."""
     ds, dDr, dDc = P.scale_gpu(lp, passes, integer)
-    for name, h, d in [("val", hs.val, ds.val), ("c", hs.c, ds.c), ("rL", hs.rL, ds.rL),
-                       ("rU", hs.rU, ds.rU), ("l", hs.l, ds.l), ("u", hs.u, ds.u),
-                       ("Dr", hDr, dDr), ("Dc", hDc, dDc)]:
-        if not bits_equal(h, d):
-            bad = np.flatnonzero(h.view(np.uint64) != d.view(np.uint64))
-            raise AssertionError(f"{lp.name}: {name} differs at {bad.size} entries, first {bad[:5]}: "
-                                 f"host {h[bad[:3]]} device {d[bad[:3]]}")
+    dev = [ds.val, ds.c, ds.rL, ds.rU, ds.l, ds.u, dDr, dDc]
+    want = oracle.scale(lp, passes, integer=None if integer is None else np.asarray(integer, np.int8))
+    hs, hDr, hDc = P.scale(lp, passes, integer)
+    host = [hs.val, hs.c, hs.rL, hs.rU, hs.l, hs.u, hDr, hDc]
+    for who, ref in (("oracle", want), ("host", host)):
+        for name, h, d in zip(("val", "c", "rL", "rU", "l", "u", "Dr", "Dc"), ref, dev):
+            h = np.ascontiguousarray(h, dtype=np.float64)
+            if not bits_equal(h, d):
+                bad = np.flatnonzero(h.view(np.uint64) != d.view(np.uint64))
+                raise AssertionError(f"{lp.name}: {name} differs from the {who} at {bad.size} entries, "
+                                     f"first {bad[:5]}: {who} {h[bad[:3]]} device {d[bad[:3]]}")
+    if golden_sha is not None:
+        assert sha(*dev) == golden_sha, f"{lp.name}: device scale() differs from the reference's digest"
 
 
 def shuffled(lp, seed):
@@ -63,8 +81,11 @@ def dense_rows_lp(m=3, n=900, seed=1):
                    name="dense rows")
 
 
+GOLDEN_SCALED = ["afiro", "sc50a", "blend", "tiny_range", "sc105"]
+
+
 def small_cases(golden):
-    out = [golden_lp(golden[nm]) for nm in ["afiro", "sc50a", "blend", "tiny_range", "sc105"]]
+    out = [golden_lp(golden[nm]) for nm in GOLDEN_SCALED]
     out.append(P.gen_random_lp(600, 2000, 6, seed=8))
     out.append(shuffled(P.gen_random_lp(600, 2000, 6, seed=9), 1))
     out.append(dense_rows_lp())
@@ -94,27 +115,38 @@ def test_csr_layout_edge_shapes():
 
 
 @pytest.mark.parametrize("passes", [0, 1, 10])
-def test_scale_gpu_is_bit_identical_to_host(golden, passes):
+def test_scale_gpu_is_bit_identical_to_oracle(oracle, golden, passes):
     cases = small_cases(golden)
-    for lp in cases:
-        assert_same_scaling(lp, passes)
-    say(f"{passes} Ruiz passes: {len(cases)} LPs bit-identical")
+    for k, lp in enumerate(cases):
+        gsha = golden[GOLDEN_SCALED[k]]["ref"]["scaled_sha"] if passes == 10 and k < len(GOLDEN_SCALED) else None
+        assert_same_scaling(oracle, lp, passes, golden_sha=gsha)
+    say(f"{passes} Ruiz passes: {len(cases)} LPs bit-identical to the oracle"
+        + (f" ({len(GOLDEN_SCALED)} to the reference's digests)" if passes == 10 else ""))
 
 
-def test_scale_gpu_integer_columns_and_edge_shapes():
+def test_scale_gpu_golden_generated_cases(oracle, golden):
+    """Generated golden cases (random, transportation, power-law, blending):
+    the device scale() gives the reference's digest."""
+    names = ["random_2000x5000_s42", "transport_60x60_s42", "power_law_1000x4000_s7", "blend_25x40_s103"]
+    for nm in names:
+        assert_same_scaling(oracle, golden_lp(golden[nm]), golden_sha=golden[nm]["ref"]["scaled_sha"])
+    say(f"{len(names)} generated golden LPs: the reference's scale() digest")
+
+
+def test_scale_gpu_integer_columns_and_edge_shapes(oracle):
     lp = P.gen_random_lp(80, 200, 5, seed=4)
-    assert_same_scaling(lp, 10, np.arange(lp.n) % 3 == 0)
+    assert_same_scaling(oracle, lp, 10, np.arange(lp.n) % 3 == 0)
     base = P.gen_random_lp(5, 9, 2, seed=3)
-    assert_same_scaling(replace(base, row=base.row[:0], col=base.col[:0], val=base.val[:0], name="empty"))
-    assert_same_scaling(replace(base, row=base.row[:3], col=base.col[:3], val=base.val[:3], name="sparse"))
+    assert_same_scaling(oracle, replace(base, row=base.row[:0], col=base.col[:0], val=base.val[:0], name="empty"))
+    assert_same_scaling(oracle, replace(base, row=base.row[:3], col=base.col[:3], val=base.val[:3], name="sparse"))
 
 
-def test_scale_gpu_at_full_sizes():
+def test_scale_gpu_at_full_sizes(oracle):
     """The 1M-variable transportation LP and a power-law LP with 7M entries
-    (long columns, 6 decades of magnitude)."""
+    (long columns, 6 decades of magnitude), against the oracle."""
     for lp in (P.gen_transport(1000, 1000, seed=42), P.gen_power_law_lp(100000, 400000, seed=42)):
         _, _, _, secs = P.scale_gpu(lp, timing=True)
-        assert_same_scaling(lp)
+        assert_same_scaling(oracle, lp)
         say(f"{lp.name}: m={lp.m} n={lp.n} nnz={lp.nnz} bit-identical; device "
             f"layout {secs[0]:.3f}s scale {secs[1]:.4f}s download {secs[2]:.3f}s")
 
@@ -218,27 +250,38 @@ def original_lps(golden):
     return out
 
 
-def test_pdlp_form_gpu_is_bit_identical(golden):
+def test_pdlp_form_gpu_is_bit_identical(golden, oracle):
+    """build_pdlp_form on the device against the oracle's (and the host
+    product's); golden MPS cases also against the reference's digest."""
     for lp in original_lps(golden):
-        hf, hm = P.build_pdlp_form(lp)
+        dims, ri, ci, arrs, perm, signs = oracle.pdlp_form(lp)
         df, dm = P.build_pdlp_form_gpu(lp)
-        for nm in ("row", "col", "val", "c", "l", "u", "rL", "rU"):
+        hf, hm = P.build_pdlp_form(lp)
+        want = dict(zip(("row", "col", "val", "c", "rL", "rU", "l", "u"), (ri, ci, *arrs)))
+        for nm, w in want.items():
+            assert np.asarray(w).tobytes() == np.asarray(getattr(df, nm)).tobytes(), (lp.name, nm)
             assert np.asarray(getattr(hf, nm)).tobytes() == np.asarray(getattr(df, nm)).tobytes(), (lp.name, nm)
-        assert hm.row_perm.tobytes() == dm.row_perm.tobytes() and hm.row_signs.tobytes() == dm.row_signs.tobytes()
-        assert (hm.n_slack, hf.m, hf.n) == (dm.n_slack, df.m, df.n)
-    say("build_pdlp_form on the device: bit-identical on", len(original_lps(golden)), "LPs")
+        assert perm.tobytes() == dm.row_perm.tobytes() and signs.tobytes() == dm.row_signs.tobytes()
+        assert (int(dims[3]), int(dims[0]), int(dims[1])) == (dm.n_slack, df.m, df.n)
+        if lp.name in golden and golden[lp.name]["kind"] == "mps":
+            assert sha(df.row, df.col, df.val, df.c, df.rL, df.rU, df.l, df.u, dm.row_perm,
+                       dm.row_signs) == golden[lp.name]["form_sha"], lp.name
+    say("build_pdlp_form on the device: bit-identical to the oracle on", len(original_lps(golden)), "LPs")
 
 
-def test_verify_gpu_is_bit_identical(golden):
+def test_verify_gpu_is_bit_identical(golden, oracle):
+    """verify() with the products on the device against the oracle's verify
+    (and the host product's), all seven outputs bit for bit."""
     rng = np.random.default_rng(5)
     lps = original_lps(golden) + [dense_rows_lp(), P.gen_transport(1000, 1000, seed=42)]
     for lp in lps:
         x, y = rng.standard_normal(lp.n) * 3, rng.standard_normal(lp.m)
         h, d = P.verify(lp, x, y), P.verify_gpu(lp, x, y)
+        dv = [d.eps_P, d.eps_D, d.eps_G, d.objective, d.dual_objective, d.max_row_violation,
+              d.max_bound_violation]
+        assert bits_equal(oracle.verify(lp, x, y), dv), lp.name
         assert bits_equal([h.eps_P, h.eps_D, h.eps_G, h.objective, h.dual_objective,
-                           h.max_row_violation, h.max_bound_violation],
-                          [d.eps_P, d.eps_D, d.eps_G, d.objective, d.dual_objective,
-                           d.max_row_violation, d.max_bound_violation]), lp.name
+                           h.max_row_violation, h.max_bound_violation], dv), lp.name
 
 
 def host_pipeline(lp, tol, mode, **kw):

@@ -14,7 +14,7 @@ from conftest import golden_lp
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-OBJ_RTOL = 1e-5
+OBJ_RTOL = 1e-6
 # room for about two of the requests below at once, so admission waits
 CAPS = dict(max_cols=2600, max_rows=420, max_nnz=6000, max_instances=3, max_blocks=16)
 
@@ -103,3 +103,127 @@ def test_server_all_at_once_and_refusals(requests_):
         assert srv.step() == (0, 0, 0)
         assert srv.device_bytes > 0
     assert P.Server.request_bytes(27, 32, 83) > P.Server.request_bytes(27, 32, 83, P.MODE_FAST_FP64)
+
+
+# ------------------------------------------------------------- serving mix --
+# Requests from the serving workload's own family: the random equality LP
+# generator (6 nonzeros per column, feasible and bounded by construction) with
+# nnz log-uniform and m / n uniform in [0.2, 0.8], submitted from another
+# thread at Poisson arrival times while the server steps.
+C4_CAPS = dict(max_cols=1 << 22, max_rows=1 << 21, max_nnz=1 << 24, max_instances=16,
+               max_blocks=1024)
+C4_OBJ_RTOL = 1e-6
+
+
+def c4_request(nnz, ratio, seed):
+    n = max(20, int(nnz / 6))
+    m = max(6, int(ratio * n))
+    lp = P.gen_random_lp(m, n, 6, seed=seed)
+    s, Dr, Dc = P.scale(lp)
+    return lp, s, Dr, Dc
+
+
+def serve_poisson(reqs, mode, qps, seed):
+    """Submits reqs[i] at Poisson arrival times (another thread) while this
+    thread steps the server; returns the results in order and the largest
+    number of requests seen running at once."""
+    import time
+    arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / qps, len(reqs)))
+    with P.Server(mode=mode, tol=TOL, max_iterations=200000, **C4_CAPS) as srv:
+        tickets = [None] * len(reqs)
+        lock = threading.Lock()
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, q in enumerate(reqs):
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                t = srv.submit(*q)
+                with lock:
+                    tickets[i] = t
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        results, most, steps = {}, 0, 0
+        try:
+            while len(results) < len(reqs):
+                running, _, _ = srv.step()
+                most = max(most, running)
+                steps += 1
+                with lock:
+                    tk = list(tickets)
+                for i, t in enumerate(tk):
+                    if t is not None and i not in results:
+                        r = srv.result(t)
+                        if r is not None:
+                            results[i] = r
+                assert steps < 2000000, "server made no progress"
+        finally:
+            th.join()
+        return [results[i] for i in range(len(reqs))], most
+
+
+@pytest.fixture(scope="module")
+def c4_mix(oracle):
+    rng = np.random.default_rng(2024)
+    nnzs = np.exp(rng.uniform(np.log(1e4), np.log(1e5), 6))
+    ratios = rng.uniform(0.2, 0.8, 6)
+    reqs = [c4_request(z, r, 3000 + k) for k, (z, r) in enumerate(zip(nnzs, ratios))]
+    refs = []
+    for lp, _, _, _ in reqs:
+        o = oracle.scale(lp)
+        ref = oracle.pdlp(lp, lp.with_values(*o[:6]), o[6], o[7], 200000, 64, TOL)
+        assert ref.converged
+        refs.append(oracle.verify(lp, ref.x, ref.y)[3])
+    say("serving mix: m x n (nnz) =", [f"{q[0].m}x{q[0].n} ({q[0].nnz})" for q in reqs])
+    return reqs, refs
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_serving_mix_matches_oracle_and_alone(oracle, c4_mix, mode):
+    """Six requests of nnz 1e4-1e5 arriving at 40 requests/s: each result is
+    verified by the oracle (max eps <= 1e-6), its objective agrees with the
+    oracle's own solve to 1e-6 relative, and it is the same bits as the same
+    request served alone on a fresh server."""
+    reqs, refs = c4_mix
+    res, most = serve_poisson(reqs, mode, qps=40.0, seed=1)
+    assert most >= 2, "the arrivals never overlapped on the device"
+    errs = []
+    for (lp, _, _, _), r, ref_obj in zip(reqs, res, refs):
+        assert r.converged, (lp.m, lp.n, r.iterations)
+        rep = oracle.verify(lp, r.x, r.y)
+        assert max(rep[:3]) <= TOL, rep[:3]
+        errs.append(abs(rep[3] - ref_obj) / (1 + abs(ref_obj)))
+        assert errs[-1] <= C4_OBJ_RTOL, (lp.m, lp.n, rep[3], ref_obj)
+    for q, r in zip(reqs, res):
+        alone, _ = serve_poisson([q], mode, qps=1e9, seed=2)
+        assert alone[0].iterations == r.iterations
+        assert np.array_equal(alone[0].x, r.x) and np.array_equal(alone[0].y, r.y)
+    say(f"mode {mode}: {len(res)} requests, up to {most} running at once, attempts "
+        f"{[r.iterations for r in res]}, objective errors vs the oracle {['%.1e' % e for e in errs]}; "
+        "each identical alone")
+
+
+def test_serving_large_request_properties(oracle, c4_mix):
+    """A request of about 1.2M nonzeros (too large for an oracle solve here)
+    in the mix with two small ones: its point is verified by the oracle's
+    verifier, the mixed and fast-FP64 servers agree on the objective to
+    1e-6 relative, and it is the same bits alone as in the mix."""
+    big = c4_request(1.2e6, 0.5, 77)
+    lp = big[0]
+    small = c4_mix[0][:2]
+    objs = {}
+    for mode in (P.MODE_MIXED, P.MODE_FAST_FP64):
+        res, most = serve_poisson([small[0], big, small[1]], mode, qps=40.0, seed=3)
+        r = res[1]
+        assert r.converged and most >= 2
+        rep = oracle.verify(lp, r.x, r.y)
+        assert max(rep[:3]) <= TOL, rep[:3]
+        objs[mode] = rep[3]
+        alone, _ = serve_poisson([big], mode, qps=1e9, seed=4)
+        assert alone[0].iterations == r.iterations
+        assert np.array_equal(alone[0].x, r.x) and np.array_equal(alone[0].y, r.y)
+        say(f"mode {mode}: {lp.m}x{lp.n} ({lp.nnz} nnz) in {r.iterations} attempts, eps {max(rep[:3]):.2e}")
+    a, b = objs[P.MODE_MIXED], objs[P.MODE_FAST_FP64]
+    assert abs(a - b) / (1 + abs(b)) <= C4_OBJ_RTOL, (a, b)

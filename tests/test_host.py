@@ -131,3 +131,14 @@ def test_blend_generator_structure(oracle):
     assert out.converged
     sizes = P.gen_blend_sizes(512)
     assert len(sizes) == 512 and all(20 <= k <= 400 and 10 <= p <= 200 for k, p in sizes)
+
+
+@pytest.mark.parametrize("name", ["afiro", "share1b", "tiny_range", "med", "israel"])
+def test_extract_solution_bit_identical_to_oracle(oracle, name):
+    lp, _ = load_fixture(name)
+    form, meta = P.build_pdlp_form(lp)
+    rng = np.random.default_rng(11)
+    x2, y2 = rng.standard_normal(form.n), rng.standard_normal(form.m)
+    x, y = P.extract_solution(meta, x2, y2)
+    xo, yo = oracle.extract_solution(lp.n, lp.m, meta.row_perm, meta.row_signs, x2, y2)
+    assert np.array_equal(x, xo) and np.array_equal(y, yo)

@@ -242,3 +242,109 @@ def test_stop_at_iteration_limit_returns_best_point(oracle):
     assert not out.converged and out.iterations == 300 and out.has_point
     rep = oracle.verify(lp, out.x, out.y)
     assert [out.eps_P, out.eps_D, out.eps_G] == [rep[0], rep[1], rep[2]]
+
+
+# ---------------------------------------------------------- extract_solution --
+
+@pytest.fixture(scope="module")
+def extract_golden():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "extract.json")) as f:
+        return {c["name"]: c for c in json.load(f)["cases"]}
+
+
+EXTRACT_CASES = ["afiro", "adlittle", "blend", "sc50a", "sc50b", "sc105", "kb2", "scagr7",
+                 "israel", "share1b", "beaconfd", "tiny_range", "med", "illcond"]
+
+
+@pytest.mark.parametrize("name", EXTRACT_CASES)
+def test_extract_solution_matches_reference_golden(oracle, golden, extract_golden, name):
+    """The reference's extract_solution() on its own solution of each MPS case
+    (tests/golden/extract.json, through oracle/_ref): the oracle's
+    extract of the oracle's (bit-identical) solution gives the same x, y bits,
+    and the oracle's verify() of them on the original LP the same report."""
+    case, want = golden[name], extract_golden[name]
+    lp, _ = load_fixture(name)
+    dims, ri, ci, arrs, perm, signs = oracle.pdlp_form(lp)
+    form = P.LP(int(dims[0]), int(dims[1]), ri, ci, *arrs, lp.obj_offset, name)
+    s = oracle.scale(form)
+    r = case["ref"]
+    out = oracle.pdlp(form, form.with_values(*s[:6]), s[6], s[7], r["max_iterations"],
+                      r["check_every"], r["tol"])
+    x, y = oracle.extract_solution(lp.n, lp.m, perm, signs, out.x, out.y)
+    assert (len(x), len(y)) == (want["n_orig"], want["m_orig"])
+    assert sha(x) == want["x_sha"] and sha(y) == want["y_sha"]
+    assert list(oracle.verify(lp, x, y)) == want["verify_on_lp"]
+
+
+def test_extract_solution_closed_form(oracle):
+    """Hand-built form metadata: 3 original columns plus 2 slack columns, 4
+    original rows of which row 1 is free (dropped from the form), the others
+    permuted with one sign flip.  x keeps the first n_orig entries; y[perm[p]]
+    = sign[p] y2[p]; the free row's multiplier is 0."""
+    perm = np.array([2, 0, 3], dtype=np.int32)
+    signs = np.array([1.0, -1.0, 1.0])
+    x2 = np.array([1.5, -2.0, 3.25, 9.0, 8.0])
+    y2 = np.array([0.5, 7.0, -4.0])
+    x, y = oracle.extract_solution(3, 4, perm, signs, x2, y2)
+    assert np.array_equal(x, [1.5, -2.0, 3.25])
+    assert np.array_equal(y, [-7.0, 0.0, 0.5, -4.0])
+    # a slip that indexes y by the form's position instead of row_perm fails here
+    assert not np.array_equal(y, np.concatenate([signs * y2, [0.0]]))
+
+
+def test_extract_solution_matches_live_reference(oracle, ref):
+    rng = np.random.default_rng(5)
+    for m_orig, n_orig, m2, n2 in ((7, 5, 6, 9), (30, 40, 30, 44), (1, 1, 1, 1)):
+        perm = rng.permutation(m_orig)[:m2].astype(np.int32)
+        signs = rng.choice([-1.0, 1.0], m2)
+        x2, y2 = rng.standard_normal(n2), rng.standard_normal(m2)
+        a = oracle.extract_solution(n_orig, m_orig, perm, signs, x2, y2)
+        b = ref.extract_solution(n_orig, m_orig, perm, signs, x2, y2)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# -------------------------------------------------------------- scalar rules --
+# orc_pdlp takes every step and check decision through these functions, so the
+# golden trajectories above pin them; these closed forms name what they compute.
+
+def test_step_rule_closed_forms(oracle):
+    # inter = 0: eta_bar infinite, accepted, eta grows by (1 + k1^-0.6)
+    acc, eb, en = oracle.step_rule(0.5, 2.0, 0, 1.0, 1.0, 0.0)
+    assert acc and eb == float("inf") and en == (1.0 + 2.0 ** -0.6) * 0.5
+    # eta_bar = (omega dx2 + dy2 / omega) / (2 |inter|) = (2*3 + 8/2) / (2*5) = 1
+    acc, eb, en = oracle.step_rule(0.75, 2.0, 8, 3.0, 8.0, -5.0)
+    assert acc and eb == 1.0
+    assert en == min((1.0 - 10.0 ** -0.3) * 1.0, (1.0 + 10.0 ** -0.6) * 0.75)
+    # eta > eta_bar: rejected, and the next eta is the shrunk eta_bar
+    acc, eb, en = oracle.step_rule(1.5, 2.0, 8, 3.0, 8.0, 5.0)
+    assert not acc and en == (1.0 - 10.0 ** -0.3) * 1.0
+    # eta == eta_bar is accepted (<=)
+    assert oracle.step_rule(1.0, 2.0, 8, 3.0, 8.0, 5.0)[0]
+
+
+def test_check_rule_closed_forms(oracle):
+    inf = float("inf")
+    # converged on the candidate with the smaller merit (the average here)
+    r = oracle.check_rule([1e-5, 0, 0], [1e-7, 2e-7, 0], 1e-6, 3, 10, inf, inf, inf)
+    assert r["converged"] and r["use_avg"] and r["best_which"] == 2
+    # first check: merit_last_restart = inf, so "sufficient" restarts
+    r = oracle.check_rule([3e-3, 4e-3, 0], [1.0, 0, 0], 1e-6, 3, 10, inf, inf, inf)
+    assert not r["converged"] and not r["use_avg"] and r["restart"]
+    assert r["cand_merit"] == 5e-3 and r["best_merit"] == 5e-3 and r["best_which"] == 1
+    assert r["merit_prev_cand"] == 5e-3
+    # no restart: merit above 0.8 x the last restart's, few steps since
+    r = oracle.check_rule([0.9, 0, 0], [1.0, 0, 0], 1e-6, 3, 10, 1.0, 0.5, 0.7)
+    assert not r["restart"] and r["best_which"] == 0 and r["best_merit"] == 0.5
+    # necessary: <= 0.8 x last and worse than the previous candidate
+    assert oracle.check_rule([0.7, 0, 0], [1.0, 0, 0], 1e-6, 3, 10, 1.0, 0.5, 0.6)["restart"]
+    # artificial: since_restart >= 0.36 accepted
+    assert oracle.check_rule([0.9, 0, 0], [1.0, 0, 0], 1e-6, 36, 100, 1.0, 0.5, 0.7)["restart"]
+    assert not oracle.check_rule([0.9, 0, 0], [1.0, 0, 0], 1e-6, 35, 100, 1.0, 0.5, 0.7)["restart"]
+
+
+def test_primal_weight_closed_forms(oracle):
+    import math
+    assert oracle.primal_weight(4.0, 2.0, 8.0) == math.exp(0.5 * math.log(4.0) + 0.5 * math.log(4.0))
+    assert oracle.primal_weight(4.0, 1e-11, 8.0) == 4.0 and oracle.primal_weight(4.0, 2.0, 0.0) == 4.0
