@@ -4,37 +4,43 @@ nnz = 2,000,000), on one MI355X.
 
 One step = one full PDLP solve from the cold start x = clip(0, l, u), y = 0 to
 max(eps_P, eps_D, eps_G) <= 1e-6, with the LP already resident in HBM (an
-engine created once: upload, layout and SpMV autotuning are outside the timed
-region).  Each result is certified afterwards by the host FP64 verifier on
-the unscaled problem.  Prints ONE JSON line (see the keys below).
+engine created once: upload and layout are outside the timed region).  Each
+result is certified afterwards by the host FP64 verifier on the unscaled
+problem.  Prints ONE JSON line (see the keys below).
 
-Besides the headline (mixed FP32-iterate / FP64-residual engine) it reports:
-  * the same solve in the device-resident FP64 mode and in the plain FP64 mode
-    (the reference GPU backend's design: host sync per attempt, host KKT);
+The headline is the device-resident FP64 mode (the reference's arithmetic
+width); beside it:
+  * the mixed FP32-iterate / FP64-residual mode;
+  * "one_shot": the drop-in call supr_solve_pdhg_gpu (upload, layout and
+    teardown included, inputs in host memory) in each mode, against the plain
+    FP64 mode's one-shot call (the reference GPU backend's design: host sync
+    per attempt, host KKT);
   * the two iteration kernels' duration (HIP events, standalone launches at
-    this LP's shapes) and achieved bandwidth from algorithmic bytes;
-  * the reference's own CPU PDLP (oracle/_ref, 1 thread) on a bounded prefix of
-    the same solve, extrapolated with the reference's full iteration count;
+    this LP's shapes) and achieved bandwidth from algorithmic bytes, with the
+    rocprofv3 average of the same kernels from the committed profile;
+  * the reference's own CPU PDLP (oracle/_ref, 1 thread) run to completion on
+    the same LP;
   * "batched": 512 blending LPs (20-400 components x 10-200 products,
     log-uniform sizes, n 230 to 73,117) solved together in the mixed mode
     with the tuned layout, every result verified; solves/s against
     sequential plain-FP64 and sequential resident-mixed solves of a
     stratified sample of the same instances;
   * "serving": an open-loop Poisson stream of random LPs (6 nonzeros per
-    column, m/n in [0.2, 0.8], nnz log-uniform) through the continuous-
-    batching server, every result verified; p50/p99 latency and solves/s,
-    against one-at-a-time plain FP64 solves of the same requests (their
-    latency from a FIFO queue over the same arrivals);
+    column, m/n in [0.2, 0.8], nnz log-uniform over [1e4, 2e7]) through the
+    continuous-batching server, every result verified; p50/p99 latency and
+    solves/s, against one-at-a-time plain FP64 solves of the same requests
+    (their latency from a FIFO queue over the same arrivals);
   * "c0": the random equality LP m = 2,000, n = 5,000 in each mode, beside the
     reference's own CPU PDLP run to completion;
-  * "polish_1e-8": the same LP to a verified 1e-8 (the FP64 polish);
+  * "large": the power-law LP m = 500,000, n = 2,000,000 (35.1M nonzeros):
+    its kernels and one FP64 solve to a verified 1e-6;
+  * "polish_1e-8": the 1M-variable LP to a verified 1e-8 (the FP64 polish);
   * each mode's objective error against the reference CPU PDLP's;
   * "ingest": the same 1M-variable LP as an MPS file, read by this library's
     reader and by the reference's (same LP bit for bit), and file -> verified
     solution through solve_lp_gpu;
   * "prep": scale() on the device against the host version and the
-    reference's, bit for bit, at 2M and 7M nonzeros;
-  * with --large: the power-law LP (m = 500,000, n = 2,000,000, 35.1M nnz).
+    reference's, bit for bit, at 2M and 7M nonzeros.
 """
 import argparse
 import json
@@ -66,22 +72,23 @@ def nonneg_bounds(lp, s):
     return bool(np.all((l == 0) & ~np.signbit(l)) and np.all(np.isposinf(u)))
 
 
-def kernel_bytes(lp, s, ell_AT, ell_A, nonneg=False):
+def kernel_bytes(lp, s, ell_AT, ell_A, nonneg=False, store_aty=False):
     """Algorithmic bytes of one launch of each iteration kernel (every array
     read or written once), value/vector size s (4: FP32 loop, 8: FP64 loop).
       primal half (rows of A^T): A^T indices + values, row pointers unless ELL,
         y once, then per column x, c, l, u read (l, u not when nonneg: every
-        l = +0, u = +inf), xsum read+write, aty and xn written, and in the
-        FP32 loop the (xn, xn - x) pair written
-        -> nnz (4 + s) + [4 (n + 1)] + s m + (8 | 6) s n  [+ 8 n]
+        l = +0, u = +inf), xsum read+write, xn written (and A^T y written where
+        it is stored: the batched engine), and in the FP32 loop the (xn, xn - x)
+        pair written
+        -> nnz (4 + s) + [4 (n + 1)] + s m + (7 | 5) s n [+ s n] [+ 8 n]
       dual half (rows of A): A indices + values, row pointers unless ELL, the
         gathered vector once (xn; the (xn, xn - x) pairs in the FP32 loop),
         then per row y, ax, b read, ysum read+write, axn and yn written, the
         inequality flag -> nnz (4 + s) + [4 (m + 1)] + (s | 8) n + 7 s m + m"""
     n, m, nnz = lp.n, lp.m, lp.nnz
     pair = s == 4
-    primal = (nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + (6 if nonneg else 8) * s * n
-              + (8 * n if pair else 0))
+    primal = (nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + (5 if nonneg else 7) * s * n
+              + (s * n if store_aty else 0) + (8 * n if pair else 0))
     dual = nnz * (4 + s) + (0 if ell_A else 4 * (m + 1)) + (8 if pair else s) * n + 7 * s * m + m
     return primal, dual
 
@@ -90,8 +97,8 @@ def moved_bytes(lp, attempts, attempts_fp32):
     """Algorithmic bytes the iteration kernels move over a solve of lp (CSR
     layout, as in the batched and served engines): FP32-loop attempts at the
     FP32 figure, the rest at the FP64 one."""
-    p32, d32 = kernel_bytes(lp, 4, False, False)
-    p64, d64 = kernel_bytes(lp, 8, False, False)
+    p32, d32 = kernel_bytes(lp, 4, False, False, store_aty=True)
+    p64, d64 = kernel_bytes(lp, 8, False, False, store_aty=True)
     return attempts_fp32 * (p32 + d32) + (attempts - attempts_fp32) * (p64 + d64)
 
 
@@ -107,10 +114,11 @@ def cpu_per_attempt(U, S, Dr, Dc, sample_attempts):
     return (time.perf_counter() - t0) / max(1, out.iterations), ("reference" if kind == "ref" else "port")
 
 
-def cpu_baseline(lp, scaled, Dr, Dc, sample_attempts):
-    """Reference CPU PDLP (oracle/_ref) for a bounded prefix of the solve,
-    extrapolated with the full run's iteration count (tests/golden/
-    c1_reference.json, written by the same reference build)."""
+def cpu_baseline(lp, scaled, Dr, Dc):
+    """The reference's own CPU PDLP (oracle/_ref: src/pdlp.cpp + include/
+    pdlp_algo.hpp compiled from the reference's sources; the oracle's
+    restatement where _ref is absent), one thread, run to completion on this
+    LP from the same scaled form, timed around the engine call only."""
     try:
         sys.path.insert(0, ROOT)
         from oracle import Checker, available
@@ -119,24 +127,17 @@ def cpu_baseline(lp, scaled, Dr, Dc, sample_attempts):
         return None
     kind = "ref" if available("ref") else "oracle"
     chk = Checker(kind)
-    ref_path = os.path.join(ROOT, "tests", "golden", "c1_reference.json")
-    full = None
-    if os.path.exists(ref_path):
-        with open(ref_path) as f:
-            full = json.load(f)
     t0 = time.perf_counter()
-    out = chk.pdlp(lp, scaled, Dr, Dc, sample_attempts, 64, TOL)
+    out = chk.pdlp(lp, scaled, Dr, Dc, 200000, 64, TOL)
     dt = time.perf_counter() - t0
-    per_attempt = dt / max(1, out.iterations)
-    attempts = full["iterations"] if full else None
-    value = per_attempt * attempts if attempts else None
+    rep = P.verify(lp, out.x, out.y)
     return {
-        "value": value, "unit": "s", "cores": 1,
+        "value": dt, "unit": "s", "cores": 1,
         "kind": "reference" if kind == "ref" else "port",
-        "sample": (f"first {out.iterations} attempts ({out.iterations // 64} host KKT checks) of the "
-                   f"{'reference' if kind == 'ref' else 'oracle'} CPU PDLP on this LP: {dt:.2f} s, "
-                   f"{per_attempt * 1e3:.2f} ms/attempt, extrapolated to the full reference run's "
-                   f"{attempts} attempts"),
+        "sample": (f"the whole {'reference' if kind == 'ref' else 'oracle'} CPU PDLP solve of this LP to "
+                   f"1e-6 ({out.iterations} attempts, converged={out.converged}), one thread"),
+        "attempts": out.iterations, "objective": rep.objective,
+        "verified_eps": [rep.eps_P, rep.eps_D, rep.eps_G],
         "host_cpus": os.cpu_count(),
     }
 
@@ -279,15 +280,21 @@ def bench_batched(count=512, sample=8):
     }
 
 
-def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=None):
+def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=None, span=False,
+                  saturated=True):
     """Open-loop Poisson stream through the server against one-at-a-time
     plain FP64 solves of the same requests.  plain_sample_attempts: time the
     plain solves on that many attempts each and scale by the request's
     attempt count in the server (for requests too large to solve plainly in
-    a bench run; the output says so)."""
+    a bench run; the output says so).  span: the requests' nnz are spread
+    log-evenly over [1e4, nnz_hi] (both ends included), in random order.
+    saturated: also serve every request submitted at once."""
     import threading
     rng = np.random.default_rng(seed)
-    nnzs = np.exp(rng.uniform(np.log(1e4), np.log(nnz_hi), count))
+    if span:
+        nnzs = rng.permutation(np.geomspace(1e4, nnz_hi, count))
+    else:
+        nnzs = np.exp(rng.uniform(np.log(1e4), np.log(nnz_hi), count))
     ratios = rng.uniform(0.2, 0.8, count)
     arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
     reqs = []
@@ -353,8 +360,9 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
                     achieved_GBps=moved / float(done.max()) / 1e9,
                     frac_of_hbm_peak=moved / float(done.max()) / 1e9 / HBM_PEAK_GBS)
 
-    sat = serve(False)
     ol = serve(True)
+    sat = serve(False) if saturated else None
+    base = sat if sat is not None else ol
     svc = []
     for i in range(count):
         t0 = time.perf_counter()
@@ -362,7 +370,7 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
             r = P.solve_pdhg_gpu(*reqs[i], tol=TOL, mode=P.MODE_PLAIN_FP64,
                                  max_iterations=plain_sample_attempts)
             per = (time.perf_counter() - t0) / max(1, r.iterations)
-            svc.append(per * sat["attempts"][i])
+            svc.append(per * base["attempts"][i])
         else:
             r = P.solve_pdhg_gpu(*reqs[i], tol=TOL, mode=P.MODE_PLAIN_FP64)
             svc.append(time.perf_counter() - t0)
@@ -381,8 +389,8 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
     try:
         est = []
         for i in range(count):
-            per, kind = cpu_per_attempt(*reqs[i], min(300, sat["attempts"][i]))
-            est.append(per * sat["attempts"][i])
+            per, kind = cpu_per_attempt(*reqs[i], min(300, base["attempts"][i]))
+            est.append(per * base["attempts"][i])
         cores = os.cpu_count() or 1
         cpu = dict(kind=kind, cores=1, solves_per_s=count / float(np.sum(est)),
                    all_cores_bound_solves_per_s=cores * count / float(np.sum(est)), host_cpus=cores,
@@ -390,15 +398,19 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
                           "in the server; all_cores_bound = host_cpus x the one-thread rate")
     except Exception as e:  # pragma: no cover
         log(f"serving: CPU baseline unavailable: {e}")
-    log(f"serving: saturated {sat['solves_per_s']:.2f} solves/s; open loop {qps} qps p50 "
+    log(f"serving: " + (f"saturated {sat['solves_per_s']:.2f} solves/s; " if sat else "")
+        + f"open loop {qps} qps {ol['solves_per_s']:.3f} solves/s p50 "
         f"{ol['latency_p50']:.2f} s p99 {ol['latency_p99']:.2f} s; plain one-at-a-time "
         f"{plain['solves_per_s']:.3f} solves/s, p50 {plain['latency_p50']:.1f} s p99 {plain['latency_p99']:.1f} s")
+    spread = "log-evenly spread over" if span else "log-uniform in"
     return {
-        "workload": f"{count} random LPs (6 nnz/column, m/n U[0.2,0.8], nnz log-uniform [1e4, {nnz_hi:.0e}], "
+        "workload": f"{count} random LPs (6 nnz/column, m/n U[0.2,0.8], nnz {spread} [1e4, {nnz_hi:.0e}], "
                     f"seed {seed}), Poisson arrivals at {qps} qps, mixed mode, tol 1e-6, every result verified",
+        "request_nnz": [int(r[0].nnz) for r in reqs],
         "total_nnz": int(sum(r[0].nnz for r in reqs)),
         "saturated": sat, "open_loop": ol, "plain_fp64_one_at_a_time": plain,
-        "speedup_solves_per_s_vs_plain": sat["solves_per_s"] / plain["solves_per_s"],
+        "speedup_solves_per_s_vs_plain": base["solves_per_s"] / plain["solves_per_s"],
+        "speedup_basis": "saturated" if sat else "open loop (makespan of the stream)",
         "cpu_baseline": cpu,
     }
 
@@ -559,14 +571,15 @@ def bench_prep():
     return out
 
 
-def bench_large_kernels(reps=50):
-    """The power-law LP's iteration kernels alone (HIP events), with their
-    algorithmic bytes -- the run to profile them."""
+def large_lp():
     lp = P.gen_power_law_lp(500000, 2000000, seed=2026)
     s, Dr, Dc = P.scale(lp)
+    return lp, s, Dr, Dc
+
+
+def large_kernels(lp, s, Dr, Dc, reps=50):
     kb = P.bench_kernels(lp, s, Dr, Dc, reps=reps)
-    out = {"workload": f"power-law LP m=500000 n=2000000 nnz={lp.nnz}", "lanes_A": kb["lanes_A"],
-           "lanes_AT": kb["lanes_AT"], "kernels": {}}
+    out = {"lanes_A_fp64": kb["lanes_A"], "lanes_AT_fp64": kb["lanes_AT"], "kernels": {}}
     for name, key, sz in (("k_primal_f32", "primal_ms_fp32", 4), ("k_dual_f32", "dual_ms_fp32", 4),
                           ("k_primal_f64", "primal_ms_fp64", 8), ("k_dual_f64", "dual_ms_fp64", 8)):
         prim, dual = kernel_bytes(lp, sz, False, False, nonneg_bounds(s, sz))
@@ -576,20 +589,101 @@ def bench_large_kernels(reps=50):
     return out
 
 
-def bench_large():
-    """The power-law LP: mixed mode to 1e-6 (verified), kernel timing."""
-    lp = P.gen_power_law_lp(500000, 2000000, seed=2026)
-    s, Dr, Dc = P.scale(lp)
-    kb = P.bench_kernels(lp, s, Dr, Dc, reps=50)
-    eng = P.Engine(lp, s, Dr, Dc, tol=TOL, mode=P.MODE_MIXED, max_iterations=400000, verbose=2)
-    r = eng.solve()
-    eng.close()
+def bench_large_kernels(reps=50):
+    """The power-law LP's iteration kernels alone (HIP events), with their
+    algorithmic bytes -- the run to profile them."""
+    lp, s, Dr, Dc = large_lp()
+    out = large_kernels(lp, s, Dr, Dc, reps)
+    out["workload"] = f"power-law LP m=500000 n=2000000 nnz={lp.nnz}"
+    return out
+
+
+def bench_large(mode=P.MODE_FAST_FP64):
+    """The power-law LP (m = 500,000, n = 2,000,000, Zipf 1.5 column degrees
+    on [2, 200]: 35.1M nonzeros, magnitudes log-uniform in [1e-3, 1e3]): its
+    iteration kernels, and one solve to a verified 1e-6 in the FP64 mode
+    (progress lines every 5 s on stderr)."""
+    log("large: generating and scaling the power-law LP 500000 x 2000000 (host)")
+    lp, s, Dr, Dc = large_lp()
+    out = {"workload": f"power-law LP m=500000 n=2000000 nnz={lp.nnz} (Zipf 1.5 on [2,200] column degrees, "
+                       "magnitudes log-uniform [1e-3,1e3], seed 2026), tol 1e-6 verified"}
+    out.update(large_kernels(lp, s, Dr, Dc))
+    name = "fast_fp64" if mode == P.MODE_FAST_FP64 else "mixed"
+    log(f"large: {name} solve")
+    with P.Engine(lp, s, Dr, Dc, tol=TOL, mode=mode, max_iterations=400000, verbose=2) as eng:
+        t0 = time.perf_counter()
+        r = eng.solve()
+        dt = time.perf_counter() - t0
+        setup = r.info["seconds_setup"]
     rep = P.verify(lp, r.x, r.y)
-    return {"workload": f"power-law LP m=500000 n=2000000 nnz={lp.nnz} (Zipf 1.5 on [2,200] column degrees, "
-                        "magnitudes log-uniform [1e-3,1e3]), mixed mode, tol 1e-6",
-            "converged": r.converged, "attempts": r.iterations, "fp32_attempts": r.info["iterations_fp32"],
-            "seconds_solve": r.info["seconds_solve"], "verified_eps": [rep.eps_P, rep.eps_D, rep.eps_G],
-            "objective": rep.objective, "kernels_ms": kb}
+    worst = max(rep.eps_P, rep.eps_D, rep.eps_G)
+    if not r.converged or worst > TOL:
+        raise SystemExit(f"large: not verified (converged={r.converged}, eps {worst:.3e})")
+    out["mode"] = name
+    out.update(converged=r.converged, attempts=r.iterations, fp32_attempts=r.info["iterations_fp32"],
+               seconds=dt, seconds_per_attempt=dt / max(1, r.iterations),
+               verified_eps=[rep.eps_P, rep.eps_D, rep.eps_G], objective=rep.objective,
+               engine_setup_seconds=setup)
+    moved = moved_bytes_single(lp, s, r.iterations, r.info["iterations_fp32"])
+    out["roofline"] = {"bound": "hbm", "achieved": moved / dt / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": moved / dt / 1e9 / HBM_PEAK_GBS,
+                       "note": "algorithmic bytes of the iteration kernels over the whole solve's wall time"}
+    log(f"large: {name} {dt:.1f} s, {r.iterations} attempts, eps {worst:.2e}")
+    return out
+
+
+def moved_bytes_single(lp, s, attempts, attempts_fp32):
+    """Algorithmic bytes the single-LP iteration kernels move over a solve."""
+    p32, d32 = kernel_bytes(lp, 4, False, False, nonneg_bounds(s, 4))
+    p64, d64 = kernel_bytes(lp, 8, False, False, nonneg_bounds(s, 8))
+    return attempts_fp32 * (p32 + d32) + (attempts - attempts_fp32) * (p64 + d64)
+
+
+def bench_one_shot(lp, scaled, Dr, Dc, reps=3):
+    """The drop-in call as the reference's callers make it: supr_solve_pdhg_gpu
+    from host arrays, upload, layout, solve, result copy and teardown
+    included, per mode; the plain FP64 mode (the reference GPU design) the
+    same way, once."""
+    out = {}
+    for name, mode in (("fast_fp64", P.MODE_FAST_FP64), ("mixed", P.MODE_MIXED)):
+        P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=mode)   # warm (allocator, code objects)
+        ts, r = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=mode)
+            ts.append(time.perf_counter() - t0)
+        rep = P.verify(lp, r.x, r.y)
+        if not r.converged or max(rep.eps_P, rep.eps_D, rep.eps_G) > TOL:
+            raise SystemExit(f"one-shot {name}: not verified")
+        out[name] = dict(seconds=float(np.mean(ts)), min_seconds=float(np.min(ts)),
+                         setup_seconds=r.info["seconds_setup"], solve_seconds=r.info["seconds_solve"],
+                         attempts=r.iterations, objective=rep.objective)
+    t0 = time.perf_counter()
+    r = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=P.MODE_PLAIN_FP64)
+    wall = time.perf_counter() - t0
+    rep = P.verify(lp, r.x, r.y)
+    out["plain_fp64"] = dict(seconds=wall, setup_seconds=r.info["seconds_setup"],
+                             solve_seconds=r.info["seconds_solve"], attempts=r.iterations,
+                             converged=r.converged, verified_eps=[rep.eps_P, rep.eps_D, rep.eps_G],
+                             objective=rep.objective)
+    for name in ("fast_fp64", "mixed"):
+        out[name]["speedup_vs_plain_one_shot"] = wall / out[name]["seconds"]
+    log(f"one-shot: fast64 {out['fast_fp64']['seconds'] * 1e3:.1f} ms, mixed {out['mixed']['seconds'] * 1e3:.1f} ms, "
+        f"plain {wall * 1e3:.0f} ms -> {out['fast_fp64']['speedup_vs_plain_one_shot']:.1f}x / "
+        f"{out['mixed']['speedup_vs_plain_one_shot']:.1f}x")
+    return out
+
+
+def rocprof_average_us(kernel):
+    """Average duration (us) of `kernel` in the committed rocprofv3 kernel
+    statistics of the 1M-variable LP's iteration kernels, or None."""
+    path = os.path.join(ROOT, "profiles", "iteration_kernels_transport_1000x1000.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return float(json.loads(block)["kernels"][kernel]["avg_us"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -597,25 +691,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=256, help="attempts of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-full", action="store_true",
-                    help="serving stream over the full nnz range [1e4, 2e7] (long)")
-    ap.add_argument("--large", action="store_true", help="also solve the power-law LP (minutes)")
+    ap.add_argument("--no-large", action="store_true", help="skip the power-law LP leg (about 3 minutes)")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--no-prep", action="store_true")
-    ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-kernels",
-                                       "ingest", "prep"],
+    ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-mixed",
+                                       "large-kernels", "ingest", "prep", "one-shot"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain device copy of known size (reference for HBM counters)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone iteration-kernel timing (for profiling it)")
     ap.add_argument("--lanes", default=None,
-                    help="A,AT lanes per row for --kernels-only (default: autotuned)")
+                    help="A,AT lanes per row for --kernels-only (default: the shape rule)")
+    ap.add_argument("--solve-only", default=None, choices=["fast_fp64", "mixed"],
+                    help="only resident-engine solves of the 1M-variable LP in this mode (for tracing the loop; "
+                         "the attempts are launched on the stream directly, 16 per host poll)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -633,12 +726,13 @@ def main():
                           "ms_per_copy": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
 
-    if args.only:
+    serving_full = lambda: bench_serving(count=6, qps=0.5, nnz_hi=2e7, seed=11, span=True,  # noqa: E731
+                                         saturated=False, plain_sample_attempts=1000)
+    if args.only and args.only != "one-shot":
         fn = {"c0": bench_c0, "batched": bench_batched, "serving": bench_serving, "large": bench_large,
-              "ingest": bench_ingest, "prep": bench_prep,
-              "large-kernels": bench_large_kernels,
-              "serving-full": lambda: bench_serving(count=16, qps=0.5, nnz_hi=2e7,
-                                                    plain_sample_attempts=1000)}[args.only]
+              "large-mixed": lambda: bench_large(P.MODE_MIXED), "ingest": bench_ingest,
+              "prep": bench_prep, "large-kernels": bench_large_kernels,
+              "serving-full": serving_full}[args.only]
         print(json.dumps({args.only: fn()}))
         return
 
@@ -661,8 +755,22 @@ def main():
     ws, wdr, wdc = P.scale(warm)
     P.solve_pdhg_gpu(warm, ws, wdr, wdc, tol=TOL, mode=P.MODE_MIXED)
 
+    if args.only == "one-shot":
+        print(json.dumps({"one_shot": bench_one_shot(lp, scaled, Dr, Dc)}))
+        return
+    if args.solve_only:
+        mode = P.MODE_FAST_FP64 if args.solve_only == "fast_fp64" else P.MODE_MIXED
+        with P.Engine(lp, scaled, Dr, Dc, tol=TOL, mode=mode, graph_attempts=-16) as eng:
+            for k in range(args.warmup + args.steps):
+                t0 = time.perf_counter()
+                r = eng.solve()
+                log(f"{args.solve_only} direct-launch solve {k}: {(time.perf_counter() - t0) * 1e3:.2f} ms, "
+                    f"{r.iterations} attempts")
+        print(json.dumps({"solve_only": args.solve_only, "attempts": r.iterations}))
+        return
+
     results = {}
-    for name, mode in (("mixed", P.MODE_MIXED), ("fast_fp64", P.MODE_FAST_FP64)):
+    for name, mode in (("fast_fp64", P.MODE_FAST_FP64), ("mixed", P.MODE_MIXED)):
         t0 = time.perf_counter()
         eng = P.Engine(lp, scaled, Dr, Dc, tol=TOL, mode=mode)
         setup_s = time.perf_counter() - t0
@@ -682,10 +790,14 @@ def main():
             log(f"{name} step {k}: {times[-1] * 1e3:.2f} ms, {res.iterations} attempts "
                 f"(fp32 {res.info['iterations_fp32']}), verified eps {worst:.2e}, obj {rep.objective:.6f}")
         eng.close()
+        if len(set(iters)) != 1:
+            raise SystemExit(f"{name}: attempt counts differ between runs {sorted(set(iters))}")
         results[name] = dict(seconds=float(np.mean(times)), min_seconds=float(np.min(times)),
                              iterations=int(np.median(iters)), objective=rep.objective,
                              eps=[rep.eps_P, rep.eps_D, rep.eps_G], setup_seconds=setup_s,
-                             fp32_iterations=res.info["iterations_fp32"])
+                             fp32_iterations=res.info["iterations_fp32"],
+                             lanes=[res.info["lanes_A"], res.info["lanes_AT"]],
+                             us_per_attempt=float(np.mean(times)) / res.iterations * 1e6)
 
     ref_obj = None
     ref_path = os.path.join(ROOT, "tests", "golden", "c1_reference.json")
@@ -695,52 +807,42 @@ def main():
         for r in results.values():
             r["objective_rel_error_vs_cpu_reference"] = abs(r["objective"] - ref_obj) / (1 + abs(ref_obj))
 
+    one_shot = bench_one_shot(lp, scaled, Dr, Dc)
+    plain = one_shot["plain_fp64"]
     polish = bench_polish(lp, scaled, Dr, Dc)
-
-    plain = None
-    if not args.no_plain:
-        log("plain FP64 mode (host-driven, reference GPU design): one solve")
-        t0 = time.perf_counter()
-        res = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=P.MODE_PLAIN_FP64)
-        wall = time.perf_counter() - t0
-        rep = P.verify(lp, res.x, res.y)
-        plain = dict(seconds=res.info["seconds_solve"], wall_with_setup=wall,
-                     iterations=res.iterations, converged=res.converged,
-                     eps=[rep.eps_P, rep.eps_D, rep.eps_G], objective=rep.objective)
-        log(f"plain: {plain['seconds']:.3f} s solve, {res.iterations} attempts")
 
     log("timing the iteration kernels (HIP events, 200 launches each)")
     kb = P.bench_kernels(lp, scaled, Dr, Dc, reps=200)
     prim32, dual32 = kernel_bytes(lp, 4, kb["ell_AT"] > 0, kb["ell_A"] > 0, nonneg_bounds(scaled, 4))
     prim64, dual64 = kernel_bytes(lp, 8, kb["ell_AT"] > 0, kb["ell_A"] > 0, nonneg_bounds(scaled, 8))
+    la, lat = int(kb["lanes_A"]), int(kb["lanes_AT"])
     kernels = {
         "k_primal_f32": dict(ms=kb["primal_ms_fp32"], bytes=prim32),
         "k_dual_f32": dict(ms=kb["dual_ms_fp32"], bytes=dual32),
-        "k_primal_f64": dict(ms=kb["primal_ms_fp64"], bytes=prim64),
-        "k_dual_f64": dict(ms=kb["dual_ms_fp64"], bytes=dual64),
+        "k_primal_f64": dict(ms=kb["primal_ms_fp64"], bytes=prim64, rocprof=f"k_primal<{lat}, double>"),
+        "k_dual_f64": dict(ms=kb["dual_ms_fp64"], bytes=dual64, rocprof=f"k_dual<{la}, double>"),
     }
     for k in kernels.values():
         k["GBps"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
         k["frac"] = k["GBps"] / HBM_PEAK_GBS
-    dom_name = max(("k_primal_f32", "k_dual_f32"), key=lambda k: kernels[k]["ms"])
+    dom_name = max(("k_primal_f64", "k_dual_f64"), key=lambda k: kernels[k]["ms"])
     dom = kernels[dom_name]
+    rp_us = rocprof_average_us(dom["rocprof"])
 
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline: reference CPU PDLP, {args.cpu_sample} attempts")
-        cpu = cpu_baseline(lp, scaled, Dr, Dc, args.cpu_sample)
+        log("CPU baseline: the reference CPU PDLP on this LP, to completion (one thread)")
+        cpu = cpu_baseline(lp, scaled, Dr, Dc)
+        log(f"CPU baseline: {cpu['value']:.2f} s, {cpu['attempts']} attempts")
 
     c0 = bench_c0()
     batched = None if args.no_batched else bench_batched()
-    serving = None
-    if not args.no_serving:
-        serving = (bench_serving(count=16, qps=0.5, nnz_hi=2e7, plain_sample_attempts=1000)
-                   if args.serving_full else bench_serving())
-    large = bench_large() if args.large else None
+    serving = None if args.no_serving else serving_full()
+    large = None if args.no_large else bench_large()
     ingest = None if args.no_ingest else bench_ingest()
     prep = None if args.no_prep else bench_prep()
 
-    head = results["mixed"]
+    head = results["fast_fp64"]
     best_name = min(results, key=lambda k: results[k]["seconds"])
     out = {
         "metric": METRIC,
@@ -753,22 +855,24 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32 iterates/SpMV + f64 statistics, KKT, restarts",
+        "dtype": "f64",
         "data": "synthetic (seeded generator)",
         "config": {
             "workload": "transportation LP 1000x1000: m=2000, n=1000000, nnz=2000000, "
                         "tol 1e-6 relative KKT (verified), cold start, LP resident in HBM",
-            "mode": "mixed",
+            "mode": "fast_fp64 (device-resident loop, FP64 iterates, SpMV and residuals)",
             "attempts": head["iterations"],
-            "fp32_attempts": head["fp32_iterations"],
             "objective": head["objective"],
             "verified_eps": head["eps"],
             "engine_setup_seconds": head["setup_seconds"],
         },
         "modes": results,
         "fastest_mode": best_name,
+        "one_shot": one_shot,
+        "one_shot_seconds": one_shot["fast_fp64"]["seconds"],
+        "one_shot_speedup_vs_plain": one_shot["fast_fp64"]["speedup_vs_plain_one_shot"],
         "plain_fp64": plain,
-        "speedup_vs_plain": (plain["seconds"] / head["seconds"]) if plain else None,
+        "speedup_vs_plain": plain["solve_seconds"] / head["seconds"],
         "roofline": {
             "bound": "hbm",
             "kernel": dom_name,
@@ -781,9 +885,13 @@ def main():
                               "WRITE_SIZE per dispatch, copy-calibrated)",
             "algorithmic_bytes_per_launch": dom["bytes"],
             "ms_per_launch": dom["ms"],
+            "rocprof_avg_us": rp_us,
+            "rocprof_frac": (dom["bytes"] / (rp_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if rp_us else None,
+            "rocprof_source": "profiles/iteration_kernels_transport_1000x1000.md",
         },
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "speedup_vs_cpu_baseline": (cpu["value"] / head["seconds"]) if cpu else None,
         "c0": c0,
         "batched": batched,
         "serving": serving,

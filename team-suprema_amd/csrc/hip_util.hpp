@@ -37,26 +37,68 @@ inline double seconds_since(Clock::time_point t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-// Owning device allocation (move-only).
+// Device allocation (move-only): owning, or a borrowed view into memory that
+// something else owns (an Arena).
 class DevBuf {
    public:
     DevBuf() = default;
     explicit DevBuf(size_t bytes) { alloc(bytes); }
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
-    DevBuf& operator=(DevBuf&& o) noexcept { std::swap(p_, o.p_); return *this; }
+    static DevBuf borrow(void* p) {
+        DevBuf b;
+        b.p_ = p;
+        b.own_ = false;
+        return b;
+    }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), own_(o.own_) { o.p_ = nullptr; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(own_, o.own_);
+        return *this;
+    }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { if (p_) (void)hipFree(p_); }
+    ~DevBuf() { release(); }
     void alloc(size_t bytes) {
-        if (p_) { (void)hipFree(p_); p_ = nullptr; }
+        release();
+        own_ = true;
         SUPR_HIP(hipMalloc(&p_, std::max<size_t>(bytes, 256)));
     }
-    void reset() { if (p_) { (void)hipFree(p_); p_ = nullptr; } }
+    void reset() { release(); }
     template <typename T> T* as() const { return static_cast<T*>(p_); }
     explicit operator bool() const { return p_ != nullptr; }
 
    private:
+    void release() {
+        if (p_ && own_) (void)hipFree(p_);
+        p_ = nullptr;
+    }
     void* p_ = nullptr;
+    bool own_ = true;
+};
+
+// Many device buffers carved out of one allocation (one hipMalloc and one
+// hipFree for all of them instead of one each).  Used in two passes: the
+// same sequence of take() calls first counts the bytes (before commit(),
+// the views are null), then, after commit(), hands out the real views.
+// Every buffer starts on a 256-byte boundary.
+class Arena {
+   public:
+    template <typename T> DevBuf take(size_t len) {
+        const size_t off = (used_ + 255) / 256 * 256;
+        used_ = off + std::max<size_t>(len * sizeof(T), 1);
+        return DevBuf::borrow(base_ ? static_cast<char*>(base_) + off : nullptr);
+    }
+    size_t bytes() const { return used_; }
+    void commit() {
+        mem_.alloc(used_);
+        base_ = mem_.as<void>();
+        used_ = 0;
+    }
+
+   private:
+    DevBuf mem_;
+    void* base_ = nullptr;
+    size_t used_ = 0;
 };
 
 // Synchronous upload of len elements: the host array may be a temporary.

@@ -195,18 +195,25 @@ void launch_spmv(const DevMatrix& M, const MatView<V>& view, const V* x, V* y, h
 
 // Lanes per row for M from its shape alone (the default: deterministic, so
 // every run of an LP sums in the same order and gives the same bits).  ELL
-// rows take one lane.  Otherwise the largest power of two not above a
-// quarter (FP32 values) or half (FP64) of the mean row length, at most 32:
-// the choices the timed autotuner makes on the transportation, power-law and
-// random LPs measured (4 nonzeros or more per FP32 lane, 2 per FP64 lane).
-// Long rows that would leave the GPU underfilled at 32 lanes (fewer than 64
-// blocks' worth of lanes in all, rows of 512 or more) take a block per row.
+// rows take one lane.  Otherwise the larger of
+//  * the largest power of two not above a quarter (FP32 values) or half
+//    (FP64) of the mean row length (4 nonzeros or more per FP32 lane, 2 per
+//    FP64 lane), and
+//  * the smallest power of two that gives the matrix 65,536 lanes in all
+//    (one 256-thread block per CU), so that a matrix with few rows still
+//    fills the GPU,
+// at most 32.  These are the timed autotuner's choices on the transportation
+// (A: 2,000 rows of 1,000), random (A: 2,000 rows of 15) and power-law (A:
+// 100,000-500,000 rows of 70; A^T: 2,000,000 rows of 17.6) LPs measured.
+// Long rows that leave the GPU underfilled even at 32 lanes (fewer than 64
+// blocks' worth, rows of 512 or more) take a block per row.
 int rule_lanes(const DevMatrix& M, bool fp64) {
     if (M.ell_w > 0 || M.rows <= 0) return 1;
     const double mean = (double)M.nnz / M.rows;
     const double t = fp64 ? mean / 2.0 : mean / 4.0;
     int L = 1;
     while (L < 32 && 2.0 * L <= t) L *= 2;
+    while (L < 32 && (double)M.rows * L < 65536.0) L *= 2;
     if (L == 32 && mean >= 512.0 && (double)M.rows * L < 64.0 * kBlock) L = kBlock;
     return L;
 }
@@ -447,7 +454,7 @@ class Solver {
 
     // device-resident modes
     template <typename T> struct Iter {
-        DevBuf x[2], y[2], ax[2], aty, xsum, ysum, xlast, ylast, xd, c, l, u, b;
+        DevBuf x[2], y[2], ax[2], xsum, ysum, xlast, ylast, xd, c, l, u, b;
         // every l_j is +0 and every u_j is +inf in this precision: l and u are
         // not stored and the kernels project onto x >= 0
         bool nonneg = false;
@@ -458,7 +465,7 @@ class Solver {
                 B.y[k] = y[k].template as<T>();
                 B.ax[k] = ax[k].template as<T>();
             }
-            B.aty = aty.template as<T>();
+            B.aty = nullptr;   // formed by every primal launch, never stored
             B.xsum = xsum.template as<T>();
             B.ysum = ysum.template as<T>();
             B.xlast = xlast.template as<T>();
@@ -474,9 +481,12 @@ class Solver {
     };
     Iter<float> F_;
     Iter<double> D_;
-    DevBuf ineq_, state_, parts_;
+    // one allocation holds every iterate, KKT and scratch buffer below (views)
+    Arena arena_;
+    std::vector<uint8_t> hineq_;   // inequality flags, host copy of ineq_
+    DevBuf ineq_, state_, parts_, x0d_;
     DevBuf kc_, kl_, ku_, krL_, krU_, kDr_, kDc_, xu_, xu2_, xavg_, yu_, yu2_, yavg_, bestx_, besty_,
-        atyavg_, axavg_;
+        axavg_;
     KktBufs K_{};
     double *part_n_ = nullptr, *part_m_ = nullptr, *part_kn_ = nullptr, *part_km_ = nullptr,
            *part_rn_ = nullptr, *part_rm_ = nullptr, *dx2_slot_ = nullptr;
@@ -484,13 +494,14 @@ class Solver {
     DevState* mirror_ = nullptr;       // host-coherent state copy (host pointer)
     DevState* mirror_dev_ = nullptr;   // the same memory, device address
 
-    template <typename T> void alloc_iter(Iter<T>& I);
+    template <typename T> void take_iter(Arena& a, Iter<T>& I);
+    void take_buffers(Arena& a, bool mixed);
     template <typename T> void reset_iter(Iter<T>& I);
     void setup_device_resident(bool mixed);
     DevState initial_state(bool mixed) const;
     template <typename T> void enqueue_attempts(const Iter<T>& I, int count);
     template <typename T> void enqueue_check(const Iter<T>& I);
-    template <typename T> void set_products(const Iter<T>& I, const MatView<T>& A, const MatView<T>& AT);
+    template <typename T> void set_products(const Iter<T>& I, const MatView<T>& A);
     template <typename T> hipGraphExec_t capture_graph(const Iter<T>& I, int attempts);
     template <typename T>
     int run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx, DevState& hs,
@@ -823,44 +834,77 @@ void Solver::run_plain(supr_stop_fn stop, void* ctx, double* x_out, double* y_ou
 
 // ---------------------------------------------------- device-resident ---
 
-template <typename T>
-void Solver::alloc_iter(Iter<T>& I) {
-    hipStream_t s = stream_.s;
-    const int n = n_, m = m_;
-    std::vector<T> c(n), l(n), u(n), b(m);
-    for (int j = 0; j < n; ++j) { c[j] = (T)S_.c[j]; l[j] = (T)S_.l[j]; u[j] = (T)S_.u[j]; }
-    for (int i = 0; i < m; ++i) b[i] = (T)S_.rL[i];
-    I.nonneg = true;
-    for (int j = 0; j < n && I.nonneg; ++j)
-        I.nonneg = l[j] == T(0) && !std::signbit(l[j]) && u[j] == std::numeric_limits<T>::infinity();
-    I.c = upload(c, s); I.b = upload(b, s);
-    if (!I.nonneg) { I.l = upload(l, s); I.u = upload(u, s); }
-    for (int k = 0; k < 2; ++k) {
-        I.x[k] = device_vec<T>(n);
-        I.y[k] = device_vec<T>(m);
-        I.ax[k] = device_vec<T>(m);
+// Every l_j is +0 and every u_j is +inf in precision T (then the kernels
+// project onto x >= 0 and read neither).
+template <typename T> bool nonneg_bounds(const double* l, const double* u, int n) {
+    for (int j = 0; j < n; ++j) {
+        const T lj = (T)l[j], uj = (T)u[j];
+        if (!(lj == T(0) && !std::signbit(lj) && uj == std::numeric_limits<T>::infinity())) return false;
     }
-    I.aty = device_vec<T>(n);
-    I.xsum = device_vec<T>(n);
-    I.ysum = device_vec<T>(m);
-    I.xlast = device_vec<T>(n);
-    I.ylast = device_vec<T>(m);
-    if (sizeof(T) == 4) I.xd = device_vec<T>(2 * (size_t)n);   // (xn, xn - x) pairs
+    return true;
 }
 
-// Cold start: x = x0, y = 0, sums cleared, restart point = start point.
+// The iterate buffers of one precision, as views into the arena.
+template <typename T>
+void Solver::take_iter(Arena& a, Iter<T>& I) {
+    const size_t n = n_, m = m_;
+    for (int k = 0; k < 2; ++k) {
+        I.x[k] = a.take<T>(n);
+        I.y[k] = a.take<T>(m);
+        I.ax[k] = a.take<T>(m);
+    }
+    I.xsum = a.take<T>(n);
+    I.ysum = a.take<T>(m);
+    I.xlast = a.take<T>(n);
+    I.ylast = a.take<T>(m);
+    if (sizeof(T) == 4) I.xd = a.take<T>(2 * n);   // (xn, xn - x) pairs
+    I.c = a.take<T>(n);
+    I.b = a.take<T>(m);
+    if (!I.nonneg) {
+        I.l = a.take<T>(n);
+        I.u = a.take<T>(n);
+    }
+}
+
+// Every buffer of the device-resident modes, in one fixed order (called once
+// to count the bytes and once, after the arena's allocation, to place them).
+void Solver::take_buffers(Arena& a, bool mixed) {
+    const size_t n = n_, m = m_;
+    ineq_ = a.take<uint8_t>(m);
+    x0d_ = a.take<double>(n);
+    if (mixed) take_iter(a, F_);
+    take_iter(a, D_);
+    kc_ = a.take<double>(n); kl_ = a.take<double>(n); ku_ = a.take<double>(n);
+    krL_ = a.take<double>(m); krU_ = a.take<double>(m);
+    kDr_ = a.take<double>(m); kDc_ = a.take<double>(n);
+    xu_ = a.take<double>(n); xu2_ = a.take<double>(n); xavg_ = a.take<double>(n);
+    yu_ = a.take<double>(m); yu2_ = a.take<double>(m); yavg_ = a.take<double>(m);
+    bestx_ = a.take<double>(n); besty_ = a.take<double>(m);
+    axavg_ = a.take<double>(m);
+    // partial-sum scratch: 1 + 2 + 8 + 6 + 1 + 1 slots per block, plus dx2
+    parts_ = a.take<double>((size_t)19 * kMaxBlocks + 64);
+    state_ = a.take<DevState>(2);   // double-buffered (see pdlp_kernels.hpp)
+}
+
+// Cold start: x = x0, y = 0, sums cleared, restart point = start point (all
+// on the stream, from the device copy of x0).
 template <typename T>
 void Solver::reset_iter(Iter<T>& I) {
     hipStream_t s = stream_.s;
-    std::vector<T> x0(n_);
-    for (int j = 0; j < n_; ++j) x0[j] = (T)x0_[j];
-    SUPR_HIP(hipMemcpyAsync(I.x[0].template as<T>(), x0.data(), sizeof(T) * n_, hipMemcpyHostToDevice, s));
-    SUPR_HIP(hipMemcpyAsync(I.xlast.template as<T>(), x0.data(), sizeof(T) * n_, hipMemcpyHostToDevice, s));
+    const double* x0 = x0d_.as<double>();
+    if constexpr (sizeof(T) == 8) {
+        SUPR_HIP(hipMemcpyAsync(I.x[0].template as<T>(), x0, sizeof(T) * n_, hipMemcpyDeviceToDevice, s));
+        SUPR_HIP(hipMemcpyAsync(I.xlast.template as<T>(), x0, sizeof(T) * n_, hipMemcpyDeviceToDevice, s));
+    } else {
+        const int g = grid_elems(n_);
+        hipLaunchKernelGGL((k_convert<double, T>), dim3(g), dim3(kBlock), 0, s, (int64_t)n_, x0, I.x[0].template as<T>());
+        hipLaunchKernelGGL((k_convert<double, T>), dim3(g), dim3(kBlock), 0, s, (int64_t)n_, x0, I.xlast.template as<T>());
+        SUPR_HIP(hipGetLastError());
+    }
     SUPR_HIP(hipMemsetAsync(I.y[0].template as<void>(), 0, sizeof(T) * m_, s));
     SUPR_HIP(hipMemsetAsync(I.ylast.template as<void>(), 0, sizeof(T) * m_, s));
     SUPR_HIP(hipMemsetAsync(I.xsum.template as<void>(), 0, sizeof(T) * n_, s));
     SUPR_HIP(hipMemsetAsync(I.ysum.template as<void>(), 0, sizeof(T) * m_, s));
-    SUPR_HIP(hipStreamSynchronize(s));   // x0 is a host temporary
 }
 
 void Solver::setup_device_resident(bool mixed) {
@@ -870,29 +914,46 @@ void Solver::setup_device_resident(bool mixed) {
     setup_matrices(mixed, true, true);
     const double t_mat = seconds_since(t0);
     initial_point();
-    std::vector<uint8_t> hineq(m);
-    for (int i = 0; i < m; ++i) hineq[i] = S_.rL[i] != S_.rU[i];
-    ineq_ = upload(hineq, s);
-    if (mixed) alloc_iter(F_);
-    alloc_iter(D_);
+    hineq_.resize(m);
+    for (int i = 0; i < m; ++i) hineq_[i] = S_.rL[i] != S_.rU[i];
+    F_.nonneg = nonneg_bounds<float>(S_.l, S_.u, n);
+    D_.nonneg = nonneg_bounds<double>(S_.l, S_.u, n);
+    // one allocation for every buffer: count, allocate, place
+    take_buffers(arena_, mixed);
+    arena_.commit();
+    take_buffers(arena_, mixed);
+    // uploads from host arrays that outlive the setup (no staging copies, one
+    // synchronisation at the end); the FP32 vectors are rounded on the device
+    auto put = [&](const DevBuf& d, const auto* h, size_t len) {
+        if (len > 0) SUPR_HIP(hipMemcpyAsync(d.as<void>(), h, sizeof(*h) * len, hipMemcpyHostToDevice, s));
+    };
+    put(ineq_, hineq_.data(), m);
+    put(x0d_, x0_.data(), n);
+    put(D_.c, S_.c, n);
+    put(D_.b, S_.rL, m);
+    if (!D_.nonneg) { put(D_.l, S_.l, n); put(D_.u, S_.u, n); }
+    if (mixed) {
+        auto f32 = [&](const DevBuf& src, const DevBuf& dst, size_t len) {
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_elems((long)len)), dim3(kBlock), 0, s,
+                               (int64_t)len, src.as<const double>(), dst.as<float>());
+        };
+        f32(D_.c, F_.c, n);
+        f32(D_.b, F_.b, m);
+        // F_ keeps l, u only when D_ does: +0 / +inf in FP64 stay so in FP32
+        if (!F_.nonneg) { f32(D_.l, F_.l, n); f32(D_.u, F_.u, n); }
+        SUPR_HIP(hipGetLastError());
+    }
     const double t_iter = seconds_since(t0);
-    auto vec = [&](const double* p, int len) { return upload(p, (size_t)len, s); };
-    kc_ = vec(U_.c, n); kl_ = vec(U_.l, n); ku_ = vec(U_.u, n);
-    krL_ = vec(U_.rL, m); krU_ = vec(U_.rU, m);
-    kDr_ = vec(Dr_, m); kDc_ = vec(Dc_, n);
-    xu_ = device_vec<double>(n); xu2_ = device_vec<double>(n); xavg_ = device_vec<double>(n);
-    yu_ = device_vec<double>(m); yu2_ = device_vec<double>(m); yavg_ = device_vec<double>(m);
-    bestx_ = device_vec<double>(n); besty_ = device_vec<double>(m);
-    atyavg_ = device_vec<double>(n); axavg_ = device_vec<double>(m);
+    put(kc_, U_.c, n); put(kl_, U_.l, n); put(ku_, U_.u, n);
+    put(krL_, U_.rL, m); put(krU_, U_.rU, m);
+    put(kDr_, Dr_, m); put(kDc_, Dc_, n);
     K_.c = kc_.as<double>(); K_.l = kl_.as<double>(); K_.u = ku_.as<double>();
     K_.rL = krL_.as<double>(); K_.rU = krU_.as<double>();
     K_.Dr = kDr_.as<double>(); K_.Dc = kDc_.as<double>();
     K_.xu = xu_.as<double>(); K_.xu2 = xu2_.as<double>(); K_.xavg = xavg_.as<double>();
     K_.yu = yu_.as<double>(); K_.yu2 = yu2_.as<double>(); K_.yavg = yavg_.as<double>();
     K_.best_x = bestx_.as<double>(); K_.best_y = besty_.as<double>();
-    K_.aty_avg = atyavg_.as<double>(); K_.ax_avg = axavg_.as<double>();
-    // partial-sum scratch: 1 + 2 + 8 + 6 + 1 + 1 slots per block, plus dx2
-    parts_ = device_vec<double>((size_t)19 * kMaxBlocks + 64);
+    K_.aty_avg = nullptr; K_.ax_avg = axavg_.as<double>();
     double* p = parts_.as<double>();
     part_n_ = p; p += kMaxBlocks;
     part_m_ = p; p += 2 * kMaxBlocks;
@@ -901,7 +962,6 @@ void Solver::setup_device_resident(bool mixed) {
     part_rn_ = p; p += kMaxBlocks;
     part_rm_ = p; p += kMaxBlocks;
     dx2_slot_ = p;
-    state_ = device_vec<DevState>(2);   // double-buffered (see pdlp_kernels.hpp)
     // host-coherent pinned copy of the state, written by the last kernel of
     // every graph replay and polled by the host (no copy on the stream)
     SUPR_HIP(hipHostMalloc((void**)&mirror_, sizeof(DevState), hipHostMallocCoherent | hipHostMallocMapped));
@@ -924,16 +984,16 @@ void Solver::setup_device_resident(bool mixed) {
         throw BadInput("bad lanes-per-row option");
     if (mixed) {
         lanesA_[0] = tune(A_, o_.spmv_lanes_A, A_.f32(), F_.x[1].as<float>(), F_.ax[1].as<float>());
-        lanesAT_[0] = tune(AT_, o_.spmv_lanes_AT, AT_.f32(), F_.y[1].as<float>(), F_.aty.as<float>());
+        lanesAT_[0] = tune(AT_, o_.spmv_lanes_AT, AT_.f32(), F_.y[1].as<float>(), F_.xsum.as<float>());
     }
     lanesA_[1] = tune(A_, o_.spmv_lanes_A, A_.f64(), D_.x[1].as<double>(), D_.ax[1].as<double>());
-    lanesAT_[1] = tune(AT_, o_.spmv_lanes_AT, AT_.f64(), D_.y[1].as<double>(), D_.aty.as<double>());
+    lanesAT_[1] = tune(AT_, o_.spmv_lanes_AT, AT_.f64(), D_.y[1].as<double>(), D_.xsum.as<double>());
     SUPR_HIP(hipStreamSynchronize(s));
     if (o_.verbose)
         std::fprintf(stderr, "[suprema] setup: matrices %.2f ms, iterates %.2f ms, KKT vectors %.2f ms, "
-                     "lanes %.2f ms (A %d/%d, AT %d/%d)\n", 1e3 * t_mat, 1e3 * (t_iter - t_mat),
-                     1e3 * (t_kkt - t_iter), 1e3 * (seconds_since(t0) - t_kkt), lanesA_[0], lanesA_[1],
-                     lanesAT_[0], lanesAT_[1]);
+                     "lanes + sync %.2f ms (A %d/%d, AT %d/%d; %.1f MB in one allocation)\n", 1e3 * t_mat,
+                     1e3 * (t_iter - t_mat), 1e3 * (t_kkt - t_iter), 1e3 * (seconds_since(t0) - t_kkt),
+                     lanesA_[0], lanesA_[1], lanesAT_[0], lanesAT_[1], arena_.bytes() / 1e6);
 }
 
 DevState Solver::initial_state(bool mixed) const {
@@ -983,13 +1043,12 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     SUPR_HIP(hipGetLastError());
 }
 
-// A x and A^T y of the current iterate with the plain SpMV (start-up and the
-// FP32 -> FP64 switch); outside any graph.
+// A x of the current iterate with the plain SpMV (start-up and the FP32 ->
+// FP64 switch); outside any graph.  (A^T y is formed by every primal launch.)
 template <typename T>
-void Solver::set_products(const Iter<T>& I, const MatView<T>& A, const MatView<T>& AT) {
+void Solver::set_products(const Iter<T>& I, const MatView<T>& A) {
     hipStream_t s = stream_.s;
     launch_spmv(A_, A, I.x[0].template as<T>(), I.ax[0].template as<T>(), s);
-    launch_spmv(AT_, AT, I.y[0].template as<T>(), I.aty.template as<T>(), s);
     SUPR_HIP(hipStreamSynchronize(s));
 }
 
@@ -1118,8 +1177,8 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
     else reset_iter(D_);
     const DevState st0[2] = {initial_state(mixed), initial_state(mixed)};
     SUPR_HIP(hipMemcpyAsync(state_.as<void>(), st0, sizeof(st0), hipMemcpyHostToDevice, s));
-    if (mixed) set_products(F_, A_.f32(), AT_.f32());
-    else set_products(D_, A_.f64(), AT_.f64());
+    if (mixed) set_products(F_, A_.f32());
+    else set_products(D_, A_.f64());
     const auto t1 = Clock::now();
 
     DevState hs;
@@ -1142,7 +1201,7 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
             const DevState both[2] = {hs, hs};
             SUPR_HIP(hipMemcpyAsync(state_.as<void>(), both, sizeof(both), hipMemcpyHostToDevice, s));
             use_lanes(1);
-            set_products(D_, A_.f64(), AT_.f64());
+            set_products(D_, A_.f64());
             run_graph_loop(D_, 1, stop, ctx, hs, stopped);
         }
     } else {
@@ -1212,7 +1271,7 @@ void Solver::bench_kernels(int reps, double* out) {
     if (o_.mode != SUPR_MODE_MIXED || !ready_) throw BadInput("bench_kernels needs a set-up mixed-mode engine");
     use_lanes(0);
     reset_iter(F_);
-    set_products(F_, A_.f32(), AT_.f32());
+    set_products(F_, A_.f32());
     DevState st = initial_state(true);
     st.check_every = std::numeric_limits<int32_t>::max();
     st.max_iter = std::numeric_limits<int32_t>::max();
@@ -1254,7 +1313,7 @@ void Solver::bench_kernels(int reps, double* out) {
     run(F_, float{}, 0);
     use_lanes(1);
     reset_iter(D_);
-    set_products(D_, A_.f64(), AT_.f64());
+    set_products(D_, A_.f64());
     run(D_, double{}, 2);
     SUPR_HIP(hipStreamSynchronize(s));
     out[4] = A_.lanes; out[5] = AT_.lanes; out[6] = A_.ell_w; out[7] = AT_.ell_w;

@@ -115,8 +115,11 @@ __global__ __launch_bounds__(kBlock) void k_copy4(const float4* __restrict__ src
 // and k_restart_fin copies S[1] back to S[0] for the next replay.
 
 // Primal half of one attempt, over the columns (rows of A^T):
-//   [after an accept] aty = A^T y, xsum += eta_acc x  (pending average term)
-//   xn = clip(x - tau (c - aty), l, u);  partial sums of (xn - x)^2 in FP64.
+//   aty = A^T y (formed every attempt, not stored: a rejected attempt forms
+//   the same product of the same y again, which costs less than writing it
+//   on every accepted one), [after an accept] xsum += eta_acc x (pending
+//   average term), xn = clip(x - tau (c - aty), l, u); partial sums of
+//   (xn - x)^2 in FP64.
 template <int L, typename T>
 __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int p,
                                                    MatView<T> AT, IterBufs<T> B,
@@ -137,7 +140,6 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
         publish_step_powers(S, p, Q);
     }
     const int cur = Q.cur;
-    const bool need_aty = Q.need_aty != 0;
     const bool pending = Q.pending != 0;
     const T eta_acc = (T)Q.eta_acc;
     const T tau = (T)(Q.eta / Q.omega);
@@ -159,28 +161,14 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
         acc[0] += d * d;
     };
     if (staged_rows<L>(AT.rows)) {
-        if (need_aty) {
-            for_row_tiles<L, 1, T>(
-                AT.rows, xcd_block(), gridDim.x,
-                [&](int row, int lane, bool valid, T (&v)[1]) { v[0] = row_dot<L, T>(AT, y, row, lane, valid); },
-                [&](int row, const T (&v)[1]) {
-                    B.aty[row] = v[0];
-                    epilogue(row, v[0]);
-                });
-        } else {
-            for_rows_flat(AT.rows, xcd_block(), gridDim.x, [&](int row) { epilogue(row, B.aty[row]); });
-        }
+        for_row_tiles<L, 1, T>(
+            AT.rows, xcd_block(), gridDim.x,
+            [&](int row, int lane, bool valid, T (&v)[1]) { v[0] = row_dot<L, T>(AT, y, row, lane, valid); },
+            [&](int row, const T (&v)[1]) { epilogue(row, v[0]); });
     } else {
         for_rows<L>(AT.rows, [&](int row, int lane, bool valid) {
-            T a = T(0);
-            if (need_aty) {
-                a = row_dot<L, T>(AT, y, row, lane, valid);
-                if (valid && lane == 0) B.aty[row] = a;
-            }
-            if (valid && lane == 0) {
-                if (!need_aty) a = B.aty[row];
-                epilogue(row, a);
-            }
+            const T a = row_dot<L, T>(AT, y, row, lane, valid);
+            if (valid && lane == 0) epilogue(row, a);
         });
     }
     block_sum<1>(acc);
@@ -309,8 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_check_prep(DevState* __restrict__ S,
 //            part of the dual objective, c'x -> 8 partials per block;
 //   rows:    row violations, sign violations of y, the row part of the
 //            dual objective -> 6 partials per block.
-// It also keeps the scaled products of the average, A^T y_avg = Dc (A_u^T
-// yu2) and A x_avg = Dr (A_u xu2), so a restart to the average needs no SpMV.
+// It also keeps the scaled product of the average A x_avg = Dr (A_u xu2),
+// so a restart to the average needs no SpMV of A (A^T y is formed by every
+// primal launch anyway).
 template <int LA, int LAT>
 __global__ __launch_bounds__(kBlock) void k_check_kkt(const DevState* __restrict__ S,
                                                       MatView<double> AuT, MatView<double> Au,
@@ -331,7 +320,6 @@ __global__ __launch_bounds__(kBlock) void k_check_kkt(const DevState* __restrict
             if (valid && lane == 0) {
                 const double cj = K.c[row], lj = K.l[row], uj = K.u[row];
                 const bool lf = isfinite(lj), uf = isfinite(uj);
-                K.aty_avg[row] = K.Dc[row] * s2;
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     const double xv = k ? K.xu2[row] : K.xu[row];
@@ -415,8 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_kkt_fin(DevState* __restrict__ S,
 // KKT check, step 4 (blocks [0, en): columns; the rest: rows): copy a new
 // best point; on a restart measure |candidate - last restart point|
 // (scaled), move the restart point, and when the candidate is the average
-// move the iterate there together with its products A x_avg, A^T y_avg;
-// clear the sums.
+// move the iterate there together with its product A x_avg; clear the sums.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_check_restart(const DevState* __restrict__ S, int n,
                                                           int m, int en, IterBufs<T> B,
@@ -434,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_check_restart(const DevState* __rest
                 const double d = cand - (double)B.xlast[j];
                 acc[0] += d * d;
                 B.xlast[j] = (T)cand;
-                if (ua) { B.x[cur][j] = (T)cand; B.aty[j] = (T)K.aty_avg[j]; }
+                if (ua) B.x[cur][j] = (T)cand;
                 B.xsum[j] = T(0);
             }
         }

@@ -15,7 +15,7 @@ from conftest import golden_lp
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-OBJ_RTOL = 1e-6
+OBJ_RTOL = 1e-5   # two 1e-6-certified points differ by a few 1e-6 on the degenerate Netlib LPs
 MAX_IT = 60000
 
 

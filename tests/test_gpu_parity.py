@@ -5,8 +5,13 @@ Tolerances (floating point; the reference path is FP64 throughout):
     FP32 (values, x and accumulation in FP32): (nnz_row + 2) * 2^-23 * (|A||x|)_row.
   * Solves: converged, the ORACLE's verifier on the returned point gives
     max(eps_P, eps_D, eps_G) <= tol (1e-6), and the objective agrees with the
-    reference engine's to |d| / (1 + |obj|) <= 1e-6 (audit_lp's WARN level,
-    tools/audit_lp.cpp:261-268; the errors measured are 1e-9 to 1e-11).
+    reference engine's own 1e-6 solution to |d| / (1 + |obj|) <= 1e-5: two
+    points that both meet a 1e-6 KKT tolerance can differ by a few 1e-6
+    (measured: up to 2.3e-6 on blend, 1e-9 to 1e-11 on the random and
+    transportation LPs).
+  * Solves to 1e-9: the objective agrees with the reference engine's own 1e-9
+    solution (tests/golden/tight.json) to 1e-7 relative, ten times audit_lp's
+    WARN level (tools/audit_lp.cpp:261-268).
 """
 import sys
 
@@ -19,7 +24,9 @@ from conftest import golden_lp, load_fixture
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-OBJ_RTOL = 1e-6
+OBJ_RTOL = 1e-5
+TIGHT_TOL = 1e-9
+TIGHT_OBJ_RTOL = 1e-7
 
 
 def _spmv_cases():
@@ -113,6 +120,35 @@ def test_device_resident_solve_matches_reference(oracle, golden, name, mode):
     case = golden[name]
     form, res = _solve(case, mode)
     _check_against_reference(oracle, case, form, res)
+
+
+@pytest.fixture(scope="module")
+def tight():
+    import json
+    import os
+    from conftest import GOLDEN_DIR
+    with open(os.path.join(GOLDEN_DIR, "tight.json")) as f:
+        return {c["name"]: c for c in json.load(f)["cases"]}
+
+
+@pytest.mark.parametrize("name", FAST_CASES)
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_tight_solve_objective_matches_reference(oracle, golden, tight, name, mode):
+    """To 1e-9 relative KKT (the mixed mode through its FP64 polish): the
+    oracle's verifier certifies the point and the objective is the
+    reference engine's 1e-9 objective within 1e-7."""
+    case, want = golden[name], tight[name]
+    form = golden_lp(case)
+    scaled, Dr, Dc = P.scale(form)
+    res = P.solve_pdhg_gpu(form, scaled, Dr, Dc, tol=TIGHT_TOL, max_iterations=500000, mode=mode)
+    assert res.converged, (name, res.iterations)
+    rep = oracle.verify(form, res.x, res.y)
+    assert max(rep[:3]) <= TIGHT_TOL, rep[:3]
+    ref_obj = want["objective_form"]
+    err = abs(rep[3] - ref_obj) / (1 + abs(ref_obj))
+    print(f"[parity] {name} to 1e-9: {res.iterations} attempts (reference {want['iterations']}), "
+          f"objective error {err:.1e}", file=sys.stderr, flush=True)
+    assert err <= TIGHT_OBJ_RTOL, (rep[3], ref_obj)
 
 
 @pytest.mark.parametrize("name", PLAIN_CASES)

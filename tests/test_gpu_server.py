@@ -14,7 +14,7 @@ from conftest import golden_lp
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-OBJ_RTOL = 1e-6
+OBJ_RTOL = 1e-5   # two 1e-6-certified points differ by a few 1e-6 on the degenerate Netlib LPs
 # room for about two of the requests below at once, so admission waits
 CAPS = dict(max_cols=2600, max_rows=420, max_nnz=6000, max_instances=3, max_blocks=16)
 
