@@ -203,9 +203,7 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=float), q))
 
 
-def bench_batched(count=512, sample=8):
-    """Batched blending LPs (mixed mode, tuned layout) against sequential
-    single-LP solves of a stratified sample of the same instances."""
+def blend_batch(count=512):
     sizes = P.gen_blend_sizes(count, seed=2026)
     lps, forms = [], []
     for i, (K, Pp) in enumerate(sizes):
@@ -214,6 +212,27 @@ def bench_batched(count=512, sample=8):
         s, Dr, Dc = P.scale(form)
         lps.append((lp, meta))
         forms.append((form, s, Dr, Dc))
+    return lps, forms
+
+
+def bench_batched_solve(count=512):
+    """The batch alone, its replays launched kernel by kernel (the run to
+    trace the batched kernels; same bits as the graph replays): the layout
+    the tuner picks in the full run (lanes bias 0.5, block cap 512)."""
+    lps, forms = blend_batch(count)
+    with P.BatchEngine(forms, tol=TOL, mode=P.MODE_MIXED, tune=False, lanes_bias=0.5, max_blocks=512,
+                       graph_attempts=-16) as eng:
+        res, secs = eng.solve()
+    conv = sum(r.converged for r in res)
+    log(f"batched solve (direct launches): {count} LPs in {secs:.3f} s, {conv} converged")
+    return {"instances": count, "seconds": secs, "converged": conv,
+            "attempts_max": int(max(r.iterations for r in res))}
+
+
+def bench_batched(count=512, sample=8):
+    """Batched blending LPs (mixed mode, tuned layout) against sequential
+    single-LP solves of a stratified sample of the same instances."""
+    lps, forms = blend_batch(count)
     t0 = time.perf_counter()
     eng = P.BatchEngine(forms, tol=TOL, mode=P.MODE_MIXED, tune=True)
     setup = time.perf_counter() - t0
@@ -686,6 +705,78 @@ def rocprof_average_us(kernel):
         return None
 
 
+def trace_summary(d, loop=False):
+    """Summary of a rocprofv3 output directory (CSV): the kernel statistics
+    and, with loop=True, the per-attempt timeline of a traced solve from the
+    kernel trace (an attempt = a k_primal launch followed by a k_dual launch;
+    its time = start of the next k_primal - start of this one)."""
+    import csv
+    import glob
+    import statistics as st
+    out = {}
+    for f in glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True):
+        rows = sorted(csv.DictReader(open(f)), key=lambda r: -float(r["TotalDurationNs"]))
+        out["kernels"] = {r["Name"].split("(")[0].replace("void ", "").replace("supr::", ""):
+                          {"calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,
+                           "min_us": float(r["MinNs"]) / 1e3, "max_us": float(r["MaxNs"]) / 1e3,
+                           "total_ms": float(r["TotalDurationNs"]) / 1e6, "pct": float(r["Percentage"])}
+                          for r in rows[:16]}
+    if loop:
+        for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+            rows = sorted(csv.DictReader(open(f)), key=lambda r: int(r["Start_Timestamp"]))
+            nm = [r["Kernel_Name"] for r in rows]
+            s0 = [int(r["Start_Timestamp"]) for r in rows]
+            e0 = [int(r["End_Timestamp"]) for r in rows]
+            att, dp, dd, checks = [], [], [], []
+            for i in range(len(rows) - 2):
+                if "k_primal" in nm[i] and "k_dual" in nm[i + 1]:
+                    dp.append(e0[i] - s0[i])
+                    dd.append(e0[i + 1] - s0[i + 1])
+                    if "k_primal" in nm[i + 2]:
+                        att.append(s0[i + 2] - s0[i])
+                if "k_check_prep" in nm[i]:
+                    j = i
+                    while j < len(nm) and "k_restart_fin" not in nm[j]:
+                        j += 1
+                    if j < len(nm):
+                        checks.append(e0[j] - s0[i])
+            q = lambda v, p: float(np.percentile(v, p)) / 1e3 if v else None  # noqa: E731
+            out["loop"] = {"dispatches": len(rows), "attempts_timed": len(att),
+                           "attempt_us": {"median": q(att, 50), "p10": q(att, 10), "p90": q(att, 90),
+                                          "mean": st.mean(att) / 1e3 if att else None},
+                           "k_primal_us": {"median": q(dp, 50), "mean": st.mean(dp) / 1e3 if dp else None},
+                           "k_dual_us": {"median": q(dd, 50), "mean": st.mean(dd) / 1e3 if dd else None},
+                           "check_group_us": {"count": len(checks), "median": q(checks, 50),
+                                              "mean": st.mean(checks) / 1e3 if checks else None},
+                           "span_ms": (e0[-1] - s0[0]) / 1e6}
+    return out
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, kernels):
+    """Bytes per dispatch from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch), each scaled by what the same counter reads
+    for the 1 GiB k_copy4 of --copy-calibration (median over dispatches)."""
+    import csv
+    import glob
+    import statistics as st
+
+    def med(d, sub):
+        f = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)[0]
+        v = [float(r["Counter_Value"]) for r in csv.DictReader(open(f)) if sub in r["Kernel_Name"]]
+        return st.median(v), len(v)
+    cf, _ = med(copy_fetch_dir, "k_copy4")
+    cw, _ = med(copy_write_dir, "k_copy4")
+    ff, wf = cf / (1 << 20), cw / (1 << 20)
+    out = {"calibration": {"copy_bytes_each_way": 1 << 30, "fetch_kib": cf, "write_kib": cw,
+                           "fetch_factor": ff, "write_factor": wf}, "kernels": {}}
+    for key, sub in kernels.items():
+        f, n = med(fetch_dir, sub)
+        w, _ = med(write_dir, sub)
+        out["kernels"][key] = {"kernel": sub, "dispatches": n, "fetch_kib": f, "write_kib": w,
+                               "read_bytes": round(f * 1024 / ff), "write_bytes": round(w * 1024 / wf)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -697,7 +788,7 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the power-law LP leg (about 3 minutes)")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--no-prep", action="store_true")
-    ap.add_argument("--only", choices=["c0", "batched", "serving", "serving-full", "large", "large-mixed",
+    ap.add_argument("--only", choices=["c0", "batched", "batched-solve", "serving", "serving-full", "large", "large-mixed",
                                        "large-kernels", "ingest", "prep", "one-shot"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -709,7 +800,24 @@ def main():
     ap.add_argument("--solve-only", default=None, choices=["fast_fp64", "mixed"],
                     help="only resident-engine solves of the 1M-variable LP in this mode (for tracing the loop; "
                          "the attempts are launched on the stream directly, 16 per host poll)")
+    ap.add_argument("--trace-summary", default=None, metavar="DIR",
+                    help="summarise a rocprofv3 --kernel-trace --stats CSV directory (no GPU needed)")
+    ap.add_argument("--loop", action="store_true", help="with --trace-summary: per-attempt timeline")
+    ap.add_argument("--pmc-summary", nargs=4, default=None,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="HBM bytes per dispatch of the 1M-variable LP's iteration kernels from "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated (no GPU needed)")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(json.dumps(trace_summary(args.trace_summary, args.loop), indent=1))
+        return
+    if args.pmc_summary:
+        lanes = [32, 1]
+        print(json.dumps(pmc_summary(*args.pmc_summary, {
+            "k_primal_f32": f"k_primal<{lanes[1]}, float>", "k_dual_f32": f"k_dual<{lanes[0]}, float>",
+            "k_primal_f64": f"k_primal<{lanes[1]}, double>", "k_dual_f64": f"k_dual<{lanes[0]}, double>"}),
+            indent=1))
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               f"(independent instances would shard across GPUs without communication)",
@@ -729,7 +837,8 @@ def main():
     serving_full = lambda: bench_serving(count=6, qps=0.5, nnz_hi=2e7, seed=11, span=True,  # noqa: E731
                                          saturated=False, plain_sample_attempts=1000)
     if args.only and args.only != "one-shot":
-        fn = {"c0": bench_c0, "batched": bench_batched, "serving": bench_serving, "large": bench_large,
+        fn = {"c0": bench_c0, "batched": bench_batched, "batched-solve": bench_batched_solve,
+              "serving": bench_serving, "large": bench_large,
               "large-mixed": lambda: bench_large(P.MODE_MIXED), "ingest": bench_ingest,
               "prep": bench_prep, "large-kernels": bench_large_kernels,
               "serving-full": serving_full}[args.only]

@@ -652,8 +652,18 @@ struct Pool {
     }
 
     int graph_attempts() const {
-        int G = opt.graph_attempts > 0 ? opt.graph_attempts : 16;
+        int G = opt.graph_attempts != 0 ? std::abs(opt.graph_attempts) : 16;
         return G + (G & 1);   // the state double buffer needs an even number per replay
+    }
+    // graph_attempts < 0: each replay's kernel sequence is launched on the
+    // stream directly instead of from a captured graph (the same kernels in
+    // the same order: the same bits; for tools that trace kernel launches)
+    bool direct() const { return opt.graph_attempts < 0; }
+
+    // One replay: the captured graph, or the same launches made directly.
+    void launch(hipGraphExec_t exec, const BatchGeo& g, int live, int G) {
+        if (direct()) enqueue_replay(g, live, G);
+        else BATCH_HIP(hipGraphLaunch(exec, stream));
     }
 
     void read_states(std::vector<DevState>& h) {
@@ -744,7 +754,7 @@ struct Batch {
         try {
             for (auto& e : ev) BATCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             if (pool.opt.verbose >= 3) pool.profile_replay(g, (int)live.size(), G);
-            exec = pool.capture(g, (int)live.size(), G);
+            if (!pool.direct()) exec = pool.capture(g, (int)live.size(), G);
             double last_progress = -1.0;
             int idle = 0, replays = 0, inflight = 0;
             size_t live_at_capture = live.size();
@@ -780,15 +790,15 @@ struct Batch {
                                          replays, still.size(), live.size());
                         live.swap(still);
                         live_at_capture = live.size();
-                        (void)hipGraphExecDestroy(exec);
+                        if (exec) (void)hipGraphExecDestroy(exec);
                         exec = nullptr;
                         g = pool.layout(live);
                         if (pool.opt.verbose >= 3) pool.profile_replay(g, (int)live.size(), G);
-                        exec = pool.capture(g, (int)live.size(), G);
+                        if (!pool.direct()) exec = pool.capture(g, (int)live.size(), G);
                     }
                 }
                 const int s = replays % R;
-                BATCH_HIP(hipGraphLaunch(exec, st));
+                pool.launch(exec, g, (int)live.size(), G);
                 BATCH_HIP(hipEventRecord(ev[s], st));
                 ++replays;
                 ++inflight;
@@ -852,6 +862,7 @@ struct Server {
     std::unordered_map<int32_t, std::unique_ptr<Request>> running;   // slot -> request
     std::unordered_map<int32_t, Clock::time_point> admitted;
     hipGraphExec_t exec = nullptr;
+    BatchGeo geo{};                     // the running set's layout (direct launches)
     bool recapture = false;             // the running set changed since the graph was captured
     int32_t finished_total = 0;
 
@@ -974,8 +985,8 @@ struct Server {
         if (recapture) {
             if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
             if (!live.empty()) {
-                const BatchGeo g = pool.layout(live);
-                exec = pool.capture(g, (int)live.size(), pool.graph_attempts());
+                geo = pool.layout(live);
+                if (!pool.direct()) exec = pool.capture(geo, (int)live.size(), pool.graph_attempts());
                 if (pool.opt.verbose)
                     std::fprintf(stderr, "[suprema] server: %zu running, graph captured\n", live.size());
             }
@@ -983,7 +994,7 @@ struct Server {
         }
         if (!live.empty()) {
             const int reps = so.replays_per_step > 0 ? so.replays_per_step : 1;
-            for (int q = 0; q < reps; ++q) BATCH_HIP(hipGraphLaunch(exec, pool.stream));
+            for (int q = 0; q < reps; ++q) pool.launch(exec, geo, (int)live.size(), pool.graph_attempts());
             BATCH_HIP(hipStreamSynchronize(pool.stream));
         }
         *n_running = (int32_t)live.size();

@@ -79,6 +79,20 @@ def test_batch_matches_oracle_and_is_mix_independent(oracle, batch_cases, mode):
         say(f"mode {mode}: alone and sub-mix results identical")
 
 
+def test_batch_direct_launches_match_graph_replay(batch_cases):
+    """graph_attempts < 0 launches each replay's kernels on the stream
+    directly (for kernel tracers): the same kernels in the same order, so the
+    same bits as the captured graph's replays, compaction included."""
+    runs = []
+    for g in (16, -16):
+        with P.BatchEngine(_forms(batch_cases), tol=TOL, max_iterations=MAX_IT, tune=False,
+                           lanes_bias=1.0, max_blocks=32, graph_attempts=g) as eng:
+            runs.append(eng.solve()[0])
+    for a, b in zip(*runs):
+        assert a.iterations == b.iterations and np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    say("direct launches identical to graph replays on", len(runs[0]), "instances")
+
+
 def test_batch_tuned_config_stays_within_tolerance(oracle, batch_cases):
     cases = batch_cases[:4]
     with P.BatchEngine(_forms(cases), tol=TOL, max_iterations=MAX_IT, tune=True) as eng:
