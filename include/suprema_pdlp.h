@@ -270,6 +270,45 @@ int supr_pdlp_form_gpu(const supr_lp* lp, int32_t device, int32_t* row_idx2, int
 int supr_verify_gpu(const supr_lp* lp, const double* x, const double* y, int32_t device,
                     double* out7);
 
+/* An LP in DEVICE memory, laid out for verify(): the CSR of A (m rows:
+ * A_ptr m + 1 entries, A_idx / A_val nnz) and of A^T (n rows: AT_ptr n + 1,
+ * AT_idx / AT_val nnz; supr_csr_gpu builds both layouts), c, l, u (n) and
+ * rL, rU (m); obj_offset is a host value.  The layouts must be valid CSR
+ * (row pointers nondecreasing from 0 to nnz, column indices in range): the
+ * device arrays are not checked.  The reference's verify() takes the same
+ * CSR pair (src/verify.cpp:24, verify(ranged, A, AT, x, y)). */
+typedef struct supr_dev_lp {
+    int32_t m, n, nnz;
+    const int32_t* A_ptr;
+    const int32_t* A_idx;
+    const double* A_val;
+    const int32_t* AT_ptr;
+    const int32_t* AT_idx;
+    const double* AT_val;
+    const double* c;
+    const double* rL;
+    const double* rU;
+    const double* l;
+    const double* u;
+    double obj_offset;
+} supr_dev_lp;
+
+/* supr_verify_gpu for a point and an LP already in device memory, allocating
+ * nothing.  No reference counterpart: a variant of supr_verify_gpu for the
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  x (n), y (m) and
+ * out7 (7 doubles: the out7 of supr_verify) are device pointers; the kernels
+ * run on `stream` (a hipStream_t; NULL: the null stream) of the current
+ * device and the call returns without waiting for them.  The same report as
+ * supr_verify_gpu to within the summation order (partial sums per block,
+ * added in a fixed order: the same bits every call).  The per-block partial
+ * sums live in one fixed array per GPU (88 KiB, held until the process ends),
+ * sized for the 1,024 blocks per pass at most that the grid-stride passes use,
+ * so any LP fits; calls that overlap on one GPU (two streams at once) race
+ * on it. */
+int supr_verify_gpu_noalloc(const supr_dev_lp* lp, const double* x, const double* y,
+                            double* out7, void* stream);
+
 /* build_pdlp_form(): dims4 = m2, n2, nnz2, n_slack. */
 int supr_pdlp_form_sizes(const supr_lp* lp, int32_t* dims4);
 int supr_pdlp_form(const supr_lp* lp, int32_t* row_idx2, int32_t* col_idx2, double* val2,

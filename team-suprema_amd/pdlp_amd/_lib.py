@@ -58,6 +58,17 @@ class SuprInfo(ctypes.Structure):
 
 STOP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
 
+class SuprDevLp(ctypes.Structure):
+    """supr_dev_lp: an LP in device memory (every pointer a device address)."""
+    _fields_ = [
+        ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("nnz", ctypes.c_int32),
+        ("A_ptr", ctypes.c_void_p), ("A_idx", ctypes.c_void_p), ("A_val", ctypes.c_void_p),
+        ("AT_ptr", ctypes.c_void_p), ("AT_idx", ctypes.c_void_p), ("AT_val", ctypes.c_void_p),
+        ("c", ctypes.c_void_p), ("rL", ctypes.c_void_p), ("rU", ctypes.c_void_p),
+        ("l", ctypes.c_void_p), ("u", ctypes.c_void_p), ("obj_offset", ctypes.c_double),
+    ]
+
+
 # (name, restype, argtypes) of every entry point declared in suprema_pdlp.h
 class SuprServerOptions(ctypes.Structure):
     _fields_ = [
@@ -106,6 +117,8 @@ SIGNATURES = [
      [_LP, ctypes.c_int32, c_int32_p, c_int32_p] + [c_double_p] * 6 + [c_int32_p, c_double_p,
                                                                        c_int32_p]),
     ("supr_verify_gpu", ctypes.c_int, [_LP, c_double_p, c_double_p, ctypes.c_int32, c_double_p]),
+    ("supr_verify_gpu_noalloc", ctypes.c_int,
+     [ctypes.POINTER(SuprDevLp), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("supr_csr_gpu", ctypes.c_int,
      [_LP, ctypes.c_int32, ctypes.c_int32, c_int32_p, c_int32_p, c_int32_p]),
     ("supr_pdlp_form_sizes", ctypes.c_int, [_LP, c_int32_p]),
