@@ -280,6 +280,69 @@ __device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict_
     return group_sum<L, Acc>(s);
 }
 
+// row_dot for long CSR rows with the next group's index and value loads
+// issued before this group's gathers are consumed (software pipelining): a
+// lane then waits about one memory latency per group of U nonzeros instead
+// of two (index, then gather).  Same terms in the same order as row_dot, so
+// the same bits; more registers in flight, so it is used where rows are long
+// and a kernel has few wavefronts (the dual half).
+template <int L, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc row_dot_pipe(const MatView<V>& M, const X* __restrict__ x, int row,
+                                            int lane, bool valid) {
+    if (M.ell_w > 0) return row_dot<L, Acc>(M, x, row, lane, valid);
+    constexpr int U = 4;
+    Acc s = Acc(0);
+    int b = 0, e = 0;
+    if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+    int p = b + lane;
+    bool full = p + (U - 1) * L < e;
+    int c[U];
+    V v[U];
+    if (full) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
+    }
+    while (full) {
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+        const int pn = p + U * L;
+        const bool nfull = pn + (U - 1) * L < e;
+        int cn[U];
+        V vn[U];
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { cn[u] = M.idx[pn + u * L]; vn[u] = M.val[pn + u * L]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
+        p = pn;
+        full = nfull;
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = cn[u]; v[u] = vn[u]; }
+        }
+    }
+    if (p < e) {
+        bool ok[U];
+        int cc[U];
+        V vv[U];
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = p + u * L < e;
+            cc[u] = ok[u] ? M.idx[p + u * L] : 0;
+            vv[u] = ok[u] ? M.val[p + u * L] : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = ok[u] ? x[cc[u]] : X(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) s += (Acc)vv[u] * (Acc)xv[u];
+    }
+    return group_sum<L, Acc>(s);
+}
+
 // Two right-hand sides sharing the index stream (FP64, KKT check).
 template <int L, typename V>
 __device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __restrict__ x1,
@@ -390,6 +453,67 @@ __device__ __forceinline__ void row_dot_diff(const MatView<float>& M, const floa
             for (int u = 0; u < U; ++u)
                 if (ok[u]) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
         }
+    }
+    s1 = group_sum<L, float>(s1);
+    sd = group_sum<L, float>(sd);
+}
+
+// row_dot_diff with the next group's loads issued before this group's
+// gathers are consumed (see row_dot_pipe); the same terms in the same order.
+template <int L>
+__device__ __forceinline__ void row_dot_diff_pipe(const MatView<float>& M,
+                                                  const float2* __restrict__ xd, int row, int lane,
+                                                  bool valid, float& s1, float& sd) {
+    if (M.ell_w > 0) { row_dot_diff<L>(M, xd, row, lane, valid, s1, sd); return; }
+    constexpr int U = 4;
+    s1 = 0.f; sd = 0.f;
+    int b = 0, e = 0;
+    if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+    int p = b + lane;
+    bool full = p + (U - 1) * L < e;
+    int c[U];
+    float a[U];
+    if (full) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = M.val[p + u * L]; }
+    }
+    while (full) {
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = xd[c[u]];
+        const int pn = p + U * L;
+        const bool nfull = pn + (U - 1) * L < e;
+        int cn[U];
+        float an[U];
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { cn[u] = M.idx[pn + u * L]; an[u] = M.val[pn + u * L]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
+        p = pn;
+        full = nfull;
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = cn[u]; a[u] = an[u]; }
+        }
+    }
+    if (p < e) {
+        bool ok[U];
+        int cc[U];
+        float aa[U];
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = p + u * L < e;
+            cc[u] = ok[u] ? M.idx[p + u * L] : 0;
+            aa[u] = ok[u] ? M.val[p + u * L] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = ok[u] ? xd[cc[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) { s1 += aa[u] * v[u].x; sd += aa[u] * v[u].y; }
     }
     s1 = group_sum<L, float>(s1);
     sd = group_sum<L, float>(sd);

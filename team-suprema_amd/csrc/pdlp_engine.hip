@@ -64,6 +64,15 @@ int grid_rows(int rows, int L) {
 int grid_elems(long len) {
     return (int)std::max(1L, std::min<long>((len + kBlock - 1) / kBlock, kMaxBlocks));
 }
+// The check group's grids: at most 512 blocks (two per CU), grid-stride over
+// the columns and rows.  Most replays have no check due, and a launch whose
+// blocks all return at once costs about 1 us per 512 blocks dispatched.
+constexpr int kCheckBlocks = 512;
+int grid_check_elems(long len) { return std::min(grid_elems(len), kCheckBlocks); }
+int grid_check_rows(int rows, int L) {
+    const int g = grid_rows(rows, L);
+    return g > kCheckBlocks ? kCheckBlocks : g;
+}
 
 #define SUPR_LANES(L, ...)                                                        \
     switch (L) {                                                                  \
@@ -1065,11 +1074,11 @@ void Solver::enqueue_check(const Iter<T>& I) {
     hipStream_t s = stream_.s;
     DevState* S = state_.as<DevState>();
     const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
-    const int en = grid_elems(n_), em = grid_elems(m_);
+    const int en = grid_check_elems(n_), em = grid_check_elems(m_);
     const int gm = grid_rows(m_, A_.lanes);
     hipLaunchKernelGGL((k_check_prep<T>), dim3(en + em), dim3(kBlock), 0, s, S, n_, m_, en, B, K_, part_m_,
                        gm, dx2_slot_);
-    const int gkn = grid_rows(n_, AT_.lanes), gkm = grid_rows(m_, A_.lanes);
+    const int gkn = grid_check_rows(n_, AT_.lanes), gkm = grid_check_rows(m_, A_.lanes);
     const MatView<double> AuT = AT_.unscaled(), Au = A_.unscaled();
     SUPR_LANES(A_.lanes, launch_check_kkt<LL>(AT_.lanes, s, S, AuT, Au, gkn, gkm, K_, part_kn_, part_km_));
     hipLaunchKernelGGL(k_kkt_fin, dim3(1), dim3(kBlock), 0, s, S, part_kn_, gkn, part_km_, gkm);
@@ -1116,7 +1125,7 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
     G += G & 1;   // the state double buffer needs an even number of attempts per replay
     if (!direct && !graph_[prec]) graph_[prec] = capture_graph(I, G);
     hipGraphExec_t exec = graph_[prec];
-    constexpr int R = 4;
+    constexpr int R = 2;   // replays in flight: a replay takes far longer than the host's poll
     hipEvent_t ev[R] = {};
     int replays = 0;
     try {

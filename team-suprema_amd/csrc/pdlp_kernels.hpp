@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
         if constexpr (kDiff) {
             row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, v[0], v[1]);
         } else {
-            v[0] = row_dot<L, T>(A, xn, row, lane, valid);
+            v[0] = row_dot_pipe<L, T>(A, xn, row, lane, valid);
             v[1] = T(0);
         }
     };
