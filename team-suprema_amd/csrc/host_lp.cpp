@@ -38,7 +38,15 @@ bool check_lp(const supr_lp* lp, const char* what, bool indices) {
         set_error(std::string(what) + ": null array");
         return false;
     }
-    for (int32_t k = 0; indices && k < lp->nnz; ++k) {
+    if (!indices || lp->nnz == 0) return true;
+    // one branch-free pass (unsigned compares fold the < 0 test in), then
+    // the first bad entry only if there is one
+    const uint32_t mu = (uint32_t)lp->m, nu = (uint32_t)lp->n;
+    uint32_t bad = 0;
+    for (int32_t k = 0; k < lp->nnz; ++k)
+        bad |= (uint32_t)((uint32_t)lp->row_idx[k] >= mu) | (uint32_t)((uint32_t)lp->col_idx[k] >= nu);
+    if (!bad) return true;
+    for (int32_t k = 0; k < lp->nnz; ++k) {
         if (lp->row_idx[k] < 0 || lp->row_idx[k] >= lp->m || lp->col_idx[k] < 0 ||
             lp->col_idx[k] >= lp->n) {
             set_error(std::string(what) + ": COO index out of range at entry " + std::to_string(k));

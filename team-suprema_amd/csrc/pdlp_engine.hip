@@ -1340,11 +1340,15 @@ int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const
         set_error("unscaled and scaled forms differ in shape");
         return SUPR_EINVAL;
     }
-    for (int k = 0; k < S->nnz; ++k)
-        if (U->row_idx[k] != S->row_idx[k] || U->col_idx[k] != S->col_idx[k]) {
-            set_error("unscaled and scaled forms must share one COO pattern (entry " + std::to_string(k) + ")");
-            return SUPR_EINVAL;
-        }
+    const size_t pat = sizeof(int32_t) * (size_t)S->nnz;
+    if (S->nnz > 0 && (std::memcmp(U->row_idx, S->row_idx, pat) != 0 ||
+                       std::memcmp(U->col_idx, S->col_idx, pat) != 0)) {
+        for (int k = 0; k < S->nnz; ++k)
+            if (U->row_idx[k] != S->row_idx[k] || U->col_idx[k] != S->col_idx[k]) {
+                set_error("unscaled and scaled forms must share one COO pattern (entry " + std::to_string(k) + ")");
+                return SUPR_EINVAL;
+            }
+    }
     if (!Dr || !Dc) { set_error("null Dr or Dc"); return SUPR_EINVAL; }
     return check_options(S, opt);
 }

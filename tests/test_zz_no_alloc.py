@@ -14,8 +14,10 @@ A^T y on the columns, divided by the norms' denominators: at a converged
 point the violations are differences of nearly equal numbers); the
 objective c'x, the dual objective and eps_G to 1e-13 of the sum of their
 terms' magnitudes; the largest row and bound violations to their rows'
-rounding bounds.  Device memory for the inputs and the
-output is the test's own (torch tensors), as a caller's would be."""
+rounding bounds.  Device memory for the inputs and the output is the test's
+own, as a caller's would be: hipMalloc of the HIP runtime the library itself
+is linked against (one runtime in the process; torch's bundled runtime is not
+loaded)."""
 import ctypes
 
 import numpy as np
@@ -27,7 +29,48 @@ from conftest import load_fixture
 
 pytestmark = pytest.mark.gpu
 
-torch = pytest.importorskip("torch")
+
+class Hip:
+    """hipMalloc / hipMemcpy / hipFree of the HIP runtime the library itself
+    loaded (the caller's own device memory for the inputs and the output)."""
+
+    def __init__(self):
+        P.lib()   # loads the HIP runtime (soname libamdhip64.so.7) with the library
+        self.rt = ctypes.CDLL("libamdhip64.so.7")   # the already-loaded object
+        self.rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.rt.hipFree.argtypes = [ctypes.c_void_p]
+        self.rt.hipDeviceSynchronize.argtypes = []
+        for f in (self.rt.hipMalloc, self.rt.hipMemcpy, self.rt.hipFree, self.rt.hipDeviceSynchronize):
+            f.restype = ctypes.c_int
+        self.live = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        assert self.rt.hipMalloc(ctypes.byref(p), max(a.nbytes, 8)) == 0
+        self.live.append(p)
+        if a.nbytes:
+            assert self.rt.hipMemcpy(p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1) == 0
+        return p.value
+
+    def get(self, ptr, count):
+        out = np.empty(count, dtype=np.float64)
+        assert self.rt.hipDeviceSynchronize() == 0
+        assert self.rt.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes, 2) == 0
+        return out
+
+    def free_all(self):
+        for p in self.live:
+            self.rt.hipFree(p)
+        self.live = []
+
+
+@pytest.fixture(scope="module")
+def hip():
+    h = Hip()
+    yield h
+    h.free_all()
 
 
 def all_row_kinds_lp(seed=3):
@@ -50,30 +93,26 @@ def all_row_kinds_lp(seed=3):
 
 
 class DeviceLp:
-    """The LP's CSR pair and vectors in device memory (torch tensors)."""
+    """The LP's CSR pair and vectors in device memory."""
 
-    def __init__(self, lp):
+    def __init__(self, hip, lp):
         ptrA, idxA, permA = P.csr_gpu(lp, transposed=False)
         ptrT, idxT, permT = P.csr_gpu(lp, transposed=True)
-        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a, dtype=dt), device="cuda")  # noqa: E731
-        self.keep = [t(ptrA, np.int32), t(idxA, np.int32), t(lp.val[permA], np.float64),
-                     t(ptrT, np.int32), t(idxT, np.int32), t(lp.val[permT], np.float64),
-                     t(lp.c, np.float64), t(lp.rL, np.float64), t(lp.rU, np.float64),
-                     t(lp.l, np.float64), t(lp.u, np.float64)]
-        k = [a.data_ptr() for a in self.keep]
+        k = [hip.put(np.asarray(a, dtype=dt)) for a, dt in (
+            (ptrA, np.int32), (idxA, np.int32), (lp.val[permA], np.float64),
+            (ptrT, np.int32), (idxT, np.int32), (lp.val[permT], np.float64),
+            (lp.c, np.float64), (lp.rL, np.float64), (lp.rU, np.float64),
+            (lp.l, np.float64), (lp.u, np.float64))]
         self.s = _lib.SuprDevLp(lp.m, lp.n, lp.nnz, *k, float(lp.obj_offset))
 
 
-def verify_noalloc(dlp, x, y):
-    xd = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device="cuda")
-    yd = torch.as_tensor(np.ascontiguousarray(y, dtype=np.float64), device="cuda")
-    out = torch.full((7,), np.nan, dtype=torch.float64, device="cuda")
-    stream = torch.cuda.current_stream()
-    rc = P.lib().supr_verify_gpu_noalloc(ctypes.byref(dlp.s), xd.data_ptr(), yd.data_ptr(),
-                                         out.data_ptr(), ctypes.c_void_p(stream.cuda_stream))
+def verify_noalloc(hip, dlp, x, y):
+    xd = hip.put(np.asarray(x, dtype=np.float64))
+    yd = hip.put(np.asarray(y, dtype=np.float64))
+    out = hip.put(np.full(7, np.nan))
+    rc = P.lib().supr_verify_gpu_noalloc(ctypes.byref(dlp.s), xd, yd, out, None)   # the null stream
     assert rc == 0, P.lib().supr_last_error().decode()
-    stream.synchronize()
-    return out.cpu().numpy()
+    return hip.get(out, 7)
 
 
 def magnitudes(lp, x, y):
@@ -117,13 +156,15 @@ def check_same(lp, x, y, got):
     return w
 
 
-def test_noalloc_verify_matches_verify_gpu_on_calls_in_a_row():
+def test_noalloc_verify_matches_verify_gpu_on_calls_in_a_row(hip):
     rng = np.random.default_rng(7)
     big = P.gen_transport(1000, 1000, seed=42)
     cases = [big, all_row_kinds_lp(), load_fixture("share1b")[0], big]
     devs = {}
     for k, lp in enumerate(cases):
-        dlp = devs.setdefault(id(lp), DeviceLp(lp))
+        if id(lp) not in devs:
+            devs[id(lp)] = DeviceLp(hip, lp)
+        dlp = devs[id(lp)]
         x = rng.standard_normal(lp.n) * 3
         y = rng.standard_normal(lp.m)
         if k == 0:
@@ -131,9 +172,9 @@ def test_noalloc_verify_matches_verify_gpu_on_calls_in_a_row():
             s, Dr, Dc = P.scale(lp)
             r = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=1e-6)
             x, y = r.x, r.y
-        got = verify_noalloc(dlp, x, y)
+        got = verify_noalloc(hip, dlp, x, y)
         check_same(lp, x, y, got)
-        again = verify_noalloc(dlp, x, y)
+        again = verify_noalloc(hip, dlp, x, y)
         assert np.array_equal(got.view(np.uint64), again.view(np.uint64))   # fixed order: same bits
         print(f"[no-alloc] call {k}: {lp.name} m={lp.m} n={lp.n} eps {got[:3]}", flush=True)
 
