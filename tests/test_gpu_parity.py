@@ -312,22 +312,23 @@ def test_stop_flag_cancels(mode):
     assert polls["n"] > 20
 
 
-def test_resident_engine_repeats_the_one_shot_solve(oracle):
-    """supr_engine_*: repeated cold-start solves on a resident LP give the same
-    bits as each other and as the one-shot entry point."""
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_default_runs_are_bit_reproducible(oracle, mode):
+    """With default options (lanes per row from the matrix's shape, no
+    timing) every setup of an LP sums in the same order: the one-shot entry
+    point, two solves on one resident engine and a second engine give the
+    same attempts and the same bits."""
     lp = P.gen_transport(60, 60, seed=42)
     scaled, Dr, Dc = P.scale(lp)
-    # the summation order depends on the SpMV schedule: fix it (the autotuner
-    # times the candidates, so its choice can change between runs)
-    sched = dict(spmv_lanes_A=32, spmv_lanes_AT=1)
-    one = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, **sched)
-    with P.Engine(lp, scaled, Dr, Dc, tol=TOL, **sched) as eng:
-        a = eng.solve()
-        b = eng.solve()
-    for r in (a, b):
-        assert r.converged and r.iterations == one.iterations
+    one = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=TOL, mode=mode)
+    runs = []
+    for _ in range(2):
+        with P.Engine(lp, scaled, Dr, Dc, tol=TOL, mode=mode) as eng:
+            runs += [eng.solve(), eng.solve()]
+    for r in runs:
+        assert r.converged and r.iterations == one.iterations and r.restarts == one.restarts
         assert np.array_equal(r.x, one.x) and np.array_equal(r.y, one.y)
-    rep = oracle.verify(lp, a.x, a.y)
+    rep = oracle.verify(lp, one.x, one.y)
     assert max(rep[:3]) <= TOL
 
 
