@@ -84,12 +84,16 @@ def kernel_bytes(lp, s, ell_AT, ell_A, nonneg=False, store_aty=False):
       dual half (rows of A): A indices + values, row pointers unless ELL, the
         gathered vector once (xn; the (xn, xn - x) pairs in the FP32 loop),
         then per row y, ax, b read, ysum read+write, axn and yn written, the
-        inequality flag -> nnz (4 + s) + [4 (m + 1)] + (s | 8) n + 7 s m + m"""
+        inequality flag -> nnz (4 + s) + [4 (m + 1)] + (s | 8) n + 7 s m + m
+    An ELL index is 2 bytes instead of 4 where the gathered vector has at most
+    65,536 entries (the library's 16-bit ELL indices)."""
     n, m, nnz = lp.n, lp.m, lp.nnz
     pair = s == 4
-    primal = (nnz * (4 + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + (5 if nonneg else 7) * s * n
+    iAT = 2 if ell_AT and m <= 65536 else 4
+    iA = 2 if ell_A and n <= 65536 else 4
+    primal = (nnz * (iAT + s) + (0 if ell_AT else 4 * (n + 1)) + s * m + (5 if nonneg else 7) * s * n
               + (s * n if store_aty else 0) + (8 * n if pair else 0))
-    dual = nnz * (4 + s) + (0 if ell_A else 4 * (m + 1)) + (8 if pair else s) * n + 7 * s * m + m
+    dual = nnz * (iA + s) + (0 if ell_A else 4 * (m + 1)) + (8 if pair else s) * n + 7 * s * m + m
     return primal, dual
 
 

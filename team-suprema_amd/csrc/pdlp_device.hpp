@@ -79,6 +79,10 @@ template <typename V> struct MatView {
     const int32_t* ptr;      // CSR row pointers (unused for ELL)
     const int32_t* idx;
     const V* val;
+    // ELL only: the same indices as 16-bit values when the gathered vector
+    // has at most 65,536 entries (A^T of an LP with few rows); null otherwise.
+    // The ELL loops read these instead of idx: 2 bytes per slot instead of 4.
+    const uint16_t* idx16 = nullptr;
 };
 
 // Iterate buffers of one precision.
@@ -239,10 +243,14 @@ __device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict_
     if (M.ell_w > 0) {
         if (valid) {
             const size_t R = (size_t)M.rows;
-            for (int k = lane; k < M.ell_w; k += L) {
-                const size_t q = (size_t)k * R + row;
-                s += (Acc)M.val[q] * (Acc)x[M.idx[q]];
-            }
+            auto run = [&](const auto* __restrict__ ix) {
+                for (int k = lane; k < M.ell_w; k += L) {
+                    const size_t q = (size_t)k * R + row;
+                    s += (Acc)M.val[q] * (Acc)x[ix[q]];
+                }
+            };
+            if (M.idx16) run(M.idx16);
+            else run(M.idx);
         }
     } else {
         int b = 0, e = 0;
@@ -353,13 +361,17 @@ __device__ __forceinline__ void row_dot2(const MatView<V>& M, const double* __re
     if (M.ell_w > 0) {
         if (valid) {
             const size_t R = (size_t)M.rows;
-            for (int k = lane; k < M.ell_w; k += L) {
-                const size_t q = (size_t)k * R + row;
-                const int c = M.idx[q];
-                const double a = (double)M.val[q];
-                s1 += a * x1[c];
-                s2 += a * x2[c];
-            }
+            auto run = [&](const auto* __restrict__ ix) {
+                for (int k = lane; k < M.ell_w; k += L) {
+                    const size_t q = (size_t)k * R + row;
+                    const int c = ix[q];
+                    const double a = (double)M.val[q];
+                    s1 += a * x1[c];
+                    s2 += a * x2[c];
+                }
+            };
+            if (M.idx16) run(M.idx16);
+            else run(M.idx);
         }
     } else {
         int b = 0, e = 0;
@@ -413,13 +425,17 @@ __device__ __forceinline__ void row_dot_diff(const MatView<float>& M, const floa
     if (M.ell_w > 0) {
         if (valid) {
             const size_t R = (size_t)M.rows;
-            for (int k = lane; k < M.ell_w; k += L) {
-                const size_t q = (size_t)k * R + row;
-                const float a = M.val[q];
-                const float2 v = xd[M.idx[q]];
-                s1 += a * v.x;
-                sd += a * v.y;
-            }
+            auto run = [&](const auto* __restrict__ ix) {
+                for (int k = lane; k < M.ell_w; k += L) {
+                    const size_t q = (size_t)k * R + row;
+                    const float a = M.val[q];
+                    const float2 v = xd[ix[q]];
+                    s1 += a * v.x;
+                    sd += a * v.y;
+                }
+            };
+            if (M.idx16) run(M.idx16);
+            else run(M.idx);
         }
     } else {
         int b = 0, e = 0;

@@ -95,7 +95,7 @@ bool valid_lanes(int L) {
 
 struct DevMatrix {
     int rows = 0, cols = 0, nnz = 0, ell_w = 0, lanes = 1;
-    DevBuf ptr, idx, v32, v64, vu;
+    DevBuf ptr, idx, idx16, v32, v64, vu;
     template <typename V> MatView<V> view(const DevBuf& v) const {
         MatView<V> m;
         m.rows = rows;
@@ -103,6 +103,7 @@ struct DevMatrix {
         m.ptr = ptr.as<int32_t>();
         m.idx = idx.as<int32_t>();
         m.val = v.as<V>();
+        m.idx16 = idx16.as<uint16_t>();
         return m;
     }
     MatView<float> f32() const { return view<float>(v32); }
@@ -111,6 +112,13 @@ struct DevMatrix {
 };
 
 constexpr int kMaxEllWidth = 16;
+
+// ELL indices narrower than int32: when every index is below 65,536 (the
+// gathered vector is that short: A^T of an LP with few rows), the ELL loops
+// read a uint16 copy -- on the 1M-variable transportation LP's A^T that is
+// 4 MB instead of 8 MB per primal launch.  The int32 copy stays for the host
+// side and the layout checks.
+bool narrow_ell(const DevMatrix& D) { return D.ell_w > 0 && D.cols > 0 && D.cols <= 65536; }
 
 // Lays out H (CSR of the scaled matrix; H.perm maps to COO positions) on the
 // device, as ELL when every row has the same length 1..16.
@@ -139,6 +147,11 @@ void build_dev_matrix(DevMatrix& D, const Csr& H, const double* unscaled_coo, bo
     }
     for (size_t q = 0; q < nnz; ++q) idx[q] = H.idx[src[q]];
     D.idx = upload(idx, s);
+    if (narrow_ell(D)) {
+        std::vector<uint16_t> i16(nnz);
+        for (size_t q = 0; q < nnz; ++q) i16[q] = (uint16_t)idx[q];
+        D.idx16 = upload(i16, s);
+    }
     if (want32) {
         std::vector<float> v(nnz);
         for (size_t q = 0; q < nnz; ++q) v[q] = (float)H.val[src[q]];
@@ -175,6 +188,10 @@ void build_dev_matrix(DevMatrix& D, DevPattern& P, DevBuf& scaled, DevBuf& unsca
         };
         D.idx = device_vec<int32_t>(nnz);
         ell(P.idx.as<const int32_t>(), D.idx.as<int32_t>());
+        if (narrow_ell(D)) {
+            D.idx16 = device_vec<uint16_t>(nnz);
+            ell(P.idx.as<const int32_t>(), D.idx16.as<uint16_t>());
+        }
         if (want32) { D.v32 = device_vec<float>(nnz); ell(scaled.as<const double>(), D.v32.as<float>()); }
         if (want64) { D.v64 = device_vec<double>(nnz); ell(scaled.as<const double>(), D.v64.as<double>()); }
         if (wantu) { D.vu = device_vec<double>(nnz); ell(unscaled.as<const double>(), D.vu.as<double>()); }

@@ -32,7 +32,10 @@ TIGHT_OBJ_RTOL = 1e-7
 def _spmv_cases():
     rng = np.random.default_rng(5)
     cases = {
+        # A^T is ELL (6 per column) with 16-bit indices (m <= 65,536) ...
         "random_c0_small": P.gen_random_lp(200, 500, 6, seed=1),
+        # ... and with 32-bit ones (m > 65,536)
+        "random_ell_wide": P.gen_random_lp(70000, 30000, 6, seed=3),
         "transport_30x40": P.gen_transport(30, 40, seed=7),
         "powerlaw_mag3": P.gen_random_lp(300, 700, 0, seed=2, mag_decades=3,
                                          col_deg=np.minimum(2 + rng.zipf(1.5, 700), 200).clip(2, 250)),
