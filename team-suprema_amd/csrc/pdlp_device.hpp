@@ -126,11 +126,60 @@ struct KktBufs {
 
 // ------------------------------------------------------------- helpers --
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
+// The value held by lane (lane ^ O) of the wavefront, O a power of two below
+// 64, moved without LDS: DPP inside a row of 16 lanes (quad_perm for 1 and 2,
+// row_shl / row_shr for 4, row_ror for 8) and the permlane swaps across rows
+// (16) and wavefront halves (32).  __shfl_xor goes through ds_bpermute, an
+// LDS round trip per 32-bit half.  All lanes of the wavefront must be active.
+template <int O> __device__ __forceinline__ int xor_lane(int v) {
+    if constexpr (O == 1) {
+        return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    } else if constexpr (O == 2) {
+        return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    } else if constexpr (O == 4) {
+        const int up = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xf, 0xf, false);   // row_shl:4
+        const int dn = __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+        return (threadIdx.x & 4) ? dn : up;
+    } else if constexpr (O == 8) {
+        return __builtin_amdgcn_update_dpp(v, v, 0x128, 0xf, 0xf, false);  // row_ror:8
+    } else if constexpr (O == 16) {
+        // odd rows of the first operand <-> even rows of the second
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (threadIdx.x & 16) ? r[0] : r[1];
+    } else {
+        static_assert(O == 32, "xor distances 1..32");
+        // upper half of the first operand <-> lower half of the second
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (threadIdx.x & 32) ? r[0] : r[1];
+    }
 }
+template <int O> __device__ __forceinline__ float xor_lane(float v) {
+    return __int_as_float(xor_lane<O>(__float_as_int(v)));
+}
+template <int O> __device__ __forceinline__ double xor_lane(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)xor_lane<O>((int)(unsigned)b);
+    const unsigned hi = (unsigned)xor_lane<O>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Butterfly sum over groups of W lanes (W = 2..64): s += s(lane ^ o) for
+// o = W/2, ..., 1 -- the additions of a __shfl_xor butterfly, in the same
+// order, so the same bits; every lane of a group ends with the group's sum.
+template <int W, typename T> __device__ __forceinline__ T xor_sum(T s) {
+    if constexpr (W >= 64) s += xor_lane<32>(s);
+    if constexpr (W >= 32) s += xor_lane<16>(s);
+    if constexpr (W >= 16) s += xor_lane<8>(s);
+    if constexpr (W >= 8) s += xor_lane<4>(s);
+    if constexpr (W >= 4) s += xor_lane<2>(s);
+    if constexpr (W >= 2) s += xor_lane<1>(s);
+    return s;
+}
+
+// Wavefront sum, valid in every lane.  Lane 0's value is that of the
+// __shfl_down tree (v_i + v_{i+32}, then + the i+16 partner, ...): for the
+// lanes below each distance the xor partner is lane + o.
+__device__ __forceinline__ double wave_sum(double v) { return xor_sum<64>(v); }
 
 // Sums N values over the block; result valid in thread 0.
 template <int N> __device__ __forceinline__ void block_sum(double (&v)[N]) {
@@ -205,19 +254,16 @@ __device__ __forceinline__ void wave_sum_partials(const double* __restrict__ par
     for (int q = 0; q < N; ++q) out[q] = wave_sum(out[q]);
 }
 
-// Sum over a group of L lanes (L <= 64: xor shuffles inside the wavefront;
+// Sum over a group of L lanes (L <= 64: an xor butterfly inside the wavefront;
 // L = 256: the whole block, wavefront sums combined through LDS).  Every lane
 // of the group gets the result.  Called with block-uniform control flow.
 template <int L, typename Acc> __device__ __forceinline__ Acc group_sum(Acc s) {
     if constexpr (L <= 64) {
-#pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
-        return s;
+        return xor_sum<L>(s);
     } else {
         static_assert(L == kBlock, "block-wide groups span the whole block");
         __shared__ Acc red[kWaves];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        s = xor_sum<64>(s);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
         __syncthreads();
         Acc t = red[0];

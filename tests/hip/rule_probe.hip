@@ -3,12 +3,14 @@
 // restart_fin_update -- the code the single-LP and batched kernels call) on
 // scripted inputs, one case per thread, so tests can compare them with the
 // oracle's orc_step_rule / orc_check_rule / orc_primal_weight
-// (include/pdlp_algo.hpp:94-100, 113-131, 135-167).  Built into
+// (include/pdlp_algo.hpp:94-100, 113-131, 135-167), and the cross-lane
+// sums every kernel reduces with against the __shfl forms.  Built into
 // tests/hip/librule_probe.so; the product library never loads it.
 #include <hip/hip_runtime.h>
 
 #include <climits>
 #include <cstring>
+#include <type_traits>
 
 #include "pdlp_device.hpp"
 
@@ -103,6 +105,43 @@ __global__ void k_weight(int n, const double* __restrict__ in, double* __restric
     out[t] = s.omega;
 }
 
+// Cross-lane sums: for each width W = 2..64, the library's DPP / permlane
+// butterfly (supr::xor_sum<W>) and the same butterfly through __shfl_xor,
+// and supr::wave_sum against the __shfl_down tree it replaced.  One value per
+// thread; out holds 6 (W) x 2 (library, shuffle) results per thread, then
+// the two wavefront sums.
+template <typename T>
+__device__ void lane_sums(T v, T* o) {
+    int k = 0;
+    auto one = [&](auto Wc) {
+        constexpr int W = decltype(Wc)::value;
+        T ref = v;
+#pragma unroll
+        for (int q = W / 2; q > 0; q >>= 1) ref += __shfl_xor(ref, q, W);
+        o[k++] = supr::xor_sum<W>(v);
+        o[k++] = ref;
+    };
+    one(std::integral_constant<int, 2>{});
+    one(std::integral_constant<int, 4>{});
+    one(std::integral_constant<int, 8>{});
+    one(std::integral_constant<int, 16>{});
+    one(std::integral_constant<int, 32>{});
+    one(std::integral_constant<int, 64>{});
+}
+
+__global__ void k_lanes(int n, const double* __restrict__ in, const float* __restrict__ inf,
+                        double* __restrict__ out, float* __restrict__ outf) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // n is a multiple of the block
+    lane_sums<double>(in[t], out + 14 * (size_t)t);
+    lane_sums<float>(inf[t], outf + 12 * (size_t)t);
+    double d = in[t];
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) d += __shfl_down(d, q, 64);
+    out[14 * (size_t)t + 12] = supr::wave_sum(in[t]);
+    out[14 * (size_t)t + 13] = d;
+    (void)n;
+}
+
 template <typename T> int up(T*& d, const T* h, size_t k) {
     if (hipMalloc((void**)&d, sizeof(T) * (k ? k : 1)) != hipSuccess) return -1;
     if (k && hipMemcpy(d, h, sizeof(T) * k, hipMemcpyHostToDevice) != hipSuccess) return -1;
@@ -157,6 +196,25 @@ int probe_primal_weight(int n, const double* in, double* out) {
             rc = 0;
     }
     (void)hipFree(din); (void)hipFree(dout);
+    return rc;
+}
+
+// n values (a multiple of 256) in in / inf; out: 14 doubles, outf: 12
+// floats per value (see k_lanes).
+int probe_lane_sums(int n, const double* in, const float* inf, double* out, float* outf) {
+    double *din = nullptr, *dout = nullptr;
+    float *dinf = nullptr, *doutf = nullptr;
+    int rc = -1;
+    if (n > 0 && n % 256 == 0 && !up(din, in, (size_t)n) && !up(dinf, inf, (size_t)n) &&
+        hipMalloc((void**)&dout, sizeof(double) * 14 * n) == hipSuccess &&
+        hipMalloc((void**)&doutf, sizeof(float) * 12 * n) == hipSuccess) {
+        hipLaunchKernelGGL(k_lanes, dim3(n / 256), dim3(256), 0, 0, n, din, dinf, dout, doutf);
+        if (hipGetLastError() == hipSuccess &&
+            hipMemcpy(out, dout, sizeof(double) * 14 * n, hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(outf, doutf, sizeof(float) * 12 * n, hipMemcpyDeviceToHost) == hipSuccess)
+            rc = 0;
+    }
+    (void)hipFree(din); (void)hipFree(dinf); (void)hipFree(dout); (void)hipFree(doutf);
     return rc;
 }
 

@@ -45,7 +45,7 @@ def probe():
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built (run __graft_entry__.build())")
     lib = ctypes.CDLL(path)
-    for f in (lib.probe_step_rule, lib.probe_check_rule, lib.probe_primal_weight):
+    for f in (lib.probe_step_rule, lib.probe_check_rule, lib.probe_primal_weight, lib.probe_lane_sums):
         f.restype = ctypes.c_int
     return lib
 
@@ -229,3 +229,32 @@ def test_primal_weight_matches_oracle(probe, oracle):
             assert close(out[k], want, rel), (k, out[k], want)
     assert 0 < unchanged < n
     say(f"primal weight: {n} cases ({unchanged} below the 1e-10 distance floor)")
+
+
+def test_lane_sums_match_shuffles_bit_for_bit(probe):
+    """The kernels' cross-lane sums (DPP / permlane moves, no LDS) add the
+    same values in the same order as the __shfl_xor butterfly, for every
+    group width 2..64 and every lane, in FP64 and FP32; the wavefront sum in
+    lane 0 equals the __shfl_down tree.  Inputs span 40 decades with mixed
+    signs, so any change of pairing shows in the low bits."""
+    rng = np.random.default_rng(11)
+    n = 256 * 64
+    x = rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-20, 20, n)
+    x[rng.random(n) < 0.02] = 0.0
+    x[rng.random(n) < 0.01] = -0.0
+    xf = (rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-15, 15, n)).astype(np.float32)
+    out = np.zeros(14 * n)
+    outf = np.zeros(12 * n, dtype=np.float32)
+    assert probe.probe_lane_sums(n, d(x), xf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), d(out),
+                                 outf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    o = out.reshape(n, 14)
+    of = outf.reshape(n, 12)
+    for w in range(6):
+        assert np.array_equal(o[:, 2 * w].view(np.uint64), o[:, 2 * w + 1].view(np.uint64)), 2 << w
+        assert np.array_equal(of[:, 2 * w].view(np.uint32), of[:, 2 * w + 1].view(np.uint32)), 2 << w
+    lane0 = o[::64]
+    assert np.array_equal(lane0[:, 12].view(np.uint64), lane0[:, 13].view(np.uint64))
+    # and the sums are sums: the 64-wide result against a sorted-order sum
+    ref = np.array([math.fsum(x[k:k + 64]) for k in range(0, n, 64)])
+    assert np.allclose(lane0[:, 12], ref, rtol=1e-12, atol=1e-12 * np.abs(x).reshape(-1, 64).sum(1).max())
+    say(f"cross-lane sums: {n} values, widths 2-64, FP64 and FP32 identical to the shuffles")
