@@ -248,7 +248,7 @@ Prepared prepare(const supr_lp& U, const supr_lp& S, const double* Dr, const dou
         p.fu.assign(p.u.begin(), p.u.end());
         p.fb.assign(p.b.begin(), p.b.end());
     }
-    p.meanA = (double)S.nnz / S.m;
+    p.meanA = S.m > 0 ? (double)S.nnz / S.m : 0.0;
     p.meanAT = (double)S.nnz / S.n;
     p.init = initial_state(S, U, o, mixed);
     return p;
@@ -259,7 +259,7 @@ Prepared prepare(const supr_lp& U, const supr_lp& S, const double* Dr, const dou
 bool check_instance(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc,
                     const std::string& what) {
     if (!check_lp(&U, "unscaled_form") || !check_lp(&S, "scaled_form")) return false;
-    if (U.m != S.m || U.n != S.n || U.nnz != S.nnz || S.m < 1 || S.n < 1 || !Dr || !Dc) {
+    if (U.m != S.m || U.n != S.n || U.nnz != S.nnz || S.m < 0 || S.n < 1 || !Dr || !Dc) {
         set_error(what + " has mismatched or empty forms");
         return false;
     }

@@ -1372,8 +1372,11 @@ int check_solve_args(const supr_lp* U, const supr_lp* S, const double* Dr, const
 
 // Shape and option checks shared by every single-LP entry point.
 int check_options(const supr_lp* S, const supr_pdlp_options* opt) {
-    if (S->m < 1 || S->n < 1) {
-        set_error("the PDLP form needs at least one row and one column");
+    // a form without rows (box constraints only) or without nonzeros is solved
+    // like any other, as the reference's pdlp_run does (include/pdlp_algo.hpp:
+    // 67-75: eta0 = 1 without entries, omega0 = 1 without finite rL)
+    if (S->n < 1) {
+        set_error("the PDLP form needs at least one column");
         return SUPR_EINVAL;
     }
     if (!opt) { set_error("null options"); return SUPR_EINVAL; }
@@ -1521,7 +1524,7 @@ int supr_solve_lp_gpu(const supr_lp* lp, const supr_pdlp_options* opt, supr_stop
         fv.m = F.m2; fv.n = F.n2; fv.nnz = F.nnz2;
         fv.c = c2.data(); fv.l = l2.data(); fv.u = u2.data(); fv.rL = rL2.data(); fv.rU = rU2.data();
         fv.obj_offset = lp->obj_offset;
-        if (fv.m < 1 || fv.n < 1) throw BadInput("the PDLP form needs at least one row and one column");
+        if (fv.n < 1) throw BadInput("the PDLP form needs at least one column");
         auto fdev = std::make_unique<DevLp>(F.m2, F.n2, F.nnz2, std::move(F.row), std::move(F.col),
                                             std::move(F.val), std::move(F.c), std::move(F.l),
                                             std::move(F.u), std::move(F.rL), std::move(F.rU), s);

@@ -487,3 +487,62 @@ def test_staged_schedules_solve_a_tall_lp(oracle, mode):
 
 STAGED_OBJ = {}
 
+
+
+def _degenerate_lps():
+    def mk(m, n, rows, cols, vals, c, rL, rU, l, u, name):
+        return P.LP(m, n, np.array(rows, dtype=np.int32), np.array(cols, dtype=np.int32),
+                    np.array(vals, dtype=float), np.array(c, dtype=float), np.array(rL, dtype=float),
+                    np.array(rU, dtype=float), np.array(l, dtype=float), np.array(u, dtype=float), 0.0, name)
+    inf = np.inf
+    return {
+        # box constraints only: the PDLP form has no rows (eta0 = 1, omega0 = 1 in the reference)
+        "no_rows": mk(0, 3, [], [], [], [1, -1, 2], [], [], [0, 0, 0], [1, 2, 3], "no rows"),
+        # rows but no nonzeros (a ranged row and a one-sided row that 0 satisfies)
+        "no_nonzeros": mk(2, 3, [], [], [], [1, -1, 2], [-1, -inf], [1, 5], [0, 0, 0], [1, 2, 3], "no nonzeros"),
+        "one_variable": mk(1, 1, [0], [0], [2.0], [1.0], [2.0], [2.0], [0.0], [inf], "one variable"),
+        # a column with no entries, a free row side, an infinite bound
+        "empty_column": mk(2, 3, [0, 1], [0, 0], [1.0, 1.0], [1, 1, 0], [1, -inf], [inf, 4], [0, 0, -1],
+                           [inf, inf, 1], "empty column"),
+    }
+
+
+DEGENERATE = _degenerate_lps()
+
+
+@pytest.mark.parametrize("name", sorted(DEGENERATE))
+@pytest.mark.parametrize("mode", [P.MODE_FAST_FP64, P.MODE_MIXED, P.MODE_PLAIN_FP64])
+def test_degenerate_lps_match_oracle(oracle, name, mode):
+    """Shapes the reference's pdlp_run accepts though no generator makes them:
+    a form without rows, rows without nonzeros, one variable, an empty
+    column.  Each solve converges, the oracle's verifier certifies 1e-6 and
+    the objective matches the oracle's own solve."""
+    lp = DEGENERATE[name]
+    form, _ = P.build_pdlp_form(lp)
+    s, Dr, Dc = P.scale(form)
+    ref = oracle.pdlp(form, form.with_values(*oracle.scale(form)[:6]), Dr, Dc, 20000, 64, TOL)
+    ref_obj = oracle.verify(form, ref.x, ref.y)[3]
+    r = P.solve_pdhg_gpu(form, s, Dr, Dc, tol=TOL, mode=mode)
+    rep = oracle.verify(form, r.x, r.y)
+    print(f"[parity] {name} mode {mode}: {r.iterations} attempts (oracle {ref.iterations}), eps {rep[:3]}, "
+          f"objective {rep[3]} (oracle {ref_obj})", file=sys.stderr, flush=True)
+    assert ref.converged and r.converged
+    assert max(rep[:3]) <= TOL
+    assert abs(rep[3] - ref_obj) <= OBJ_RTOL * (1 + abs(ref_obj))
+    if mode != P.MODE_PLAIN_FP64:
+        sol = P.solve_lp_gpu(lp, tol=TOL, mode=mode)
+        assert sol.status == "optimal"
+        orig = oracle.verify(lp, sol.x, sol.y)
+        assert max(orig[:3]) <= TOL and abs(orig[3] - ref_obj) <= OBJ_RTOL * (1 + abs(ref_obj))
+
+
+def test_degenerate_lps_in_a_batch(oracle):
+    forms = []
+    for name in sorted(DEGENERATE):
+        form, _ = P.build_pdlp_form(DEGENERATE[name])
+        forms.append((form,) + tuple(P.scale(form)))
+    with P.BatchEngine(forms, tol=TOL, mode=P.MODE_MIXED, tune=False) as eng:
+        res, _ = eng.solve()
+    for (form, *_), r in zip(forms, res):
+        rep = oracle.verify(form, r.x, r.y)
+        assert r.converged and max(rep[:3]) <= TOL, (form.name, rep[:3])
