@@ -123,7 +123,7 @@ typedef struct supr_pdlp_options {
     int32_t verbose;           /* 0 quiet, 1 phase summary on stderr, 2 also a progress line every 5 s, 3 also per-kernel timing (batched) */
 } supr_pdlp_options;
 
-/* Fills the reference's defaults: mode MIXED, max_iterations 200000,
+/* Fills the reference's defaults: mode FAST_FP64 (FP64 arithmetic), max_iterations 200000,
  * check_every 64, tol 1e-6, the rest automatic. */
 void supr_pdlp_default_options(supr_pdlp_options* opt);
 
@@ -349,7 +349,7 @@ int supr_gen_random_lp(int32_t m, int32_t n, int32_t col_nnz, const int32_t* col
  * covers every running instance, a fixed number of blocks per instance, each
  * instance with its own step sizes, checks, restarts and (MIXED) FP32 -> FP64
  * hand-over, so an instance's result does not depend on the other instances
- * of the solve.  Modes: MIXED (default) or FAST_FP64.  opt->spmv_lanes_A == 0
+ * of the solve.  Modes: FAST_FP64 (default) or MIXED.  opt->spmv_lanes_A == 0
  * tunes the lanes-per-row rule (lanes_bias) on a short run at creation;
  * supr_batch_configure sets it and the per-instance block cap (max_blocks)
  * explicitly (defaults 1.0 and 512).  x_out[r] / y_out[r] (may be NULL) receive

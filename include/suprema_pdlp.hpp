@@ -47,13 +47,15 @@ supr_lp view(const LP& r) {
 }
 
 // Engine mode for the drop-in: SUPREMA_PDLP_MODE = plain | fast64 | mixed
-// (default mixed: FP32 iterates with FP64 residuals, restarts and polish).
+// (default fast64: FP64 arithmetic throughout, as the reference's
+// solve_pdhg_gpu; mixed runs FP32 iterates with FP64 residuals, restarts and
+// polish and must be asked for).
 inline int mode_from_env() {
     const char* m = std::getenv("SUPREMA_PDLP_MODE");
-    if (!m) return SUPR_MODE_MIXED;
+    if (!m) return SUPR_MODE_FAST_FP64;
     if (!std::strcmp(m, "plain")) return SUPR_MODE_PLAIN_FP64;
-    if (!std::strcmp(m, "fast64")) return SUPR_MODE_FAST_FP64;
-    return SUPR_MODE_MIXED;
+    if (!std::strcmp(m, "mixed")) return SUPR_MODE_MIXED;
+    return SUPR_MODE_FAST_FP64;
 }
 
 inline int poll_stop(void* ctx) {

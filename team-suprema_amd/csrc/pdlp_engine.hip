@@ -1423,7 +1423,7 @@ int supr_device_count(void) {
 void supr_pdlp_default_options(supr_pdlp_options* opt) {
     if (!opt) return;
     std::memset(opt, 0, sizeof(*opt));
-    opt->mode = SUPR_MODE_MIXED;
+    opt->mode = SUPR_MODE_FAST_FP64;   // the reference's arithmetic; MIXED is opt-in
     opt->max_iterations = 200000;
     opt->check_every = 64;
     opt->tol = 1e-6;

@@ -275,7 +275,7 @@ def _result(info, x, y):
 
 
 def solve_pdhg_gpu(unscaled_form, scaled_form, Dr, Dc, eta=0.99, max_iterations=200000,
-                   check_every=64, tol=1e-6, stop_flag=None, mode=MODE_MIXED, **options):
+                   check_every=64, tol=1e-6, stop_flag=None, mode=MODE_FAST_FP64, **options):
     """solve_pdhg_gpu() -- include/pdlp_gpu.hpp:22-25.
 
     eta is accepted and ignored, as in the reference (the step size is
@@ -300,7 +300,7 @@ def solve_pdhg_gpu(unscaled_form, scaled_form, Dr, Dc, eta=0.99, max_iterations=
 
 
 def solve_form_gpu(form, ruiz_passes=10, integer=None, max_iterations=200000, check_every=64,
-                   tol=1e-6, stop_flag=None, mode=MODE_MIXED, **options):
+                   tol=1e-6, stop_flag=None, mode=MODE_FAST_FP64, **options):
     """scale(form) then solve_pdhg_gpu(form, scaled, Dr, Dc) with the scaling
     on the device (src/solve.cpp:195-207); the same result as
     solve_pdhg_gpu(form, *scale(form)), bit for bit."""
@@ -340,7 +340,7 @@ def certified_status(engine_status, eps, opt_tol):
 
 
 def solve_lp_gpu(lp, max_iterations=200000, check_every=64, tol=1e-6, stop_flag=None,
-                 mode=MODE_MIXED, **options):
+                 mode=MODE_FAST_FP64, **options):
     """solve_mps()'s PDLP stage after presolve on the GPU: form, scale, solve,
     extract, z, verify on lp, and the 10x-tighter retries."""
     o = _options(mode, max_iterations, check_every, tol, **options)
@@ -375,7 +375,7 @@ class Engine:
 
     @classmethod
     def from_form(cls, form, ruiz_passes=10, integer=None, max_iterations=200000,
-                  check_every=64, tol=1e-6, mode=MODE_MIXED, **options):
+                  check_every=64, tol=1e-6, mode=MODE_FAST_FP64, **options):
         self = cls.__new__(cls)
         self._h = ctypes.c_void_p()
         self.m, self.n = form.m, form.n
@@ -388,7 +388,7 @@ class Engine:
         return self
 
     def __init__(self, unscaled_form, scaled_form, Dr, Dc, max_iterations=200000,
-                 check_every=64, tol=1e-6, mode=MODE_MIXED, **options):
+                 check_every=64, tol=1e-6, mode=MODE_FAST_FP64, **options):
         Dr = np.ascontiguousarray(Dr, dtype=_f64)
         Dc = np.ascontiguousarray(Dc, dtype=_f64)
         if Dr.shape != (scaled_form.m,) or Dc.shape != (scaled_form.n,):
@@ -437,7 +437,7 @@ class BatchEngine:
     rule on a short run at creation (block cap 512); otherwise lanes_bias and
     max_blocks are used."""
 
-    def __init__(self, forms, max_iterations=200000, check_every=64, tol=1e-6, mode=MODE_MIXED,
+    def __init__(self, forms, max_iterations=200000, check_every=64, tol=1e-6, mode=MODE_FAST_FP64,
                  tune=True, lanes_bias=1.0, max_blocks=512, **options):
         self._keep = []
         us, ss, drs, dcs = [], [], [], []
@@ -522,7 +522,7 @@ class Server:
     Capacities (max_cols, max_rows, max_nnz, max_instances) bound what is
     resident at once; a request that does not fit waits in the queue."""
 
-    def __init__(self, mode=MODE_MIXED, tol=1e-6, max_iterations=200000, check_every=64,
+    def __init__(self, mode=MODE_FAST_FP64, tol=1e-6, max_iterations=200000, check_every=64,
                  max_cols=1 << 24, max_rows=1 << 23, max_nnz=1 << 27, max_instances=256,
                  max_blocks=1024, lanes_bias=1.0, replays_per_step=1, **options):
         so = _lib.SuprServerOptions()
@@ -542,7 +542,7 @@ class Server:
         return int(b.value)
 
     @staticmethod
-    def request_bytes(m, n, nnz, mode=MODE_MIXED):
+    def request_bytes(m, n, nnz, mode=MODE_FAST_FP64):
         return int(lib().supr_server_request_bytes(int(m), int(n), int(nnz), int(mode)))
 
     def submit(self, unscaled_form, scaled_form, Dr, Dc):

@@ -41,7 +41,8 @@ def test_version_and_device_count():
 def test_default_options_are_the_reference_defaults():
     o = _lib.SuprOptions()
     P.lib().supr_pdlp_default_options(ctypes.byref(o))
-    assert (o.max_iterations, o.check_every, o.tol, o.mode) == (200000, 64, 1e-6, P.MODE_MIXED)
+    # the reference's defaults (include/pdlp.hpp:35-38) and its arithmetic (FP64)
+    assert (o.max_iterations, o.check_every, o.tol, o.mode) == (200000, 64, 1e-6, P.MODE_FAST_FP64)
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")

@@ -102,7 +102,7 @@ def test_server_all_at_once_and_refusals(requests_):
         assert srv.result(12345) is None
         assert srv.step() == (0, 0, 0)
         assert srv.device_bytes > 0
-    assert P.Server.request_bytes(27, 32, 83) > P.Server.request_bytes(27, 32, 83, P.MODE_FAST_FP64)
+    assert P.Server.request_bytes(27, 32, 83, P.MODE_MIXED) > P.Server.request_bytes(27, 32, 83, P.MODE_FAST_FP64)
 
 
 # ------------------------------------------------------------- serving mix --
