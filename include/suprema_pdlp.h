@@ -218,12 +218,14 @@ int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32
 /* Times the two iteration kernels (primal half: A^T y + projection +
  * averages; dual half: A x + dual step + step decision) at this LP's shapes,
  * each launched reps times after a warm-up with every launch doing an
- * accepted step's full work.  out8 = ms per primal-half launch (FP32), ms per
+ * accepted step's full work.  out9 = ms per primal-half launch (FP32), ms per
  * dual-half launch (FP32), the same two in FP64, lanes-per-row of A and of
- * A^T (FP64 choice), ELL width of A and of A^T (0 = CSR). */
+ * A^T (FP64 choice), ELL width of A and of A^T (0 = CSR), 1 if the FP64
+ * primal half forms A^T's values from the scaling factors (every unscaled
+ * entry the same value) instead of reading them. */
 int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                        const double* Dr, const double* Dc, const supr_pdlp_options* opt,
-                       int32_t reps, double* out8);
+                       int32_t reps, double* out9);
 
 /* Times a plain device-to-device copy of `bytes` (a positive multiple of
  * 16) with a 16-byte-vector stream kernel, reps launches after one warm-up;

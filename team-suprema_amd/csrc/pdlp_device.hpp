@@ -31,6 +31,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 namespace supr {
 
@@ -83,6 +84,14 @@ template <typename V> struct MatView {
     // has at most 65,536 entries (A^T of an LP with few rows); null otherwise.
     // The ELL loops read these instead of idx: 2 bytes per slot instead of 4.
     const uint16_t* idx16 = nullptr;
+    // ELL, FP64 values only: when every stored value of row r, slot k equals
+    // csc[idx] * rsc[r] bit for bit (a matrix whose unscaled entries share one
+    // value a: scaled a_ij = (Dr_i a) Dc_j), row_dot forms each value from
+    // these two factors instead of reading it -- one factor per row and one
+    // gathered from a short, cached vector instead of 8 bytes per slot.  The
+    // products are the stored values, so the sums are the same bits.
+    const double* rsc = nullptr;
+    const double* csc = nullptr;
 };
 
 // Iterate buffers of one precision.
@@ -295,8 +304,25 @@ __device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict_
                     s += (Acc)M.val[q] * (Acc)x[ix[q]];
                 }
             };
-            if (M.idx16) run(M.idx16);
-            else run(M.idx);
+            bool done = false;
+            if constexpr (std::is_same<V, double>::value) {
+                if (M.csc) {
+                    const double r = M.rsc[row];
+                    auto rebuilt = [&](const auto* __restrict__ ix) {
+                        for (int k = lane; k < M.ell_w; k += L) {
+                            const int c = ix[(size_t)k * R + row];
+                            s += (Acc)(M.csc[c] * r) * (Acc)x[c];
+                        }
+                    };
+                    if (M.idx16) rebuilt(M.idx16);
+                    else rebuilt(M.idx);
+                    done = true;
+                }
+            }
+            if (!done) {
+                if (M.idx16) run(M.idx16);
+                else run(M.idx);
+            }
         }
     } else {
         int b = 0, e = 0;

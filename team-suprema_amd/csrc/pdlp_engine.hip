@@ -96,6 +96,10 @@ bool valid_lanes(int L) {
 struct DevMatrix {
     int rows = 0, cols = 0, nnz = 0, ell_w = 0, lanes = 1;
     DevBuf ptr, idx, idx16, v32, v64, vu;
+    // ELL, FP64: the factors the device-resident loop forms the values from
+    // (MatView::rsc / csc), set when they reproduce every stored value
+    const double* rsc = nullptr;
+    const double* csc = nullptr;
     template <typename V> MatView<V> view(const DevBuf& v) const {
         MatView<V> m;
         m.rows = rows;
@@ -108,6 +112,13 @@ struct DevMatrix {
     }
     MatView<float> f32() const { return view<float>(v32); }
     MatView<double> f64() const { return view<double>(v64); }
+    // the device-resident loop's FP64 view (the plain mode reads the values)
+    MatView<double> f64_iter() const {
+        MatView<double> m = f64();
+        m.rsc = rsc;
+        m.csc = csc;
+        return m;
+    }
     MatView<double> unscaled() const { return view<double>(vu); }
 };
 
@@ -511,6 +522,8 @@ class Solver {
     Arena arena_;
     std::vector<uint8_t> hineq_;   // inequality flags, host copy of ineq_
     DevBuf ineq_, state_, parts_, x0d_;
+    DevBuf ellc_;   // (Dr_i a) per row of A, when A^T's FP64 values are formed from factors
+    void rebuild_ell_values();
     DevBuf kc_, kl_, ku_, krL_, krU_, kDr_, kDc_, xu_, xu2_, xavg_, yu_, yu2_, yavg_, bestx_, besty_,
         axavg_;
     KktBufs K_{};
@@ -994,6 +1007,7 @@ void Solver::setup_device_resident(bool mixed) {
     std::memset(mirror_, 0, sizeof(DevState));
     SUPR_HIP(hipHostGetDevicePointer((void**)&mirror_dev_, mirror_, 0));
     const double t_kkt = seconds_since(t0);
+    rebuild_ell_values();
 
     // SpMV schedules per iterate precision: given, from the shape rule (0),
     // or timed on scratch vectors (-1)
@@ -1020,6 +1034,38 @@ void Solver::setup_device_resident(bool mixed) {
                      "lanes + sync %.2f ms (A %d/%d, AT %d/%d; %.1f MB in one allocation)\n", 1e3 * t_mat,
                      1e3 * (t_iter - t_mat), 1e3 * (t_kkt - t_iter), 1e3 * (seconds_since(t0) - t_kkt),
                      lanesA_[0], lanesA_[1], lanesAT_[0], lanesAT_[1], arena_.bytes() / 1e6);
+}
+
+// A^T in ELL with FP64 values that are all (Dr_i a) Dc_j for one unscaled
+// value a (transportation, assignment and other matrices whose entries share
+// one value): the FP64 primal half forms each value from Dc_j (one per row of
+// A^T) and Dr_i a (gathered from an m-vector that stays in cache) instead of
+// reading 8 bytes per slot.  Used only if a check on the device finds every
+// stored value equal, bit for bit, to the product the kernel forms, so the
+// iteration is unchanged.  Not in the plain mode (the reference's design);
+// SUPR_ELL_VALUES=stored turns it off.
+void Solver::rebuild_ell_values() {
+    if (o_.mode == SUPR_MODE_PLAIN_FP64 || AT_.ell_w <= 0 || AT_.nnz <= 0 || m_ <= 0) return;
+    if (const char* e = std::getenv("SUPR_ELL_VALUES"); e && std::strcmp(e, "stored") == 0) return;
+    hipStream_t s = stream_.s;
+    double a = 0.0;   // the unscaled value of A^T's first slot
+    SUPR_HIP(hipMemcpyAsync(&a, AT_.vu.as<void>(), sizeof(double), hipMemcpyDeviceToHost, s));
+    SUPR_HIP(hipStreamSynchronize(s));
+    std::vector<double> cs((size_t)m_);
+    for (int i = 0; i < m_; ++i) cs[(size_t)i] = Dr_[i] * a;
+    DevBuf c = upload(cs, s);
+    DevBuf bad = device_vec<uint32_t>(1);
+    SUPR_HIP(hipMemsetAsync(bad.as<void>(), 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_ell_rebuild_check, dim3(grid_elems((long)AT_.nnz)), dim3(kBlock), 0, s, AT_.f64(),
+                       c.as<const double>(), kDc_.as<const double>(), bad.as<uint32_t>());
+    SUPR_HIP(hipGetLastError());
+    uint32_t hb = 1;
+    SUPR_HIP(hipMemcpyAsync(&hb, bad.as<void>(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SUPR_HIP(hipStreamSynchronize(s));
+    if (hb != 0) return;
+    ellc_ = std::move(c);
+    AT_.csc = ellc_.as<double>();
+    AT_.rsc = kDc_.as<double>();
 }
 
 DevState Solver::initial_state(bool mixed) const {
@@ -1050,7 +1096,7 @@ DevState Solver::initial_state(bool mixed) const {
 
 template <typename T> MatView<T> iter_view(const DevMatrix& M);
 template <> MatView<float> iter_view<float>(const DevMatrix& M) { return M.f32(); }
-template <> MatView<double> iter_view<double>(const DevMatrix& M) { return M.f64(); }
+template <> MatView<double> iter_view<double>(const DevMatrix& M) { return M.f64_iter(); }
 
 template <typename T>
 void Solver::enqueue_attempts(const Iter<T>& I, int count) {
@@ -1287,7 +1333,8 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
 // kernels the graph replays), for the roofline figure:
 //   out[0] ms per k_primal, out[1] ms per k_dual (FP32), out[2], out[3] the
 //   same in FP64, out[4] lanes A, out[5] lanes A^T, out[6] ELL width of A,
-//   out[7] ELL width of A^T.
+//   out[7] ELL width of A^T, out[8] 1 if the FP64 primal half forms A^T's
+//   values from factors (rebuild_ell_values).
 // The state is set so every launch does a full accepted step's work (A^T y,
 // average terms, projections, A x) and no decision is pending: k_primal
 // reads S[0] and writes S[1], k_dual reads S[1], so repeated launches see the
@@ -1343,6 +1390,7 @@ void Solver::bench_kernels(int reps, double* out) {
     run(D_, double{}, 2);
     SUPR_HIP(hipStreamSynchronize(s));
     out[4] = A_.lanes; out[5] = AT_.lanes; out[6] = A_.ell_w; out[7] = AT_.ell_w;
+    out[8] = AT_.csc ? 1.0 : 0.0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 }
@@ -1699,11 +1747,11 @@ int supr_spmv(const supr_lp* lp, int32_t transposed, int32_t lanes, int32_t fp32
 // Times the iteration kernels (see Solver::bench_kernels).  out: 8 doubles.
 int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                        const double* Dr, const double* Dc, const supr_pdlp_options* opt,
-                       int32_t reps, double* out8) {
+                       int32_t reps, double* out9) {
     using namespace supr;
     int rc = check_solve_args(unscaled_form, scaled_form, Dr, Dc, opt);
     if (rc != SUPR_OK) return rc;
-    if (!out8 || reps < 1) { set_error("null output or reps < 1"); return SUPR_EINVAL; }
+    if (!out9 || reps < 1) { set_error("null output or reps < 1"); return SUPR_EINVAL; }
     try {
         SUPR_HIP(hipSetDevice(opt->device));
         supr_pdlp_options o = *opt;
@@ -1712,7 +1760,7 @@ int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
         const bool rl = relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R);
         Solver solver(rl ? R.U : *unscaled_form, rl ? R.S : *scaled_form, Dr, rl ? R.dc.data() : Dc, o);
         solver.setup();
-        solver.bench_kernels(reps, out8);
+        solver.bench_kernels(reps, out9);
         return SUPR_OK;
     } catch (...) {
         return map_exception();

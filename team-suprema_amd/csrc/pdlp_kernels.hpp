@@ -507,6 +507,24 @@ __global__ __launch_bounds__(kBlock) void k_ell_from_csr(int32_t rows, int32_t w
     }
 }
 
+// Sets *bad when some ELL slot q (row r) of M holds a value other than
+// csc[idx[q]] * rsc[r], bit for bit -- the product the FP64 row products form
+// instead of reading the value (MatView::rsc / csc).
+__global__ __launch_bounds__(kBlock) void k_ell_rebuild_check(MatView<double> M,
+                                                               const double* __restrict__ csc,
+                                                               const double* __restrict__ rsc,
+                                                               uint32_t* __restrict__ bad) {
+    const int64_t total = (int64_t)M.rows * M.ell_w;
+    uint32_t mine = 0;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = q % M.rows;   // slot-major: q = k * rows + r
+        const double v = csc[M.idx[q]] * rsc[r];
+        mine |= (uint32_t)(__double_as_longlong(v) != __double_as_longlong(M.val[q]));
+    }
+    if (mine) atomicOr(bad, 1u);
+}
+
 // dst = (D) src, elementwise (FP64 -> FP32 rounds to nearest even).
 template <typename S, typename D>
 __global__ __launch_bounds__(kBlock) void k_convert(int64_t len, const S* __restrict__ src,

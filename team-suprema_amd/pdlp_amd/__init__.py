@@ -692,12 +692,12 @@ def bench_kernels(unscaled_form, scaled_form, Dr, Dc, reps=50, **options):
     Dr = np.ascontiguousarray(Dr, dtype=_f64)
     Dc = np.ascontiguousarray(Dc, dtype=_f64)
     o = _options(MODE_MIXED, 200000, 64, 1e-6, **options)
-    out = np.zeros(8)
+    out = np.zeros(9)
     su, ss = unscaled_form.cstruct(), scaled_form.cstruct()
     check(lib().supr_bench_kernels(ctypes.byref(su), ctypes.byref(ss), _dp(Dr), _dp(Dc),
                                    ctypes.byref(o), int(reps), _dp(out)))
     keys = ["primal_ms_fp32", "dual_ms_fp32", "primal_ms_fp64", "dual_ms_fp64", "lanes_A",
-            "lanes_AT", "ell_A", "ell_AT"]
+            "lanes_AT", "ell_A", "ell_AT", "rebuilt_AT_fp64"]
     return dict(zip(keys, (float(v) for v in out)))
 
 

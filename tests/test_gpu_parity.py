@@ -546,3 +546,29 @@ def test_degenerate_lps_in_a_batch(oracle):
     for (form, *_), r in zip(forms, res):
         rep = oracle.verify(form, r.x, r.y)
         assert r.converged and max(rep[:3]) <= TOL, (form.name, rep[:3])
+
+
+@pytest.mark.parametrize("mode", [P.MODE_FAST_FP64, P.MODE_MIXED])
+def test_rebuilt_ell_values_give_the_same_bits(monkeypatch, mode):
+    """A^T's FP64 values formed from the scaling factors (every unscaled entry
+    the same value: transportation LPs) instead of read: the setup check
+    accepts it only if every product equals the stored value, so the solve is
+    the one that reads the values, bit for bit (SUPR_ELL_VALUES=stored turns
+    the rebuild off).  An LP with varied entries is refused by the check."""
+    for S, D in ((60, 60), (300, 400)):
+        lp = P.gen_transport(S, D, seed=5)
+        s, Dr, Dc = P.scale(lp)
+        assert P.bench_kernels(lp, s, Dr, Dc, reps=2)["rebuilt_AT_fp64"] == 1.0
+        on = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode)
+        monkeypatch.setenv("SUPR_ELL_VALUES", "stored")
+        assert P.bench_kernels(lp, s, Dr, Dc, reps=2)["rebuilt_AT_fp64"] == 0.0
+        off = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode)
+        monkeypatch.delenv("SUPR_ELL_VALUES")
+        assert on.converged and on.iterations == off.iterations and on.restarts == off.restarts
+        assert np.array_equal(on.x, off.x) and np.array_equal(on.y, off.y)
+        print(f"[parity] transport {S}x{D} mode {mode}: rebuilt values, {on.iterations} attempts, same bits",
+              file=sys.stderr, flush=True)
+    lp = P.gen_random_lp(200, 500, 6, seed=1)   # A^T is ELL, entries all different
+    s, Dr, Dc = P.scale(lp)
+    kb = P.bench_kernels(lp, s, Dr, Dc, reps=2)
+    assert kb["ell_AT"] > 0 and kb["rebuilt_AT_fp64"] == 0.0
