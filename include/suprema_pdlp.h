@@ -232,6 +232,13 @@ int supr_bench_kernels(const supr_lp* unscaled_form, const supr_lp* scaled_form,
  * *ms_per_copy = ms per launch (reads `bytes`, writes `bytes`). */
 int supr_bench_copy(int64_t bytes, int32_t reps, int32_t device, double* ms_per_copy);
 
+/* No reference counterpart (callers: long-lived processes that make many
+ * calls).  Device memory a call releases is kept per device, up to
+ * SUPR_DEVICE_CACHE_MB megabytes (default 4096; 0 turns the cache off), and
+ * reused by later calls, which then skip hipMalloc / hipFree.  This frees
+ * every kept block now (engines, batches and servers keep what they hold). */
+void supr_release_cached_memory(void);
+
 /* ------------------------------------------------------- host-side form -- */
 /* verify(): out7 = eps_P, eps_D, eps_G, objective, dual_objective,
  * max_row_violation, max_bound_violation (KKTReport, include/verify.hpp:7-11). */

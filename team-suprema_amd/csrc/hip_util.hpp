@@ -37,6 +37,15 @@ inline double seconds_since(Clock::time_point t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
+// Device memory behind DevBuf (pdlp_engine.hip).  Blocks a call releases
+// are kept per device, up to SUPR_DEVICE_CACHE_MB (default 4096; 0 turns the
+// cache off), and handed to later allocations they fit, so repeated calls
+// skip hipMalloc / hipFree; supr_release_cached_memory() frees them.  A block
+// is cached only after the device has finished its work (a device-wide
+// synchronisation, as hipFree does), so a reused block is never still in use.
+void* device_alloc(size_t bytes, size_t* reserved, int* device);
+void device_free(void* p, size_t reserved, int device);
+
 // Device allocation (move-only): owning, or a borrowed view into memory that
 // something else owns (an Arena).
 class DevBuf {
@@ -49,9 +58,11 @@ class DevBuf {
         b.own_ = false;
         return b;
     }
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), own_(o.own_) { o.p_ = nullptr; }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_), dev_(o.dev_), own_(o.own_) { o.p_ = nullptr; }
     DevBuf& operator=(DevBuf&& o) noexcept {
         std::swap(p_, o.p_);
+        std::swap(bytes_, o.bytes_);
+        std::swap(dev_, o.dev_);
         std::swap(own_, o.own_);
         return *this;
     }
@@ -61,7 +72,7 @@ class DevBuf {
     void alloc(size_t bytes) {
         release();
         own_ = true;
-        SUPR_HIP(hipMalloc(&p_, std::max<size_t>(bytes, 256)));
+        p_ = device_alloc(std::max<size_t>(bytes, 256), &bytes_, &dev_);
     }
     void reset() { release(); }
     template <typename T> T* as() const { return static_cast<T*>(p_); }
@@ -69,10 +80,12 @@ class DevBuf {
 
    private:
     void release() {
-        if (p_ && own_) (void)hipFree(p_);
+        if (p_ && own_) device_free(p_, bytes_, dev_);
         p_ = nullptr;
     }
     void* p_ = nullptr;
+    size_t bytes_ = 0;
+    int dev_ = 0;
     bool own_ = true;
 };
 

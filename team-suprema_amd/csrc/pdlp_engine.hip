@@ -35,6 +35,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -48,6 +49,107 @@
 #include "suprema_pdlp.h"
 
 namespace supr {
+
+// ------------------------------------------------------ device memory cache --
+
+namespace {
+struct CachedBlock {
+    void* p;
+    size_t bytes;
+    int device;
+};
+struct DeviceCache {
+    std::mutex mu;
+    std::vector<CachedBlock> blocks;
+    size_t held = 0;
+};
+// never destroyed: no HIP call may run while the process exits
+DeviceCache& device_cache() {
+    static DeviceCache* c = new DeviceCache();
+    return *c;
+}
+size_t device_cache_cap() {
+    static const size_t cap = [] {
+        const char* e = std::getenv("SUPR_DEVICE_CACHE_MB");
+        return (e ? (size_t)std::strtoull(e, nullptr, 10) : (size_t)4096) << 20;
+    }();
+    return cap;
+}
+void release_cached(int device) {   // device < 0: all devices
+    std::vector<CachedBlock> drop;
+    {
+        DeviceCache& c = device_cache();
+        std::lock_guard<std::mutex> g(c.mu);
+        for (size_t k = 0; k < c.blocks.size();) {
+            if (device < 0 || c.blocks[k].device == device) {
+                c.held -= c.blocks[k].bytes;
+                drop.push_back(c.blocks[k]);
+                c.blocks[k] = c.blocks.back();
+                c.blocks.pop_back();
+            } else {
+                ++k;
+            }
+        }
+    }
+    for (const CachedBlock& b : drop) (void)hipFree(b.p);
+}
+}  // namespace
+
+void* device_alloc(size_t bytes, size_t* reserved, int* device) {
+    int dev = 0;
+    SUPR_HIP(hipGetDevice(&dev));
+    {
+        // the smallest cached block of this device that fits, if it is at
+        // most twice the request
+        DeviceCache& c = device_cache();
+        std::lock_guard<std::mutex> g(c.mu);
+        size_t best = c.blocks.size();
+        for (size_t k = 0; k < c.blocks.size(); ++k) {
+            const CachedBlock& b = c.blocks[k];
+            if (b.device == dev && b.bytes >= bytes && b.bytes <= 2 * bytes &&
+                (best == c.blocks.size() || b.bytes < c.blocks[best].bytes))
+                best = k;
+        }
+        if (best < c.blocks.size()) {
+            const CachedBlock b = c.blocks[best];
+            c.blocks[best] = c.blocks.back();
+            c.blocks.pop_back();
+            c.held -= b.bytes;
+            *reserved = b.bytes;
+            *device = dev;
+            return b.p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {   // give the cached blocks back and try again
+        (void)hipGetLastError();
+        release_cached(dev);
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess)
+        throw HipFailure(std::string("HIP error: hipMalloc of ") + std::to_string(bytes) + " bytes -- " +
+                         hipGetErrorString(e));
+    *reserved = bytes;
+    *device = dev;
+    return p;
+}
+
+void device_free(void* p, size_t reserved, int device) {
+    if (!p) return;
+    int cur = -1;
+    const bool same = hipGetDevice(&cur) == hipSuccess && cur == device;
+    if (same && reserved <= device_cache_cap() && hipDeviceSynchronize() == hipSuccess) {
+        DeviceCache& c = device_cache();
+        std::lock_guard<std::mutex> g(c.mu);
+        if (c.held + reserved <= device_cache_cap()) {
+            c.blocks.push_back({p, reserved, device});
+            c.held += reserved;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
 
 namespace {
 
@@ -440,6 +542,7 @@ class Solver {
     }
     void set_tol(double tol) { o_.tol = tol; }
     ~Solver() {
+        if (stream_.s) (void)hipStreamSynchronize(stream_.s);
         for (auto& g : graph_) if (g) (void)hipGraphExecDestroy(g);
         if (mirror_) (void)hipHostFree(mirror_);
     }
@@ -1482,23 +1585,30 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
                         supr_pdlp_info* info) {
     using namespace supr;
+    const auto t0 = Clock::now();
     int rc = check_solve_args(unscaled_form, scaled_form, Dr, Dc, opt);
     if (rc != SUPR_OK) return rc;
     if (!x_out || !y_out || !info) { set_error("null output"); return SUPR_EINVAL; }
+    const double t_args = seconds_since(t0);
     try {
         SUPR_HIP(hipSetDevice(opt->device));
         ColumnRelabel R;
-        if (relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R)) {
-            std::vector<double> xp(R.S.n, 0.0);
-            Solver solver(R.U, R.S, Dr, R.dc.data(), *opt, OwnedLp::Borrow{});
+        const bool rl = relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R);
+        const double t_relabel = seconds_since(t0);
+        double t_run = 0.0;
+        {
+            Solver solver(rl ? R.U : *unscaled_form, rl ? R.S : *scaled_form, Dr, rl ? R.dc.data() : Dc, *opt,
+                          OwnedLp::Borrow{});
             solver.setup();
-            solver.run(stop, stop_ctx, xp.data(), y_out, info);
-            map_back(R.newpos, unscaled_form->c, unscaled_form->obj_offset, xp.data(), x_out, info);
-            return SUPR_OK;
+            std::vector<double> xp(rl ? (size_t)R.S.n : 0, 0.0);
+            solver.run(stop, stop_ctx, rl ? xp.data() : x_out, y_out, info);
+            if (rl) map_back(R.newpos, unscaled_form->c, unscaled_form->obj_offset, xp.data(), x_out, info);
+            t_run = seconds_since(t0);
         }
-        Solver solver(*unscaled_form, *scaled_form, Dr, Dc, *opt, OwnedLp::Borrow{});
-        solver.setup();
-        solver.run(stop, stop_ctx, x_out, y_out, info);
+        if (opt->verbose)
+            std::fprintf(stderr, "[suprema] one-shot call: argument checks %.2f ms, relabel rule %.2f ms, "
+                         "setup + solve %.2f ms, teardown %.2f ms\n", 1e3 * t_args,
+                         1e3 * (t_relabel - t_args), 1e3 * (t_run - t_relabel), 1e3 * (seconds_since(t0) - t_run));
         return SUPR_OK;
     } catch (...) {
         return map_exception();
@@ -1809,3 +1919,5 @@ int supr_bench_copy(int64_t bytes, int32_t reps, int32_t device, double* ms_per_
 }
 
 }  // extern "C"
+
+void supr_release_cached_memory(void) { supr::release_cached(-1); }

@@ -701,6 +701,12 @@ def bench_kernels(unscaled_form, scaled_form, Dr, Dc, reps=50, **options):
     return dict(zip(keys, (float(v) for v in out)))
 
 
+def release_cached_memory():
+    """Frees the device memory the library keeps for reuse between calls
+    (see supr_release_cached_memory)."""
+    lib().supr_release_cached_memory()
+
+
 def bench_copy(nbytes=1 << 30, reps=20, device=0):
     """ms per launch of a plain device copy of nbytes (supr_bench_copy)."""
     ms = ctypes.c_double()

@@ -154,6 +154,7 @@ SIGNATURES = [
       ctypes.POINTER(SuprInfo), c_int32_p]),
     ("supr_server_destroy", None, [ctypes.c_void_p]),
     ("supr_bench_copy", ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_double_p]),
+    ("supr_release_cached_memory", None, []),
     ("supr_gen_blend", ctypes.c_int,
      [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_int32_p, c_int32_p] + [c_double_p] * 6),
     ("supr_gen_blend_sizes", ctypes.c_int,

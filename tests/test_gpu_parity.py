@@ -572,3 +572,19 @@ def test_rebuilt_ell_values_give_the_same_bits(monkeypatch, mode):
     s, Dr, Dc = P.scale(lp)
     kb = P.bench_kernels(lp, s, Dr, Dc, reps=2)
     assert kb["ell_AT"] > 0 and kb["rebuilt_AT_fp64"] == 0.0
+
+
+def test_device_memory_cache_reuse_and_release():
+    """Device memory a call releases is kept and reused by the next call (no
+    hipMalloc / hipFree between one-shot calls); results do not depend on
+    whether a block was fresh or reused, and releasing the cache between calls
+    changes nothing either."""
+    lp = P.gen_transport(200, 300, seed=9)
+    s, Dr, Dc = P.scale(lp)
+    runs = [P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL) for _ in range(2)]
+    P.release_cached_memory()
+    runs.append(P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL))
+    for r in runs[1:]:
+        assert r.iterations == runs[0].iterations
+        assert np.array_equal(r.x, runs[0].x) and np.array_equal(r.y, runs[0].y)
+    P.release_cached_memory()
