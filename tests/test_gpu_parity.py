@@ -588,3 +588,30 @@ def test_device_memory_cache_reuse_and_release():
         assert r.iterations == runs[0].iterations
         assert np.array_equal(r.x, runs[0].x) and np.array_equal(r.y, runs[0].y)
     P.release_cached_memory()
+
+
+def test_device_memory_cache_off_gives_the_same_bits(tmp_path):
+    """SUPR_DEVICE_CACHE_MB=0 (every block freed at once, as without the
+    cache): a fresh process solves the same LP to the same bits."""
+    import json
+    import os
+    import subprocess
+    from conftest import ROOT
+    lp = P.gen_transport(120, 150, seed=4)
+    s, Dr, Dc = P.scale(lp)
+    here = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL)
+    script = tmp_path / "solve.py"
+    script.write_text(
+        "import json, sys\n"
+        f"sys.path.insert(0, {os.path.join(ROOT, 'team-suprema_amd')!r})\n"
+        "import pdlp_amd as P\n"
+        "lp = P.gen_transport(120, 150, seed=4)\n"
+        "s, Dr, Dc = P.scale(lp)\n"
+        "r = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=1e-6)\n"
+        "print(json.dumps({'iterations': r.iterations, 'x': r.x.tobytes().hex()[:4096], 'sum': float(r.x.sum())}))\n")
+    env = dict(os.environ, SUPR_DEVICE_CACHE_MB="0")
+    out = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=120, env=env)
+    assert out.returncode == 0, out.stderr[-2000:]
+    got = json.loads(out.stdout.strip().splitlines()[-1])
+    assert got["iterations"] == here.iterations
+    assert got["x"] == here.x.tobytes().hex()[:4096] and got["sum"] == float(here.x.sum())
