@@ -112,8 +112,12 @@ typedef struct supr_pdlp_options {
     double tol;                /* converged iff max(eps_P, eps_D, eps_G) <= tol */
     double switch_tol;         /* MIXED: FP32 -> FP64 when merit <= switch_tol (0: auto) */
     int32_t stall_checks;      /* MIXED: ... or after this many checks without a new best (0: auto) */
-    int32_t graph_attempts;    /* device-resident modes: attempts per graph replay (0: auto = 16); < 0: the same
-                                  kernel sequence launched on the stream directly, -graph_attempts attempts per poll */
+    int32_t graph_attempts;    /* device-resident modes: attempts per graph replay, each replay ending with the KKT
+                                  check group; < 0: the same kernel sequence launched on the stream directly,
+                                  -graph_attempts attempts per poll; 0 (auto): A with 2^20+ nonzeros launches
+                                  directly, 16 attempts per poll, smaller LPs replay 8 x (16 attempts + check group)
+                                  per graph (single-LP engine; the batched engine and the server: 16 per replay).
+                                  Every choice runs the same kernels in the same order: the same bits. */
     int32_t spmv_lanes_A;      /* lanes per row for A: 0 = a rule of the matrix shape (deterministic: every
                                   run sums in the same order); -1 = timed autotune (may differ run to run);
                                   1..64 powers of two, or 256 = one block per row.  The batched engine:

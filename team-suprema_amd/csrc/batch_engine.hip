@@ -284,7 +284,6 @@ struct Pool {
     std::vector<int64_t> nnz_slot;
     std::vector<DevState> init;
     std::vector<double> meanA, meanAT;
-    std::vector<std::vector<double>> c_unscaled;
     std::vector<double> obj_offset;
     double bias = 1.0;
     int cap = 32;
@@ -330,7 +329,6 @@ struct Pool {
         init.resize(Scap);
         meanA.assign(Scap, 0.0);
         meanAT.assign(Scap, 0.0);
-        c_unscaled.resize(Scap);
         obj_offset.assign(Scap, 0.0);
         BATCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         arr<int32_t>(ptrA, Mcap); arr<int32_t>(idxA, Zcap);
@@ -453,7 +451,6 @@ struct Pool {
         meanAT[k] = p.meanAT;
         shape_rule(k);
         init[k] = p.init;
-        c_unscaled[k] = p.cu;
         obj_offset[k] = p.init.obj_offset;
         const hipStream_t st = stream;
         auto put = [st](auto* dst, const auto& v) {
@@ -512,7 +509,6 @@ struct Pool {
         if (nnz_slot[k] > 0) az.release(I.za, nnz_slot[k]);
         nnz_slot[k] = 0;
         I = BatchInst{};
-        c_unscaled[k].clear();
         free_slots.push_back(k);
     }
 
@@ -686,13 +682,10 @@ struct Pool {
         inf.lanes_A = I.LA; inf.lanes_AT = I.LAT;
         double obj = obj_offset[k];
         if (s.has_best) {
-            // the returned point and its objective c'x + offset (unscaled form)
-            std::vector<double> xh(I.n);
-            BATCH_HIP(hipMemcpy(xh.data(), K.best_x + I.co, sizeof(double) * I.n, hipMemcpyDeviceToHost));
-            double cx = 0.0;
-            for (int j = 0; j < I.n; ++j) cx += c_unscaled[k][j] * xh[j];
-            obj = cx + obj_offset[k];
-            if (x_out) std::copy(xh.begin(), xh.end(), x_out);
+            // the returned point; its objective c'x + offset (unscaled form)
+            // was formed by the KKT check that kept it
+            obj = s.best_obj;
+            if (x_out) BATCH_HIP(hipMemcpy(x_out, K.best_x + I.co, sizeof(double) * I.n, hipMemcpyDeviceToHost));
             if (y_out) BATCH_HIP(hipMemcpy(y_out, K.best_y + I.ro, sizeof(double) * I.m, hipMemcpyDeviceToHost));
         }
         inf.objective = obj;

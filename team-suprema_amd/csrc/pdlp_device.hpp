@@ -53,6 +53,7 @@ struct DevState {
     double best_eps[3];
     double last_eps[3];      // eps of the candidate chosen at the last check
     double last_obj;
+    double best_obj;         // c'x + offset of the point kept as best (unscaled form)
     double tol, switch_tol;
     double denom_p, denom_d, obj_offset;
     // (accepted + 2)^-0.3 and ^-0.6 for the next step decision, formed by a
@@ -791,6 +792,7 @@ __device__ __forceinline__ void kkt_fin_update(DevState* st, const double (&sn)[
     if (mc_avg < st->best_merit) { st->best_merit = mc_avg; bw = 2; }
     if (bw) {
         for (int q = 0; q < 3; ++q) st->best_eps[q] = eps[bw - 1][q];
+        st->best_obj = obj[bw - 1];
         st->has_best = 1;
     }
     const int use_avg = mc_avg < mc_cur;
@@ -806,6 +808,7 @@ __device__ __forceinline__ void kkt_fin_update(DevState* st, const double (&sn)[
         st->done = 1;
         st->best_which = use_avg ? 2 : 1;
         for (int q = 0; q < 3; ++q) st->best_eps[q] = rep[q];
+        st->best_obj = obj[use_avg];
         st->has_best = 1;
         return;
     }

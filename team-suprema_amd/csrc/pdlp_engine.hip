@@ -496,16 +496,13 @@ bool relabel_columns(const supr_lp& U, const supr_lp& S, const double* Dc, int m
     return true;
 }
 
-// x in the caller's column order from the relabelled solve's xp, and the
-// objective c'x + offset summed in that order (as the solver sums it).
-void map_back(const std::vector<int32_t>& newpos, const double* c, double obj_offset,
-              const double* xp, double* x_out, supr_pdlp_info* info) {
+// x in the caller's column order from the relabelled solve's xp (the
+// objective is the solver's: the same terms, summed by the device's check).
+void map_back(const std::vector<int32_t>& newpos, const double* xp, double* x_out,
+              const supr_pdlp_info* info) {
     if (!info->has_point) return;   // x_out untouched, as the solver leaves it
     const size_t n = newpos.size();
     for (size_t j = 0; j < n; ++j) x_out[j] = xp[newpos[j]];
-    double obj = 0.0;
-    for (size_t j = 0; j < n; ++j) obj += c[j] * x_out[j];
-    info->objective = obj + obj_offset;
 }
 
 class Solver {
@@ -638,13 +635,13 @@ class Solver {
 
     template <typename T> void take_iter(Arena& a, Iter<T>& I);
     void take_buffers(Arena& a, bool mixed);
-    template <typename T> void reset_iter(Iter<T>& I);
+    template <typename T> void reset_iter(Iter<T>& I, const DevState* st = nullptr);
     void setup_device_resident(bool mixed);
     DevState initial_state(bool mixed) const;
     template <typename T> void enqueue_attempts(const Iter<T>& I, int count);
     template <typename T> void enqueue_check(const Iter<T>& I);
     template <typename T> void set_products(const Iter<T>& I, const MatView<T>& A);
-    template <typename T> hipGraphExec_t capture_graph(const Iter<T>& I, int attempts);
+    template <typename T> hipGraphExec_t capture_graph(const Iter<T>& I, int attempts, int units);
     template <typename T>
     int run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx, DevState& hs,
                        bool& stopped);
@@ -1028,25 +1025,16 @@ void Solver::take_buffers(Arena& a, bool mixed) {
     state_ = a.take<DevState>(2);   // double-buffered (see pdlp_kernels.hpp)
 }
 
-// Cold start: x = x0, y = 0, sums cleared, restart point = start point (all
-// on the stream, from the device copy of x0).
+// Cold start: x = x0, y = 0, sums cleared, restart point = start point, and
+// (st given) both halves of the state set: one launch on the stream, from
+// the device copy of x0, with the state passed by value.
 template <typename T>
-void Solver::reset_iter(Iter<T>& I) {
-    hipStream_t s = stream_.s;
-    const double* x0 = x0d_.as<double>();
-    if constexpr (sizeof(T) == 8) {
-        SUPR_HIP(hipMemcpyAsync(I.x[0].template as<T>(), x0, sizeof(T) * n_, hipMemcpyDeviceToDevice, s));
-        SUPR_HIP(hipMemcpyAsync(I.xlast.template as<T>(), x0, sizeof(T) * n_, hipMemcpyDeviceToDevice, s));
-    } else {
-        const int g = grid_elems(n_);
-        hipLaunchKernelGGL((k_convert<double, T>), dim3(g), dim3(kBlock), 0, s, (int64_t)n_, x0, I.x[0].template as<T>());
-        hipLaunchKernelGGL((k_convert<double, T>), dim3(g), dim3(kBlock), 0, s, (int64_t)n_, x0, I.xlast.template as<T>());
-        SUPR_HIP(hipGetLastError());
-    }
-    SUPR_HIP(hipMemsetAsync(I.y[0].template as<void>(), 0, sizeof(T) * m_, s));
-    SUPR_HIP(hipMemsetAsync(I.ylast.template as<void>(), 0, sizeof(T) * m_, s));
-    SUPR_HIP(hipMemsetAsync(I.xsum.template as<void>(), 0, sizeof(T) * n_, s));
-    SUPR_HIP(hipMemsetAsync(I.ysum.template as<void>(), 0, sizeof(T) * m_, s));
+void Solver::reset_iter(Iter<T>& I, const DevState* st) {
+    const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
+    hipLaunchKernelGGL((k_reset<T>), dim3(grid_elems(std::max(n_, m_))), dim3(kBlock), 0, stream_.s, n_, m_,
+                       x0d_.as<const double>(), B, st ? state_.as<DevState>() : nullptr,
+                       st ? *st : DevState{});
+    SUPR_HIP(hipGetLastError());
 }
 
 void Solver::setup_device_resident(bool mixed) {
@@ -1219,12 +1207,11 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
 }
 
 // A x of the current iterate with the plain SpMV (start-up and the FP32 ->
-// FP64 switch); outside any graph.  (A^T y is formed by every primal launch.)
+// FP64 switch), on the stream; outside any graph.  (A^T y is formed by every
+// primal launch.)
 template <typename T>
 void Solver::set_products(const Iter<T>& I, const MatView<T>& A) {
-    hipStream_t s = stream_.s;
-    launch_spmv(A_, A, I.x[0].template as<T>(), I.ax[0].template as<T>(), s);
-    SUPR_HIP(hipStreamSynchronize(s));
+    launch_spmv(A_, A, I.x[0].template as<T>(), I.ax[0].template as<T>(), stream_.s);
 }
 
 template <int LA>
@@ -1255,14 +1242,16 @@ void Solver::enqueue_check(const Iter<T>& I) {
 }
 
 template <typename T>
-hipGraphExec_t Solver::capture_graph(const Iter<T>& I, int attempts) {
+hipGraphExec_t Solver::capture_graph(const Iter<T>& I, int attempts, int units) {
     hipStream_t s = stream_.s;
     hipGraph_t g = nullptr;
     hipGraphExec_t exec = nullptr;
     SUPR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-        enqueue_attempts(I, attempts);
-        enqueue_check(I);
+        for (int u = 0; u < units; ++u) {
+            enqueue_attempts(I, attempts);
+            enqueue_check(I);
+        }
     } catch (...) {
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -1284,12 +1273,23 @@ template <typename T>
 int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* ctx,
                            DevState& hs, bool& stopped) {
     hipStream_t s = stream_.s;
-    // graph_attempts < 0: the same kernel sequence launched on the stream
-    // directly (|graph_attempts| attempts + the check group per poll unit).
-    const bool direct = o_.graph_attempts < 0;
+    // A poll unit is K times (G attempts + the check group).
+    //   graph_attempts > 0: one graph replay of G = graph_attempts, K = 1;
+    //   graph_attempts < 0: the same kernels launched on the stream directly,
+    //                       G = -graph_attempts, K = 1;
+    //   0 (auto): matrices of 2^20+ nonzeros launch directly (G = 16): each
+    //     launch runs several us longer than the host takes to enqueue it,
+    //     and direct launches skip the graph-launch boundary (on MI355X about
+    //     1 ms less per solve of the 1M-variable LP than 16-attempt replays);
+    //     smaller ones replay graphs of 8 units of 16 attempts (the random
+    //     2,000 x 5,000 LP: 10.3 against 10.8 us per attempt at one unit, and
+    //     12.1 launched directly, where the host cannot keep ahead).
+    const bool autodirect = o_.graph_attempts == 0 && A_.nnz >= (1L << 20);
+    const bool direct = o_.graph_attempts < 0 || autodirect;
     int G = o_.graph_attempts != 0 ? std::abs(o_.graph_attempts) : 16;
-    G += G & 1;   // the state double buffer needs an even number of attempts per replay
-    if (!direct && !graph_[prec]) graph_[prec] = capture_graph(I, G);
+    G += G & 1;   // the state double buffer needs an even number of attempts per unit
+    const int K = o_.graph_attempts == 0 && !direct ? 8 : 1;
+    if (!direct && !graph_[prec]) graph_[prec] = capture_graph(I, G, K);
     hipGraphExec_t exec = graph_[prec];
     constexpr int R = 2;   // replays in flight: a replay takes far longer than the host's poll
     hipEvent_t ev[R] = {};
@@ -1324,8 +1324,10 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
             }
             const int slot = replays % R;
             if (direct) {
-                enqueue_attempts(I, G);
-                enqueue_check(I);
+                for (int u = 0; u < K; ++u) {
+                    enqueue_attempts(I, G);
+                    enqueue_check(I);
+                }
             } else {
                 SUPR_HIP(hipGraphLaunch(exec, s));
             }
@@ -1346,12 +1348,14 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
 void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, double* x_out,
                                  double* y_out, supr_pdlp_info* info) {
     hipStream_t s = stream_.s;
+    const auto t0 = Clock::now();
     const int prec0 = mixed ? 0 : 1;
     use_lanes(prec0);
-    if (mixed) reset_iter(F_);
-    else reset_iter(D_);
-    const DevState st0[2] = {initial_state(mixed), initial_state(mixed)};
-    SUPR_HIP(hipMemcpyAsync(state_.as<void>(), st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    // the cold start, the state and A x0 are enqueued without a host wait:
+    // the loop's first launches follow them on the stream
+    const DevState st0 = initial_state(mixed);
+    if (mixed) reset_iter(F_, &st0);
+    else reset_iter(D_, &st0);
     if (mixed) set_products(F_, A_.f32());
     else set_products(D_, A_.f64());
     const auto t1 = Clock::now();
@@ -1377,14 +1381,19 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
             SUPR_HIP(hipMemcpyAsync(state_.as<void>(), both, sizeof(both), hipMemcpyHostToDevice, s));
             use_lanes(1);
             set_products(D_, A_.f64());
+            SUPR_HIP(hipStreamSynchronize(s));
             run_graph_loop(D_, 1, stop, ctx, hs, stopped);
         }
     } else {
         run_graph_loop(D_, 1, stop, ctx, hs, stopped);
     }
     const double t_solve = seconds_since(t1);
+    const double t_prep = std::chrono::duration<double>(t1 - t0).count();
 
     bool has_point = hs.has_best != 0;
+    // c'x + offset of the returned point: the KKT check formed it on the
+    // device (FP64, fixed order) when it kept that point
+    double obj = has_point ? hs.best_obj : U_.obj_offset;
     if (has_point) {
         SUPR_HIP(hipMemcpy(x_out, bestx_.as<void>(), sizeof(double) * n_, hipMemcpyDeviceToHost));
         SUPR_HIP(hipMemcpy(y_out, besty_.as<void>(), sizeof(double) * m_, hipMemcpyDeviceToHost));
@@ -1412,6 +1421,7 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
             std::copy(xh.begin(), xh.end(), x_out);
             std::copy(yh.begin(), yh.end(), y_out);
             hs.best_eps[0] = rep[0]; hs.best_eps[1] = rep[1]; hs.best_eps[2] = rep[2];
+            obj = rep[3];
         }
     }
     info->iterations = hs.attempts;
@@ -1419,17 +1429,16 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
     info->converged = hs.done == 1 ? 1 : 0;
     info->has_point = has_point ? 1 : 0;
     info->eps_P = hs.best_eps[0]; info->eps_D = hs.best_eps[1]; info->eps_G = hs.best_eps[2];
-    double obj = U_.obj_offset;
-    if (has_point) { obj = 0.0; for (int j = 0; j < n_; ++j) obj += U_.c[j] * x_out[j]; obj += U_.obj_offset; }
     info->objective = obj;
+    const double t_out = seconds_since(t1) - t_solve;
     info->iterations_fp32 = fp32_attempts;
     info->checks = hs.checks;
     info->lanes_A = A_.lanes; info->lanes_AT = AT_.lanes;
     info->seconds_solve = t_solve;
     if (o_.verbose)
-        std::fprintf(stderr, "[suprema] done: attempts=%d (fp32 %d) accepted=%d restarts=%d checks=%d eps=%.2e/%.2e/%.2e conv=%d solve=%.4fs setup=%.4fs lanes A=%d AT=%d ell A=%d AT=%d\n",
+        std::fprintf(stderr, "[suprema] done: attempts=%d (fp32 %d) accepted=%d restarts=%d checks=%d eps=%.2e/%.2e/%.2e conv=%d solve=%.4fs (start %.2f ms, loop, result %.2f ms) setup=%.4fs lanes A=%d AT=%d ell A=%d AT=%d\n",
                      hs.attempts, fp32_attempts, hs.accepted, hs.restarts, hs.checks, info->eps_P, info->eps_D, info->eps_G,
-                     info->converged, t_solve, setup_s_, A_.lanes, AT_.lanes, A_.ell_w, AT_.ell_w);
+                     info->converged, t_solve, 1e3 * t_prep, 1e3 * t_out, setup_s_, A_.lanes, AT_.lanes, A_.ell_w, AT_.ell_w);
 }
 
 // Standalone timing of the two iteration kernels at their real shapes (the
@@ -1602,7 +1611,7 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
             solver.setup();
             std::vector<double> xp(rl ? (size_t)R.S.n : 0, 0.0);
             solver.run(stop, stop_ctx, rl ? xp.data() : x_out, y_out, info);
-            if (rl) map_back(R.newpos, unscaled_form->c, unscaled_form->obj_offset, xp.data(), x_out, info);
+            if (rl) map_back(R.newpos, xp.data(), x_out, info);
             t_run = seconds_since(t0);
         }
         if (opt->verbose)
@@ -1618,11 +1627,9 @@ int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form
 struct supr_engine {
     int device;
     std::unique_ptr<supr::Solver> solver;
-    // columns relabelled for the solver (empty: as given), with the caller's
-    // objective to map results back
+    // columns relabelled for the solver (empty: as given): x is mapped back
+    // through newpos from xp
     std::vector<int32_t> newpos;
-    std::vector<double> c;
-    double obj_offset = 0.0;
     std::vector<double> xp;
 };
 
@@ -1770,8 +1777,6 @@ int supr_engine_create(const supr_lp* unscaled_form, const supr_lp* scaled_form,
         if (relabel_columns(*unscaled_form, *scaled_form, Dc, opt->mode, R)) {
             e->solver = std::make_unique<Solver>(R.U, R.S, Dr, R.dc.data(), *opt);
             e->newpos = std::move(R.newpos);
-            e->c.assign(unscaled_form->c, unscaled_form->c + unscaled_form->n);
-            e->obj_offset = unscaled_form->obj_offset;
             e->xp.assign(unscaled_form->n, 0.0);
         } else {
             e->solver = std::make_unique<Solver>(*unscaled_form, *scaled_form, Dr, Dc, *opt);
@@ -1795,7 +1800,7 @@ int supr_engine_solve(supr_engine* engine, supr_stop_fn stop, void* stop_ctx, do
         SUPR_HIP(hipSetDevice(engine->device));
         if (!engine->newpos.empty()) {
             engine->solver->run(stop, stop_ctx, engine->xp.data(), y_out, info);
-            map_back(engine->newpos, engine->c.data(), engine->obj_offset, engine->xp.data(), x_out, info);
+            map_back(engine->newpos, engine->xp.data(), x_out, info);
             return SUPR_OK;
         }
         engine->solver->run(stop, stop_ctx, x_out, y_out, info);

@@ -525,6 +525,31 @@ __global__ __launch_bounds__(kBlock) void k_ell_rebuild_check(MatView<double> M,
     if (mine) atomicOr(bad, 1u);
 }
 
+// Cold start of the device-resident loop in one launch: x = xlast = x0
+// (rounded to T), y = ylast = 0, the averages' sums cleared, and, when S is
+// given, both halves of the state set to st (passed by value).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reset(int n, int m, const double* __restrict__ x0,
+                                                  IterBufs<T> B, DevState* __restrict__ S,
+                                                  DevState st) {
+    const int stride = gridDim.x * kBlock;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+        const T v = (T)x0[j];
+        B.x[0][j] = v;
+        B.xlast[j] = v;
+        B.xsum[j] = T(0);
+    }
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += stride) {
+        B.y[0][i] = T(0);
+        B.ylast[i] = T(0);
+        B.ysum[i] = T(0);
+    }
+    if (S && blockIdx.x == 0 && threadIdx.x == 0) {
+        S[0] = st;
+        S[1] = st;
+    }
+}
+
 // dst = (D) src, elementwise (FP64 -> FP32 rounds to nearest even).
 template <typename S, typename D>
 __global__ __launch_bounds__(kBlock) void k_convert(int64_t len, const S* __restrict__ src,

@@ -101,6 +101,10 @@ def _check_against_reference(oracle, case, form, res):
     assert max(rep[0], rep[1], rep[2]) <= TOL, f"{case['name']}: oracle verify {rep[:3]}"
     # the engine's own eps are the oracle verifier's within FP64 reduction order
     np.testing.assert_allclose([res.eps_P, res.eps_D, res.eps_G], rep[:3], rtol=1e-6, atol=1e-12)
+    # the engine's objective (summed by the device check) is the oracle's c'x
+    # + offset of the returned point to FP64 summation-order rounding
+    scale = 1.0 + float(np.dot(np.abs(form.c), np.abs(res.x))) + abs(form.obj_offset)
+    assert abs(res.info["objective"] - rep[3]) <= 1e-13 * scale, (res.info["objective"], rep[3])
     ref_obj = case["ref"]["objective_form"]
     err = abs(rep[3] - ref_obj) / (1 + abs(ref_obj))
     print(f"[parity] {case['name']}: {res.iterations} attempts (reference {case['ref']['iterations']}), "
@@ -355,15 +359,18 @@ def test_cpp_drop_in_api():
 @pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
 def test_direct_launches_match_graph_replay(mode):
     """graph_attempts < 0 launches the replay's kernel sequence on the stream
-    directly: the same kernels in the same order, so the same bits."""
+    directly, and 0 (auto) replays 8 units of 16 attempts and the check group
+    per graph on an LP this small: the same kernels in the same order, so the
+    same bits (x, y and the objective)."""
     lp = P.gen_random_lp(300, 800, 6, seed=21)
     s, Dr, Dc = P.scale(lp)
     runs = [P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, graph_attempts=g,
-                             spmv_lanes_A=4, spmv_lanes_AT=1) for g in (16, -16, -6)]
+                             spmv_lanes_A=4, spmv_lanes_AT=1) for g in (16, 0, -16, -6)]
     assert runs[0].converged
     for r in runs[1:]:
         assert r.iterations == runs[0].iterations and r.restarts == runs[0].restarts
         assert np.array_equal(r.x, runs[0].x) and np.array_equal(r.y, runs[0].y)
+        assert r.info["objective"] == runs[0].info["objective"]
 
 
 @pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
@@ -412,8 +419,9 @@ def test_skewed_columns_are_relabelled_and_mapped_back(monkeypatch, mode):
     with its columns relabelled by degree: the one-shot call and the engine
     give exactly the solve of the LP relabelled here by the same rule (with the
     library's own relabelling off, SUPR_COLUMN_ORDER=given), x mapped back to
-    the caller's order -- same bits, same attempts, the objective summed in
-    the caller's order."""
+    the caller's order -- same bits, same attempts, the same objective (the
+    device check's sum of the relabelled terms), which agrees with c'x summed
+    in the caller's order to rounding."""
     lp = P.gen_power_law_lp(20000, 80000, seed=5)
     deg = np.bincount(lp.col, minlength=lp.n)
     assert lp.n >= 1 << 16 and deg.max() >= 4 * lp.nnz / lp.n
@@ -437,10 +445,9 @@ def test_skewed_columns_are_relabelled_and_mapped_back(monkeypatch, mode):
     for r in (auto, via_engine):
         assert r.iterations == ref.iterations and r.restarts == ref.restarts
         assert np.array_equal(r.x, ref.x[newpos]) and np.array_equal(r.y, ref.y)
-        obj = 0.0
-        for cj, xj in zip(lp.c.tolist(), r.x.tolist()):
-            obj += cj * xj
-        assert r.info["objective"] == obj + lp.obj_offset
+        assert r.info["objective"] == ref.info["objective"]
+        obj = float(np.dot(lp.c, r.x)) + lp.obj_offset
+        assert abs(r.info["objective"] - obj) <= 1e-13 * (1 + float(np.dot(np.abs(lp.c), np.abs(r.x))))
 
 
 def test_cpu_gpu_race_through_the_drop_in():
