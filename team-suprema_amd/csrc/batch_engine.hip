@@ -634,7 +634,7 @@ struct Pool {
         hipGraphExec_t exec = nullptr;
         BATCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
-            enqueue_replay(g, live, G);
+            for (int u = 0; u < units; ++u) enqueue_replay(g, live, G);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &gr);
             if (gr) (void)hipGraphDestroy(gr);
@@ -657,8 +657,9 @@ struct Pool {
     bool direct() const { return opt.graph_attempts < 0; }
 
     // One replay: the captured graph, or the same launches made directly.
+    int units = 1;   // units of (G attempts + check group) per replay or poll
     void launch(hipGraphExec_t exec, const BatchGeo& g, int live, int G) {
-        if (direct()) enqueue_replay(g, live, G);
+        if (direct()) for (int u = 0; u < units; ++u) enqueue_replay(g, live, G);
         else BATCH_HIP(hipGraphLaunch(exec, stream));
     }
 
@@ -735,6 +736,10 @@ struct Batch {
         pool.cold_start(live, max_iter);
         BatchGeo g = pool.layout(live);
         const int G = pool.graph_attempts();
+        // auto: 4 units (G attempts + the check group) per replay or poll,
+        // fewer graph-launch boundaries (the 512-LP blending batch: 6.27 ->
+        // 6.19 s, the same bits); an explicit graph_attempts: one unit
+        pool.units = pool.opt.graph_attempts == 0 ? 4 : 1;
         hipGraphExec_t exec = nullptr;
         constexpr int R = 4;
         hipEvent_t ev[R] = {};
