@@ -608,6 +608,263 @@ __device__ __forceinline__ void row_dot_diff_pipe(const MatView<float>& M,
     sd = group_sum<L, float>(sd);
 }
 
+// ---------------------------------------------- one-pass launches --
+//
+// When a launch's grid covers every row in a single pass (no grid-stride
+// loop, no LDS-staged tiles), a lane's first matrix slots and its row's
+// epilogue operands do not depend on the step decision or on the state, so
+// the one-pass kernels load them before the decision prologue: their
+// latency overlaps the prologue's loads instead of following them.  On a
+// small LP an iteration kernel is a chain of such latencies, not bytes.  The
+// row products add the same terms in the same order as row_dot /
+// row_dot_pipe / row_dot_diff (the prefetched group first, then the rest
+// of the row exactly as those loops run it), so the bits are the same.
+constexpr int kPreU = 4;
+template <typename V> struct PreSlots {
+    int next = 0, end = 0;   // CSR: the lane's next entry and the row end; ELL: next slot and width
+    int c[kPreU];
+    V v[kPreU];
+    bool ok[kPreU];
+    double r = 0.0;          // ELL values formed from factors: rsc[row]
+};
+
+template <int L, typename V>
+__device__ __forceinline__ void pre_slots(const MatView<V>& M, int row, int lane, bool valid,
+                                          PreSlots<V>& P) {
+    if (M.ell_w > 0) {
+        const size_t R = (size_t)M.rows;
+        bool rebuilt = false;
+        if constexpr (std::is_same<V, double>::value) rebuilt = M.csc != nullptr;
+#pragma unroll
+        for (int u = 0; u < kPreU; ++u) {
+            const int k = lane + u * L;
+            P.ok[u] = valid && k < M.ell_w;
+            const size_t q = (size_t)k * R + row;
+            P.c[u] = P.ok[u] ? (M.idx16 ? (int)M.idx16[q] : M.idx[q]) : 0;
+            P.v[u] = (P.ok[u] && !rebuilt) ? M.val[q] : V(0);
+        }
+        if constexpr (std::is_same<V, double>::value) P.r = (rebuilt && valid) ? M.rsc[row] : 0.0;
+        P.next = lane + kPreU * L;
+        P.end = valid ? M.ell_w : 0;
+    } else {
+        int b = 0, e = 0;
+        if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+        const int p = b + lane;
+#pragma unroll
+        for (int u = 0; u < kPreU; ++u) {
+            P.ok[u] = p + u * L < e;
+            P.c[u] = P.ok[u] ? M.idx[p + u * L] : 0;
+            P.v[u] = P.ok[u] ? M.val[p + u * L] : V(0);
+        }
+        P.next = p + kPreU * L;
+        P.end = e;
+    }
+}
+
+// row_dot's CSR loop from entry p on, added into s (groups of 4, then the
+// predicated last group).
+template <int L, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc csr_dot_from(const MatView<V>& M, const X* __restrict__ x, int p,
+                                            int e, Acc s) {
+    constexpr int U = 4;
+    for (; p + (U - 1) * L < e; p += U * L) {
+        int c[U];
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
+    }
+    if (p < e) {
+        bool ok[U];
+        int c[U];
+        V v[U];
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = p + u * L < e;
+            c[u] = ok[u] ? M.idx[p + u * L] : 0;
+            v[u] = ok[u] ? M.val[p + u * L] : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = ok[u] ? x[c[u]] : X(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) s += (Acc)v[u] * (Acc)xv[u];
+    }
+    return s;
+}
+
+// row_dot_pipe's CSR loop from entry p on, added into s.
+template <int L, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc csr_dot_from_pipe(const MatView<V>& M, const X* __restrict__ x,
+                                                 int p, int e, Acc s) {
+    constexpr int U = 4;
+    bool full = p + (U - 1) * L < e;
+    int c[U];
+    V v[U];
+    if (full) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
+    }
+    while (full) {
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+        const int pn = p + U * L;
+        const bool nfull = pn + (U - 1) * L < e;
+        int cn[U];
+        V vn[U];
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { cn[u] = M.idx[pn + u * L]; vn[u] = M.val[pn + u * L]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
+        p = pn;
+        full = nfull;
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = cn[u]; v[u] = vn[u]; }
+        }
+    }
+    if (p < e) {
+        bool ok[U];
+        int cc[U];
+        V vv[U];
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = p + u * L < e;
+            cc[u] = ok[u] ? M.idx[p + u * L] : 0;
+            vv[u] = ok[u] ? M.val[p + u * L] : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = ok[u] ? x[cc[u]] : X(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) s += (Acc)vv[u] * (Acc)xv[u];
+    }
+    return s;
+}
+
+// The row product from the prefetched slots: row_dot's terms in row_dot's
+// order (Pipe: the CSR rest as row_dot_pipe runs it).  Every lane of the
+// group gets the sum.
+template <int L, bool Pipe, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc row_dot_pre(const MatView<V>& M, const X* __restrict__ x, int row,
+                                           const PreSlots<V>& P) {
+    Acc s = Acc(0);
+    if (M.ell_w > 0) {
+        const size_t R = (size_t)M.rows;
+        bool rebuilt = false;
+        if constexpr (std::is_same<V, double>::value) rebuilt = M.csc != nullptr;
+        if (rebuilt) {
+            if constexpr (std::is_same<V, double>::value) {
+                double f[kPreU];
+                X xv[kPreU];
+#pragma unroll
+                for (int u = 0; u < kPreU; ++u) {
+                    f[u] = P.ok[u] ? M.csc[P.c[u]] : 0.0;
+                    xv[u] = P.ok[u] ? x[P.c[u]] : X(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kPreU; ++u)
+                    if (P.ok[u]) s += (Acc)(f[u] * P.r) * (Acc)xv[u];
+                for (int k = P.next; k < P.end; k += L) {
+                    const size_t q = (size_t)k * R + row;
+                    const int c = M.idx16 ? (int)M.idx16[q] : M.idx[q];
+                    s += (Acc)(M.csc[c] * P.r) * (Acc)x[c];
+                }
+            }
+        } else {
+            X xv[kPreU];
+#pragma unroll
+            for (int u = 0; u < kPreU; ++u) xv[u] = P.ok[u] ? x[P.c[u]] : X(0);
+#pragma unroll
+            for (int u = 0; u < kPreU; ++u)
+                if (P.ok[u]) s += (Acc)P.v[u] * (Acc)xv[u];
+            for (int k = P.next; k < P.end; k += L) {
+                const size_t q = (size_t)k * R + row;
+                const int c = M.idx16 ? (int)M.idx16[q] : M.idx[q];
+                s += (Acc)M.val[q] * (Acc)x[c];
+            }
+        }
+    } else {
+        X xv[kPreU];
+#pragma unroll
+        for (int u = 0; u < kPreU; ++u) xv[u] = P.ok[u] ? x[P.c[u]] : X(0);
+#pragma unroll
+        for (int u = 0; u < kPreU; ++u)
+            if (P.ok[u]) s += (Acc)P.v[u] * (Acc)xv[u];
+        if constexpr (Pipe) s = csr_dot_from_pipe<L, Acc>(M, x, P.next, P.end, s);
+        else s = csr_dot_from<L, Acc>(M, x, P.next, P.end, s);
+    }
+    return group_sum<L, Acc>(s);
+}
+
+// row_dot_diff from the prefetched slots (FP32 loop: A xn and A (xn - x)
+// from the interleaved pairs), in row_dot_diff's order.
+template <int L>
+__device__ __forceinline__ void row_dot_diff_pre(const MatView<float>& M,
+                                                 const float2* __restrict__ xd, int row,
+                                                 const PreSlots<float>& P, float& s1, float& sd) {
+    constexpr int U = 4;
+    s1 = 0.f; sd = 0.f;
+    float2 v[kPreU];
+#pragma unroll
+    for (int u = 0; u < kPreU; ++u) v[u] = P.ok[u] ? xd[P.c[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < kPreU; ++u)
+        if (P.ok[u]) { s1 += P.v[u] * v[u].x; sd += P.v[u] * v[u].y; }
+    if (M.ell_w > 0) {
+        const size_t R = (size_t)M.rows;
+        for (int k = P.next; k < P.end; k += L) {
+            const size_t q = (size_t)k * R + row;
+            const float a = M.val[q];
+            const float2 w = xd[M.idx16 ? (int)M.idx16[q] : M.idx[q]];
+            s1 += a * w.x;
+            sd += a * w.y;
+        }
+    } else {
+        int p = P.next;
+        const int e = P.end;
+        for (; p + (U - 1) * L < e; p += U * L) {
+            int c[U];
+            float a[U];
+            float2 w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = M.val[p + u * L]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) w[u] = xd[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { s1 += a[u] * w[u].x; sd += a[u] * w[u].y; }
+        }
+        if (p < e) {
+            bool ok[U];
+            int c[U];
+            float a[U];
+            float2 w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ok[u] = p + u * L < e;
+                c[u] = ok[u] ? M.idx[p + u * L] : 0;
+                a[u] = ok[u] ? M.val[p + u * L] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) w[u] = ok[u] ? xd[c[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u]) { s1 += a[u] * w[u].x; sd += a[u] * w[u].y; }
+        }
+    }
+    s1 = group_sum<L, float>(s1);
+    sd = group_sum<L, float>(sd);
+}
+
 // Grid-stride loop over rows, L lanes per row; body(row, lane, valid) runs
 // with block-uniform trip counts so group reductions always see full groups.
 // XCD-aware mapping: the hardware deals blocks round-robin to the 8 XCDs
@@ -649,6 +906,13 @@ __device__ __forceinline__ int xcd_block() {
 constexpr int kStageRows = 1 << 17;
 template <int L> __host__ __device__ __forceinline__ bool staged_rows(int rows) {
     return L >= 2 && rows >= kStageRows;
+}
+
+// A launch over `rows` rows with L lanes per row takes one pass when its
+// grid (grid_rows: at most kMaxBlocks blocks) holds every row: then each
+// thread has one row and lane, for_rows' single iteration.
+template <int L> __host__ __device__ __forceinline__ bool one_pass_rows(int rows) {
+    return L <= 64 && !staged_rows<L>(rows) && (long)rows <= (long)kMaxBlocks * (kBlock / L);
 }
 
 // LDS staging buffer, one per (type, width) whatever the lanes per row, so

@@ -579,6 +579,15 @@ class Solver {
 
     // device preprocessing (the unscaled-form constructor)
     bool device_prep_ = false;
+    // one-pass iteration kernels where the launch covers every row in one
+    // pass (SUPR_ONE_PASS=off: the grid-stride kernels everywhere; same bits)
+    bool one_pass_ = true;
+    // ... and where the prefetched group holds a mean row (kPreU slots per
+    // lane): on long rows the prefetch only adds registers (the 1M-variable
+    // LP's A, 1,000 per row at 32 lanes: 61.4 against 59.6 ms per solve)
+    bool one_pass(const DevMatrix& M) const {
+        return one_pass_ && (long)M.nnz <= (long)kPreU * M.lanes * M.rows;
+    }
     int passes_ = 10;
     std::vector<char> integer_;
     std::unique_ptr<DevLp> dlp_;
@@ -1050,6 +1059,7 @@ void Solver::setup_device_resident(bool mixed) {
     hipStream_t s = stream_.s;
     const int n = n_, m = m_;
     const auto t0 = Clock::now();
+    if (const char* e = std::getenv("SUPR_ONE_PASS"); e && std::strcmp(e, "off") == 0) one_pass_ = false;
     setup_matrices(mixed, true, true);
     const double t_mat = seconds_since(t0);
     initial_point();
@@ -1197,6 +1207,26 @@ template <typename T> MatView<T> iter_view(const DevMatrix& M);
 template <> MatView<float> iter_view<float>(const DevMatrix& M) { return M.f32(); }
 template <> MatView<double> iter_view<double>(const DevMatrix& M) { return M.f64_iter(); }
 
+// The iteration kernels, in their one-pass form (matrix slots and epilogue
+// operands loaded before the step decision) when pre is set and the launch
+// covers every row in one pass.
+template <int L, typename T>
+void launch_primal(bool pre, int grid, hipStream_t s, DevState* S, int p, const MatView<T>& AT,
+                   const IterBufs<T>& B, const double* part_m, int nb_m, const double* dx2, double* part_n) {
+    if (pre && one_pass_rows<L>(AT.rows))
+        hipLaunchKernelGGL((k_primal<L, T, true>), dim3(grid), dim3(kBlock), 0, s, S, p, AT, B, part_m, nb_m, dx2, part_n);
+    else
+        hipLaunchKernelGGL((k_primal<L, T, false>), dim3(grid), dim3(kBlock), 0, s, S, p, AT, B, part_m, nb_m, dx2, part_n);
+}
+template <int L, typename T>
+void launch_dual(bool pre, int grid, hipStream_t s, DevState* S, int p1, const MatView<T>& A,
+                 const IterBufs<T>& B, const double* part_n, int nb_n, double* dx2, double* part_m) {
+    if (pre && one_pass_rows<L>(A.rows))
+        hipLaunchKernelGGL((k_dual<L, T, true>), dim3(grid), dim3(kBlock), 0, s, S, p1, A, B, part_n, nb_n, dx2, part_m);
+    else
+        hipLaunchKernelGGL((k_dual<L, T, false>), dim3(grid), dim3(kBlock), 0, s, S, p1, A, B, part_n, nb_n, dx2, part_m);
+}
+
 template <typename T>
 void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     hipStream_t s = stream_.s;
@@ -1204,12 +1234,11 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
     const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
     const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
+    const bool pn = one_pass(AT_), pm = one_pass(A_);
     for (int k = 0; k < count; ++k) {
         const int p = k & 1;
-        SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, S, p, AT, B,
-                                                 part_m_, gm, dx2_slot_, part_n_));
-        SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, S, p ^ 1, A, B,
-                                                part_n_, gn, dx2_slot_, part_m_));
+        SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(pn, gn, s, S, p, AT, B, part_m_, gm, dx2_slot_, part_n_)));
+        SUPR_LANES(A_.lanes, (launch_dual<LL, T>(pm, gm, s, S, p ^ 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
     }
     SUPR_HIP(hipGetLastError());
 }
@@ -1494,12 +1523,10 @@ void Solver::bench_kernels(int reps, double* out) {
         const DevState both[2] = {st, st};
         SUPR_HIP(hipMemcpyAsync(S, both, sizeof(both), hipMemcpyHostToDevice, s));
         out[slot] = time_kernel([&] {
-            SUPR_LANES(AT_.lanes, hipLaunchKernelGGL((k_primal<LL, T>), dim3(gn), dim3(kBlock), 0, s, S, 0, AT, B,
-                                                     part_m_, gm, dx2_slot_, part_n_));
+            SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(one_pass(AT_), gn, s, S, 0, AT, B, part_m_, gm, dx2_slot_, part_n_)));
         });
         out[slot + 1] = time_kernel([&] {
-            SUPR_LANES(A_.lanes, hipLaunchKernelGGL((k_dual<LL, T>), dim3(gm), dim3(kBlock), 0, s, S, 1, A, B,
-                                                    part_n_, gn, dx2_slot_, part_m_));
+            SUPR_LANES(A_.lanes, (launch_dual<LL, T>(one_pass(A_), gm, s, S, 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
         });
         SUPR_HIP(hipGetLastError());
     };

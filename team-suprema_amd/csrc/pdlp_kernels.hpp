@@ -120,13 +120,33 @@ __global__ __launch_bounds__(kBlock) void k_copy4(const float4* __restrict__ src
 //   on every accepted one), [after an accept] xsum += eta_acc x (pending
 //   average term), xn = clip(x - tau (c - aty), l, u); partial sums of
 //   (xn - x)^2 in FP64.
-template <int L, typename T>
+// Pre: a one-pass launch (one_pass_rows); the lane's first matrix slots and
+// its row's epilogue operands (both x buffers, c, xsum, l, u) are loaded
+// before the decision prologue.  The same arithmetic, so the same bits.
+template <int L, typename T, bool Pre>
 __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int p,
                                                    MatView<T> AT, IterBufs<T> B,
                                                    const double* __restrict__ part_m, int nb_m,
                                                    const double* __restrict__ dx2_slot,
                                                    double* __restrict__ part_n) {
     __shared__ DevState Q;
+    int prow = 0, plane = 0;
+    bool pvalid = false;
+    PreSlots<T> P;
+    T px[2] = {T(0), T(0)}, pc = T(0), ps = T(0), pl = T(0), pu = T(0);
+    if constexpr (Pre) {
+        prow = xcd_block() * (kBlock / L) + (int)threadIdx.x / L;
+        plane = (int)threadIdx.x % L;
+        pvalid = prow < AT.rows;
+        pre_slots<L>(AT, prow, plane, pvalid, P);
+        if (pvalid && plane == 0) {
+            px[0] = B.x[0][prow];
+            px[1] = B.x[1][prow];
+            pc = B.c[prow];
+            ps = B.xsum[prow];
+            if (B.l) { pl = B.l[prow]; pu = B.u[prow]; }
+        }
+    }
     // the publisher is the last block: its slot has the fewest rows, so the
     // step powers it forms before its rows stay off the launch's critical
     // path.  (Formed after the rows instead, the publisher flag and S stay
@@ -160,7 +180,21 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
         const double d = (double)v - (double)xj;
         acc[0] += d * d;
     };
-    if (staged_rows<L>(AT.rows)) {
+    if constexpr (Pre) {
+        const T a = row_dot_pre<L, false, T>(AT, y, prow, P);
+        if (pvalid && plane == 0) {
+            // the epilogue above with the operands loaded before the decision
+            const T xj = px[cur];
+            if (pending) B.xsum[prow] = ps + eta_acc * xj;
+            const T z = xj - tau * (pc - a);
+            const T v = B.l ? smin(smax(z, pl), pu) : smax(z, T(0));
+            xn[prow] = v;
+            if constexpr (sizeof(T) == 4)
+                reinterpret_cast<float2*>(B.xd)[prow] = make_float2(v, v - xj);
+            const double d = (double)v - (double)xj;
+            acc[0] += d * d;
+        }
+    } else if (staged_rows<L>(AT.rows)) {
         for_row_tiles<L, 1, T>(
             AT.rows, xcd_block(), gridDim.x,
             [&](int row, int lane, bool valid, T (&v)[1]) { v[0] = row_dot<L, T>(AT, y, row, lane, valid); },
@@ -179,12 +213,38 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
 //   axn = A xn, ad = A (xn - x);  [pending] ysum += eta_acc y;
 //   yn = proj(y + sigma (b - (axn + ad)));  partials of (yn-y)^2 and
 //   (yn-y)'ad.  The last block also sums k_primal's partials into *dx2_slot.
-template <int L, typename T>
+// Pre: a one-pass launch; the lane's first matrix slots and its row's
+// epilogue operands (both y and A x buffers, b, ysum, the row kind) are
+// loaded before the state.  The same arithmetic, so the same bits.
+template <int L, typename T, bool Pre>
 __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S, int p1,
                                                  MatView<T> A, IterBufs<T> B,
                                                  const double* __restrict__ part_n, int nb_n,
                                                  double* __restrict__ dx2_slot,
                                                  double* __restrict__ part_m) {
+    constexpr bool kDiff = sizeof(T) == 4;
+    int prow = 0, plane = 0;
+    bool pvalid = false;
+    PreSlots<T> P;
+    T py[2] = {T(0), T(0)}, pax[2] = {T(0), T(0)}, pb = T(0), ps = T(0);
+    uint8_t pineq = 0;
+    if constexpr (Pre) {
+        prow = xcd_block() * (kBlock / L) + (int)threadIdx.x / L;
+        plane = (int)threadIdx.x % L;
+        pvalid = prow < A.rows;
+        pre_slots<L>(A, prow, plane, pvalid, P);
+        if (pvalid && plane == 0) {
+            py[0] = B.y[0][prow];
+            py[1] = B.y[1][prow];
+            if constexpr (!kDiff) {
+                pax[0] = B.ax[0][prow];
+                pax[1] = B.ax[1][prow];
+            }
+            pb = B.b[prow];
+            ps = B.ysum[prow];
+            pineq = B.ineq[prow];
+        }
+    }
     const DevState* st = S + p1;
     if (!st->active) return;
     const int cur = st->cur;
@@ -199,7 +259,6 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
     double acc[2] = {0.0, 0.0};
     // FP32: A(xn - x) from the difference (see row_dot_diff); FP64: the
     // reference's 2 A xn - A x (pdlp_gpu_solve.cu:89) with A x carried.
-    constexpr bool kDiff = sizeof(T) == 4;
     auto dot = [&](int row, int lane, bool valid, T (&v)[2]) {
         if constexpr (kDiff) {
             row_dot_diff<L>(A, reinterpret_cast<const float2*>(B.xd), row, lane, valid, v[0], v[1]);
@@ -228,7 +287,33 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
         acc[0] += d * d;
         acc[1] += d * (double)ad;
     };
-    if (staged_rows<L>(A.rows)) {
+    if constexpr (Pre) {
+        T a, ad = T(0);
+        if constexpr (kDiff) {
+            row_dot_diff_pre<L>(A, reinterpret_cast<const float2*>(B.xd), prow, P, a, ad);
+        } else {
+            a = row_dot_pre<L, true, T>(A, xn, prow, P);
+        }
+        if (pvalid && plane == 0) {
+            // the epilogue above with the operands loaded before the state
+            axn[prow] = a;
+            const T yi = py[cur];
+            if (pending) B.ysum[prow] = ps + eta_acc * yi;
+            T v;
+            if constexpr (kDiff) {
+                v = yi + sigma * (pb - (a + ad));
+            } else {
+                const T axr = pax[cur];
+                ad = a - axr;
+                v = yi + sigma * (pb - (T(2) * a - axr));
+            }
+            if (pineq && v < T(0)) v = T(0);
+            yn[prow] = v;
+            const double d = (double)v - (double)yi;
+            acc[0] += d * d;
+            acc[1] += d * (double)ad;
+        }
+    } else if (staged_rows<L>(A.rows)) {
         for_row_tiles<L, 2, T>(A.rows, xcd_block(), gridDim.x, dot,
                                [&](int row, const T (&v)[2]) { epilogue(row, v[0], v[1]); });
     } else {

@@ -581,6 +581,37 @@ def test_rebuilt_ell_values_give_the_same_bits(monkeypatch, mode):
     assert kb["ell_AT"] > 0 and kb["rebuilt_AT_fp64"] == 0.0
 
 
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_one_pass_kernels_give_the_same_bits(monkeypatch, golden, mode):
+    """Launches that cover every row in one pass use the one-pass iteration
+    kernels (matrix slots and epilogue operands loaded before the step
+    decision); SUPR_ONE_PASS=off runs the grid-stride kernels everywhere.
+    Same terms in the same order, so the same solve bit for bit: ELL A^T with
+    several lanes per row (random LP), ELL A^T with values formed from factors
+    (transportation), CSR on both sides with rows longer than one prefetched
+    group (Netlib, power-law), bounded columns and inequality rows, and lanes
+    pinned from 1 to 64."""
+    cases = [("random", P.gen_random_lp(400, 1000, 6, seed=13), {}),
+             ("transport", P.gen_transport(60, 80, seed=4), {}),
+             ("power_law", P.gen_power_law_lp(1000, 4000, seed=3), {}),
+             ("power_law_l1", P.gen_power_law_lp(1000, 4000, seed=3), dict(spmv_lanes_A=1, spmv_lanes_AT=2)),
+             ("random_l64", P.gen_random_lp(400, 1000, 6, seed=13), dict(spmv_lanes_A=64, spmv_lanes_AT=4))]
+    for name in ("afiro", "blend", "sc50a", "tiny_range", "israel"):
+        cases.append((name, golden_lp(golden[name]), {}))
+    for name, lp, pin in cases:
+        s, Dr, Dc = P.scale(lp)
+        on = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, max_iterations=400000, **pin)
+        monkeypatch.setenv("SUPR_ONE_PASS", "off")
+        off = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, max_iterations=400000, **pin)
+        monkeypatch.delenv("SUPR_ONE_PASS")
+        assert on.converged, name
+        assert on.iterations == off.iterations and on.restarts == off.restarts, name
+        assert np.array_equal(on.x, off.x) and np.array_equal(on.y, off.y), name
+        assert on.info["objective"] == off.info["objective"], name
+        print(f"[parity] one-pass {name} mode {mode}: {on.iterations} attempts, lanes "
+              f"{on.info['lanes_A']}/{on.info['lanes_AT']}, same bits", file=sys.stderr, flush=True)
+
+
 def test_device_memory_cache_reuse_and_release():
     """Device memory a call releases is kept and reused by the next call (no
     hipMalloc / hipFree between one-shot calls); results do not depend on
