@@ -208,6 +208,18 @@ def measured_traffic(kernel):
         return None
 
 
+def copy_rates():
+    """GB/s (bytes read + written) of the plain stream copy k_copy4 at
+    1 GiB each way (HBM) and at 64 MB and 16 MB each way (held in the 256 MB
+    last-level cache between launches): the box's attainable streaming
+    rates, for context beside the 8 TB/s peak."""
+    out = {}
+    for label, nb, reps in (("1GiB", 1 << 30, 10), ("64MiB", 64 << 20, 100), ("16MiB", 16 << 20, 200)):
+        ms = P.bench_copy(nb, reps=reps)
+        out[label] = 2 * nb / (ms * 1e-3) / 1e9
+    return out
+
+
 def _pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=float), q))
 
@@ -822,10 +834,12 @@ def main():
         return
     if args.pmc_summary:
         lanes = [32, 1]
+        # the 1M-variable LP's launches take the grid-stride kernels (template
+        # flag false: A^T's rows exceed one pass, A's rows are long)
         print(json.dumps(pmc_summary(*args.pmc_summary, {
-            "k_primal_f32": f"k_primal<{lanes[1]}, float>", "k_dual_f32": f"k_dual<{lanes[0]}, float>",
-            "k_primal_f64": f"k_primal<{lanes[1]}, double>", "k_dual_f64": f"k_dual<{lanes[0]}, double>"}),
-            indent=1))
+            "k_primal_f32": f"k_primal<{lanes[1]}, float, false>", "k_dual_f32": f"k_dual<{lanes[0]}, float, false>",
+            "k_primal_f64": f"k_primal<{lanes[1]}, double, false>",
+            "k_dual_f64": f"k_dual<{lanes[0]}, double, false>"}), indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -867,6 +881,20 @@ def main():
         print(json.dumps({"kernels_ms": kb}))
         return
 
+    if args.solve_only:
+        mode = P.MODE_FAST_FP64 if args.solve_only == "fast_fp64" else P.MODE_MIXED
+        # the default options: on this LP (2M nonzeros) the loop's kernels are
+        # launched on the stream directly, 16 attempts and the check group per
+        # host poll, which a kernel trace can follow
+        with P.Engine(lp, scaled, Dr, Dc, tol=TOL, mode=mode) as eng:
+            for k in range(args.warmup + args.steps):
+                t0 = time.perf_counter()
+                r = eng.solve()
+                log(f"{args.solve_only} solve {k}: {(time.perf_counter() - t0) * 1e3:.2f} ms, "
+                    f"{r.iterations} attempts")
+        print(json.dumps({"solve_only": args.solve_only, "attempts": r.iterations}))
+        return
+
     # the process's first device use (runtime start, code-object loading) is
     # paid here, so that setup_seconds below is the engine's own setup
     warm = P.gen_random_lp(50, 120, 4, seed=1)
@@ -875,16 +903,6 @@ def main():
 
     if args.only == "one-shot":
         print(json.dumps({"one_shot": bench_one_shot(lp, scaled, Dr, Dc)}))
-        return
-    if args.solve_only:
-        mode = P.MODE_FAST_FP64 if args.solve_only == "fast_fp64" else P.MODE_MIXED
-        with P.Engine(lp, scaled, Dr, Dc, tol=TOL, mode=mode, graph_attempts=-16) as eng:
-            for k in range(args.warmup + args.steps):
-                t0 = time.perf_counter()
-                r = eng.solve()
-                log(f"{args.solve_only} direct-launch solve {k}: {(time.perf_counter() - t0) * 1e3:.2f} ms, "
-                    f"{r.iterations} attempts")
-        print(json.dumps({"solve_only": args.solve_only, "attempts": r.iterations}))
         return
 
     results = {}
@@ -944,10 +962,10 @@ def main():
         "k_primal_f32": dict(ms=kb["primal_ms_fp32"], bytes=u32[0], format_bytes=prim32),
         "k_dual_f32": dict(ms=kb["dual_ms_fp32"], bytes=u32[1], format_bytes=dual32),
         "k_primal_f64": dict(ms=kb["primal_ms_fp64"], bytes=u64[0], format_bytes=prim64,
-                             rocprof=f"k_primal<{lat}, double>",
+                             rocprof=f"k_primal<{lat}, double, false>",
                              rebuilt_values=bool(kb.get("rebuilt_AT_fp64", 0) > 0)),
         "k_dual_f64": dict(ms=kb["dual_ms_fp64"], bytes=u64[1], format_bytes=dual64,
-                           rocprof=f"k_dual<{la}, double>"),
+                           rocprof=f"k_dual<{la}, double, false>"),
     }
     for k in kernels.values():
         k["GBps"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
@@ -956,6 +974,8 @@ def main():
     dom_name = max(("k_primal_f64", "k_dual_f64"), key=lambda k: kernels[k]["ms"])
     dom = kernels[dom_name]
     rp_us = rocprof_average_us(dom["rocprof"])
+    box_copy = copy_rates()
+    log("plain copy: " + ", ".join(f"{k} each way {v:.0f} GB/s" for k, v in box_copy.items()))
 
     cpu = None
     if not args.no_cpu:
@@ -1016,12 +1036,15 @@ def main():
             "rocprof_avg_us": rp_us,
             "rocprof_frac": (dom["bytes"] / (rp_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if rp_us else None,
             "rocprof_source": "profiles/iteration_kernels_transport_1000x1000.md",
+            "copy_GBps": box_copy,
             "format_bytes_per_launch": dom["format_bytes"],
             "format_frac": dom["format_frac"],
             "note": "achieved: 4-byte indices and 8-byte values per nonzero plus the kernel's vectors "
                     "once; format_*: the bytes this layout moves (16-bit A^T indices, A^T's FP64 "
                     "values formed from Dr_i a and Dc_j when every unscaled entry shares one value); "
-                    "traffic: measured HBM bytes",
+                    "traffic: measured HBM bytes; copy_GBps: the plain stream copy on this box at 1 GiB "
+                    "each way (HBM) and at 64 / 16 MB each way (resident in the 256 MB last-level cache, "
+                    "as the iteration kernels' working set is)",
         },
         "kernels": kernels,
         "cpu_baseline": cpu,

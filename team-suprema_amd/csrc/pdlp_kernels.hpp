@@ -635,6 +635,12 @@ __global__ __launch_bounds__(kBlock) void k_reset(int n, int m, const double* __
     }
 }
 
+// out[0, len) = v (setup: vectors whose every entry is one value)
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t len, double v, double* __restrict__ out) {
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < len; q += (int64_t)gridDim.x * kBlock)
+        out[q] = v;
+}
+
 // dst = (D) src, elementwise (FP64 -> FP32 rounds to nearest even).
 template <typename S, typename D>
 __global__ __launch_bounds__(kBlock) void k_convert(int64_t len, const S* __restrict__ src,
