@@ -73,6 +73,7 @@ struct DevState {
     int32_t switch_req, phase;   // switch_req: reason code (1 merit, 2 stall near floor, 3 long stall); phase 0: FP32 iterates, 1: FP64
     int32_t attempts_fp32;   // attempts when the FP32 loop handed over (mixed mode)
     int32_t pw_k1;           // the k1 that pw3 / pw6 belong to (0: none)
+    int32_t groups;          // check groups run (k_restart_fin), for the host's launch plan
 };
 
 template <typename V> struct MatView {

@@ -1339,6 +1339,60 @@ hipGraphExec_t Solver::capture_graph(const Iter<T>& I, int attempts, int units) 
     return exec;
 }
 
+// Launch plan of the direct-launch loop (auto mode, large LPs).  A KKT
+// check becomes due when `accepted` reaches the next multiple of check_every
+// (or `attempts` reaches max_iterations); until then a check group is a
+// no-op, and once it is due the attempts are no-ops until a check group
+// runs.  Instead of a check group after every 16 attempts, the host places
+// one where the check is expected to become due (the attempts the accepted
+// steps need at the rejection rate seen so far), then one more after a
+// short unit in case more attempts were rejected, and launches the attempts
+// between them in units without checks.  It plans from the state the last
+// check group left in the host-coherent copy (which group it was is the
+// state's `groups` count), so it never waits for the device.  The device
+// follows the same state machine whatever the plan: only the number of
+// no-op launches changes, never the bits.
+struct CheckPlan {
+    int check_every = 64, max_iter = 0;
+    int64_t launched = 0;                 // attempts launched in this loop
+    std::vector<int64_t> group_at;        // `launched` after each check group of this loop
+    int32_t base_groups = 0;              // state.groups when the loop started
+    bool slack_due = false;               // the last unit ended at an expected check
+    static constexpr int kMaxUnit = 32;   // attempts per unit without a check group
+    static constexpr int kSlack = 4;      // attempts in the unit after an expected check
+
+    // The next unit from the last known state: attempts (even) and whether
+    // a check group follows them.
+    std::pair<int, bool> next(const DevState& hs) {
+        if (slack_due) {
+            slack_due = false;
+            return {kSlack, true};
+        }
+        const int64_t jp = std::max<int64_t>(0, hs.groups - base_groups);
+        const int64_t at = jp == 0 ? 0 : group_at[(size_t)std::min<int64_t>(jp, (int64_t)group_at.size()) - 1];
+        const double rej = hs.attempts > 0 ? std::min(0.5, (double)(hs.attempts - hs.accepted) / hs.attempts) : 0.0;
+        const int64_t ce = check_every;
+        const int64_t due_acc = hs.need_check ? hs.accepted : ((int64_t)hs.accepted / ce + 1) * ce;
+        int64_t target = at + (int64_t)std::ceil((double)(due_acc - hs.accepted) / (1.0 - rej));
+        target = std::min<int64_t>(target, at + std::max<int64_t>(0, (int64_t)max_iter - hs.attempts));
+        // check groups already launched at or after the expected point are
+        // taken to have run that check; expect the next one a check interval on
+        const int64_t interval = (int64_t)std::ceil((double)ce / (1.0 - rej));
+        for (size_t j = (size_t)jp; j < group_at.size(); ++j)
+            if (group_at[j] >= target) target = group_at[j] + interval;
+        int64_t k = target - launched;
+        if (k <= 0) return {2, true};
+        if (k > kMaxUnit) return {kMaxUnit, false};
+        k += k & 1;
+        slack_due = true;
+        return {(int)k, true};
+    }
+    void launched_unit(int k, bool check) {
+        launched += k;
+        if (check) group_at.push_back(launched);
+    }
+};
+
 // Replays the graph until the device reports done / a precision switch, or
 // the stop callback fires.  A few replays stay in flight; after the oldest
 // one's event the host reads the host-coherent state copy the replays'
@@ -1359,6 +1413,8 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
     //     smaller ones replay graphs of 8 units of 16 attempts (the random
     //     2,000 x 5,000 LP: 10.3 against 10.8 us per attempt at one unit, and
     //     12.1 launched directly, where the host cannot keep ahead).
+    //   0 (auto), 2^20+ nonzeros: launched directly, planned by CheckPlan
+    //     (units of up to 32 attempts, check groups where checks are expected).
     const bool autodirect = o_.graph_attempts == 0 && A_.nnz >= (1L << 20);
     const bool direct = o_.graph_attempts < 0 || autodirect;
     int G = o_.graph_attempts != 0 ? std::abs(o_.graph_attempts) : 16;
@@ -1366,12 +1422,24 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
     const int K = o_.graph_attempts == 0 && !direct ? 8 : 1;
     if (!direct && !graph_[prec]) graph_[prec] = capture_graph(I, G, K);
     hipGraphExec_t exec = graph_[prec];
-    constexpr int R = 2;   // replays in flight: a replay takes far longer than the host's poll
-    hipEvent_t ev[R] = {};
+    CheckPlan plan;
+    plan.check_every = o_.check_every;
+    plan.max_iter = o_.max_iterations;
+    plan.base_groups = hs.groups;   // hs: the state the loop starts from
+    // The host-coherent copy holds the last check group's state, which may be
+    // the previous loop's (done, or a precision switch); polls before this
+    // loop's first check group must see this loop's start instead.  Nothing
+    // on the device writes it now: the previous loop ended with a stream sync.
+    std::memcpy((void*)mirror_, &hs, sizeof(DevState));
+    std::atomic_thread_fence(std::memory_order_release);
+    // units in flight: a replay takes far longer than the host's poll; the
+    // planned units can be short, so one more is kept queued
+    const int R = autodirect ? 3 : 2;
+    hipEvent_t ev[3] = {};
     int replays = 0;
     try {
-        for (auto& e : ev) SUPR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        int last_attempts = -1, last_checks = -1, idle = 0;
+        for (int r = 0; r < R; ++r) SUPR_HIP(hipEventCreateWithFlags(&ev[r], hipEventDisableTiming));
+        int last_attempts = -1, last_checks = -1, last_groups = hs.groups, idle = 0;
         auto last_log = Clock::now();
         while (true) {
             if (replays >= R) {
@@ -1381,12 +1449,15 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
                 std::memcpy(&hs, (const void*)mirror_, sizeof(DevState));
                 if (hs.done || hs.switch_req) break;
                 if (stop && stop(ctx)) { stopped = true; break; }
-                // A replay always advances attempts or checks; a loop that
-                // stops moving is a logic error -- fail instead of spinning.
-                if (hs.attempts == last_attempts && hs.checks == last_checks) {
-                    if (++idle > 64) throw std::runtime_error("device loop made no progress");
-                } else {
-                    idle = 0;
+                // A check group always advances attempts or checks; a loop
+                // that stops moving is a logic error -- fail instead of spinning.
+                if (hs.groups != last_groups) {
+                    if (hs.attempts == last_attempts && hs.checks == last_checks) {
+                        if (++idle > 64) throw std::runtime_error("device loop made no progress");
+                    } else {
+                        idle = 0;
+                    }
+                    last_groups = hs.groups;
                 }
                 last_attempts = hs.attempts;
                 last_checks = hs.checks;
@@ -1398,7 +1469,12 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
                 }
             }
             const int slot = replays % R;
-            if (direct) {
+            if (autodirect) {
+                const auto [k, check] = plan.next(hs);
+                enqueue_attempts(I, k);
+                if (check) enqueue_check(I);
+                plan.launched_unit(k, check);
+            } else if (direct) {
                 for (int u = 0; u < K; ++u) {
                     enqueue_attempts(I, G);
                     enqueue_check(I);
@@ -1416,7 +1492,7 @@ int Solver::run_graph_loop(const Iter<T>& I, int prec, supr_stop_fn stop, void* 
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         throw;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     return replays;
 }
 
@@ -1435,7 +1511,7 @@ void Solver::run_device_resident(bool mixed, supr_stop_fn stop, void* ctx, doubl
     else set_products(D_, A_.f64());
     const auto t1 = Clock::now();
 
-    DevState hs;
+    DevState hs = st0;   // the state the loop starts from (its launch plan reads it)
     bool stopped = false;
     int fp32_attempts = 0;
     if (mixed) {

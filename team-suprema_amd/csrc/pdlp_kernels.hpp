@@ -546,6 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_restart_fin(DevState* __restrict__ S
         if (threadIdx.x == 0) restart_fin_update(st, sn[0], sm[0]);
     }
     if (threadIdx.x == 0) {
+        st->groups += 1;
         S[0] = *st;
         if (mirror) *mirror = *st;   // host-coherent copy for the host's poll
     }

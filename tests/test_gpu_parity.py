@@ -374,6 +374,39 @@ def test_direct_launches_match_graph_replay(mode):
 
 
 @pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_planned_direct_launches_match_graph_replay(mode):
+    """LPs of 2^20+ nonzeros launch the loop directly with check groups placed
+    where checks are expected (the default) instead of after every 16
+    attempts: the device runs the same state machine, so the solve is the
+    graph replay's bit for bit -- to convergence (in the mixed mode across the
+    FP32 -> FP64 hand-over), stopped by the iteration limit (a check group
+    placed at the limit), and on a resident engine's repeated solves."""
+    cases = [(P.gen_transport(1000, 600, seed=12), {}),
+             (P.gen_power_law_lp(30000, 120000, seed=4), dict(max_iterations=6000)),
+             (P.gen_transport(1000, 600, seed=12), dict(max_iterations=777))]
+    for lp, kw in cases:
+        assert lp.nnz >= 1 << 20
+        s, Dr, Dc = P.scale(lp)
+        kw = dict(dict(tol=TOL, mode=mode, max_iterations=60000), **kw)
+        planned = P.solve_pdhg_gpu(lp, s, Dr, Dc, **kw)
+        replayed = P.solve_pdhg_gpu(lp, s, Dr, Dc, graph_attempts=16, **kw)
+        assert planned.iterations == replayed.iterations and planned.restarts == replayed.restarts
+        assert planned.converged == replayed.converged
+        assert np.array_equal(planned.x, replayed.x) and np.array_equal(planned.y, replayed.y)
+        assert planned.info["objective"] == replayed.info["objective"]
+        assert planned.info["checks"] == replayed.info["checks"]
+        # a resident engine's later solves start from a state copy that the
+        # previous solve left finished: they must run whole solves all the same
+        with P.Engine(lp, s, Dr, Dc, **kw) as eng:
+            for _ in range(2):
+                again = eng.solve()
+                assert again.iterations == planned.iterations
+                assert np.array_equal(again.x, planned.x) and np.array_equal(again.y, planned.y)
+        print(f"[parity] planned launches m={lp.m} n={lp.n} mode {mode}: {planned.iterations} attempts, "
+              f"converged {planned.converged}, same bits as graph replays", file=sys.stderr, flush=True)
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
 def test_device_layout_matches_host_layout(monkeypatch, mode):
     """Caller-scaled forms of 2^16 or more nonzeros are laid out on the device
     (a stable counting sort, as the host's build_csr): with the lanes pinned,
