@@ -137,7 +137,8 @@ typedef struct supr_pdlp_info {
     int32_t converged;
     int32_t has_point;         /* 0: no finite-merit point was seen; x, y untouched */
     double eps_P, eps_D, eps_G;
-    double objective;          /* c'x + obj_offset of the returned point (unscaled form) */
+    double objective;          /* c'x + obj_offset of the returned point (unscaled form), as the
+                                  device's KKT check summed it (FP64, fixed order) */
     int32_t iterations_fp32;   /* MIXED: attempts spent in the FP32 loop */
     int32_t checks;            /* KKT checks performed */
     int32_t lanes_A, lanes_AT; /* SpMV schedules used */
@@ -162,8 +163,8 @@ typedef int (*supr_stop_fn)(void* ctx);
  * supr_engine_create).  An LP with 2^16 or more columns whose largest column
  * degree is at least 4x the mean is solved with its columns relabelled
  * (sorted by degree, in groups of 16 dealt over 256-column tiles: the dual
- * half's gathers of x then share cache lines); x_out and the objective come
- * back in the caller's column order.  SUPR_COLUMN_ORDER=given turns this off
+ * half's gathers of x then share cache lines); x_out comes back in the
+ * caller's column order.  SUPR_COLUMN_ORDER=given turns this off
  * (also for supr_engine_create and supr_bench_kernels); SUPR_MODE_PLAIN_FP64,
  * the reference engine's design, always solves the columns as given. */
 int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
