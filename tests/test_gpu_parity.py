@@ -320,6 +320,28 @@ def test_stop_flag_cancels(mode):
 
 
 @pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_stop_flag_cancels_planned_launches(oracle, mode):
+    """The stop callback on an LP of 2^20+ nonzeros (direct launches with
+    planned check groups): polled once per launch unit, it ends the solve
+    early with the best point a check kept, whose eps the oracle's verifier
+    reproduces."""
+    lp = P.gen_transport(1000, 600, seed=12)
+    scaled, Dr, Dc = P.scale(lp)
+    polls = {"n": 0}
+
+    def stop():
+        polls["n"] += 1
+        return polls["n"] > 40
+
+    res = P.solve_pdhg_gpu(lp, scaled, Dr, Dc, tol=1e-12, stop_flag=stop, mode=mode)
+    assert not res.converged and res.iterations < 200000
+    assert polls["n"] == 41
+    assert res.info["has_point"] == 1 and res.info["checks"] >= 1
+    rep = oracle.verify(lp, res.x, res.y)
+    np.testing.assert_allclose([res.eps_P, res.eps_D, res.eps_G], rep[:3], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
 def test_default_runs_are_bit_reproducible(oracle, mode):
     """With default options (lanes per row from the matrix's shape, no
     timing) every setup of an LP sums in the same order: the one-shot entry
