@@ -410,8 +410,8 @@ struct Pool {
         part_m = p; p += 2 * need_m;
         part_km = p; p += 6 * need_m;
         part_rm = p;
-        dtabn.alloc<int32_t>(need_n);
-        dtabm.alloc<int32_t>(need_m);
+        dtabn.alloc<BatchInst>(need_n);
+        dtabm.alloc<BatchInst>(need_m);
     }
 
     // Lanes per row, staged or not, and block counts of instance k: functions
@@ -514,31 +514,32 @@ struct Pool {
 
     // Lays the instances `live` out over the launches; returns the geometry.
     BatchGeo layout(const std::vector<int32_t>& live) {
-        std::vector<int32_t> tn, tm;
+        std::vector<BatchInst> tn, tm;   // per block: a copy of its instance's description
         int32_t on = 0, om = 0;
         for (int32_t k : live) {
             BatchInst& I = inst[k];
             I.on = on;
             I.om = om;
-            tn.insert(tn.end(), I.bn, k);
-            tm.insert(tm.end(), I.bm, k);
+            I.id = k;
+            tn.insert(tn.end(), I.bn, I);
+            tm.insert(tm.end(), I.bm, I);
             on += I.bn;
             om += I.bm;
         }
         hipStream_t st = stream;
         BATCH_HIP(hipMemcpyAsync(dinst.get<BatchInst>(), inst.data(), sizeof(BatchInst) * Scap,
                                  hipMemcpyHostToDevice, st));
-        BATCH_HIP(hipMemcpyAsync(dtabn.get<int32_t>(), tn.data(), sizeof(int32_t) * tn.size(),
+        BATCH_HIP(hipMemcpyAsync(dtabn.get<BatchInst>(), tn.data(), sizeof(BatchInst) * tn.size(),
                                  hipMemcpyHostToDevice, st));
-        BATCH_HIP(hipMemcpyAsync(dtabm.get<int32_t>(), tm.data(), sizeof(int32_t) * tm.size(),
+        BATCH_HIP(hipMemcpyAsync(dtabm.get<BatchInst>(), tm.data(), sizeof(BatchInst) * tm.size(),
                                  hipMemcpyHostToDevice, st));
         BATCH_HIP(hipMemcpyAsync(dids.get<int32_t>(), live.data(), sizeof(int32_t) * live.size(),
                                  hipMemcpyHostToDevice, st));
         BATCH_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
         BatchGeo g;
         g.inst = dinst.get<BatchInst>();
-        g.tab_n = dtabn.get<int32_t>();
-        g.tab_m = dtabm.get<int32_t>();
+        g.blk_n = dtabn.get<BatchInst>();
+        g.blk_m = dtabm.get<BatchInst>();
         g.ids = dids.get<int32_t>();
         g.BN = on;
         g.BM = om;

@@ -30,7 +30,7 @@ struct BatchInst {
     int32_t on, om;      // its first block in each (also its first partial slot)
     int32_t sn, sm;      // column / row side uses the LDS-staged tiles (for_row_tiles)
     int32_t nonneg;      // every l_j = +0 and u_j = +inf: the primal projects onto x >= 0
-    int32_t pad0;
+    int32_t id;          // the instance's slot (its state pair, its results)
     int64_t ro, co;      // offsets into the concatenated row / column vectors
     int64_t pa, pat;     // offsets of its row-pointer arrays (m + 1 and n + 1 entries)
     int64_t za;          // offset into the nonzero arrays (A and A^T hold nnz each)
@@ -39,8 +39,11 @@ struct BatchInst {
 // Shared description of one solve's launches.
 struct BatchGeo {
     const BatchInst* inst;
-    const int32_t* tab_n;    // column-side block -> instance
-    const int32_t* tab_m;    // row-side block -> instance
+    // block -> a copy of its instance's BatchInst (column side, row side): a
+    // block reads its instance's description in one load instead of an
+    // index and then the description
+    const BatchInst* blk_n;
+    const BatchInst* blk_m;
     const int32_t* ids;      // instances of this solve (one block each in the *_fin kernels)
     int32_t BN, BM;          // column-side / row-side blocks
     const int32_t *ptrA, *idxA, *ptrAT, *idxAT;
@@ -103,23 +106,19 @@ template <typename P> __device__ __forceinline__ P sel(P const (&a)[2], int i) {
 
 // Which instance a block of a two-sided launch (column side first) serves.
 struct Seat {
-    int k;       // instance
+    BatchInst I; // the instance
+    int k;       // its slot
     int lb, nb;  // block within the instance's side, blocks on that side
     bool col;    // column side
 };
-__device__ __forceinline__ Seat seat_two_sided(const BatchGeo& g, const BatchInst* __restrict__ inst) {
+__device__ __forceinline__ Seat seat_two_sided(const BatchGeo& g) {
     Seat s;
     const int b = blockIdx.x;
     s.col = b < g.BN;
-    if (s.col) {
-        s.k = g.tab_n[b];
-        s.lb = b - inst[s.k].on;
-        s.nb = inst[s.k].bn;
-    } else {
-        s.k = g.tab_m[b - g.BN];
-        s.lb = b - g.BN - inst[s.k].om;
-        s.nb = inst[s.k].bm;
-    }
+    s.I = s.col ? g.blk_n[b] : g.blk_m[b - g.BN];
+    s.k = s.I.id;
+    s.lb = s.col ? b - s.I.on : b - g.BN - s.I.om;
+    s.nb = s.col ? s.I.bn : s.I.bm;
     return s;
 }
 
@@ -226,8 +225,8 @@ __global__ __launch_bounds__(kBlock) void kb_primal(DevState* __restrict__ S, in
                                                     double* __restrict__ part_n) {
     __shared__ DevState Q;
     const int b = blockIdx.x;
-    const int k = g.tab_n[b];
-    const BatchInst I = g.inst[k];
+    const BatchInst I = g.blk_n[b];
+    const int k = I.id;
     const int lb = b - I.on;
     DevState* Sk = S + 2 * k;
     // the publisher is the instance's last block (the fewest rows): the step
@@ -314,8 +313,8 @@ __global__ __launch_bounds__(kBlock) void kb_dual(const DevState* __restrict__ S
                                                   double* __restrict__ dx2,
                                                   double* __restrict__ part_m) {
     const int b = blockIdx.x;
-    const int k = g.tab_m[b];
-    const BatchInst I = g.inst[k];
+    const BatchInst I = g.blk_m[b];
+    const int k = I.id;
     const int lb = b - I.om;
     const DevState* st = S + 2 * k + p1;
     if (!st->active) return;
@@ -376,8 +375,8 @@ __global__ __launch_bounds__(kBlock) void kb_check_prep(DevState* __restrict__ S
                                                         const double* __restrict__ part_m,
                                                         const double* __restrict__ dx2) {
     __shared__ DevState Q;
-    const Seat s = seat_two_sided(g, g.inst);
-    const BatchInst I = g.inst[s.k];
+    const Seat s = seat_two_sided(g);
+    const BatchInst& I = s.I;
     load_and_decide(S + 2 * s.k, 0, part_m + 2 * (size_t)I.om, I.bm, dx2 + s.k, Q,
                     s.col && s.lb == 0);
     if (!Q.need_check) return;
@@ -392,8 +391,8 @@ __global__ __launch_bounds__(kBlock) void kb_check_kkt(const DevState* __restric
                                                        BatchVals V, KktBufs Kc,
                                                        double* __restrict__ part_kn,
                                                        double* __restrict__ part_km) {
-    const Seat s = seat_two_sided(g, g.inst);
-    const BatchInst I = g.inst[s.k];
+    const Seat s = seat_two_sided(g);
+    const BatchInst& I = s.I;
     if (!S[2 * s.k + 1].need_check) return;
     const KktBufs K = inst_kkt(Kc, I);
     if (s.col) {
@@ -546,8 +545,8 @@ __global__ __launch_bounds__(kBlock) void kb_check_restart(const DevState* __res
                                                            IterBufs<double> D, KktBufs K,
                                                            double* __restrict__ part_rn,
                                                            double* __restrict__ part_rm) {
-    const Seat s = seat_two_sided(g, g.inst);
-    const BatchInst I = g.inst[s.k];
+    const Seat s = seat_two_sided(g);
+    const BatchInst& I = s.I;
     const DevState* st = S + 2 * s.k + 1;
     if (!st->need_check) return;
     double acc[1];
@@ -566,8 +565,8 @@ __global__ __launch_bounds__(kBlock) void kb_check_restart(const DevState* __res
 __global__ __launch_bounds__(kBlock) void kb_switch_products(const DevState* __restrict__ S,
                                                              BatchGeo g, BatchVals V,
                                                              IterBufs<double> Dc) {
-    const Seat s = seat_two_sided(g, g.inst);
-    const BatchInst I = g.inst[s.k];
+    const Seat s = seat_two_sided(g);
+    const BatchInst& I = s.I;
     const DevState* st = S + 2 * s.k + 1;
     if (!st->need_check || !st->switch_req) return;
     const IterBufs<double> B = inst_bufs(Dc, I);
@@ -638,8 +637,8 @@ __global__ __launch_bounds__(kBlock) void kb_status(const DevState* __restrict__
 // x = clip(0, l, u), y = 0, restart points = the start, sums = 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kb_cold(BatchGeo g, IterBufs<T> Bc) {
-    const Seat s = seat_two_sided(g, g.inst);
-    const BatchInst I = g.inst[s.k];
+    const Seat s = seat_two_sided(g);
+    const BatchInst& I = s.I;
     const IterBufs<T> B = inst_bufs(Bc, I);
     if (s.col) {
         for (int j = s.lb * kBlock + threadIdx.x; j < I.n; j += s.nb * kBlock) {
@@ -660,8 +659,7 @@ __global__ __launch_bounds__(kBlock) void kb_cold(BatchGeo g, IterBufs<T> Bc) {
 // A x of the cold start (row side only; grid = BM blocks).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kb_cold_ax(BatchGeo g, BatchVals V, IterBufs<T> Bc) {
-    const int k = g.tab_m[blockIdx.x];
-    const BatchInst I = g.inst[k];
+    const BatchInst I = g.blk_m[blockIdx.x];
     const IterBufs<T> B = inst_bufs(Bc, I);
     const MatView<T> A = inst_A(g, I, vals_A<T>(V));
     with_lanes(I.LA, [&](auto Lc) {
