@@ -386,9 +386,13 @@ void supr_batch_destroy(supr_batch* batch);
  * queue; every supr_server_step retires finished instances, admits queued
  * requests whose column, row and nonzero ranges fit the device arenas
  * (capacities fixed at creation: admission by HBM footprint), starts them
- * cold, and runs replays_per_step graph replays of the device-resident loop
- * over all running instances.  An instance's block counts and lanes are fixed
- * at admission, so its result is the same alone and in any mix.
+ * cold, and launches replays_per_step graph replays of the device-resident
+ * loop over all running instances.  A step does not wait for its replays (at
+ * most two steps' replays are in flight); admitted requests are uploaded on a
+ * stream of their own while the running replays continue, and the running set
+ * changes only after the replays in flight have drained.  An instance's block
+ * counts and lanes are fixed at admission, so its result is the same alone and
+ * in any mix.
  * supr_server_result returns a finished request's point (unscaled PDLP-form
  * coordinates) and information once (*ready = 1; then the result is
  * released); with x_out, y_out and info all NULL it only reports readiness

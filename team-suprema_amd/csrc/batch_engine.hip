@@ -305,9 +305,15 @@ struct Pool {
     int32_t* done_h = nullptr;    // host-coherent done flag per slot, written by kb_restart_fin
     int32_t* done_d = nullptr;
 
+    // uploads of admitted requests (the server): on their own stream, so they
+    // overlap the running set's replays on `stream`; null: uploads on `stream`
+    hipStream_t upload_stream = nullptr;
+
     ~Pool() {
+        if (stream) (void)hipStreamSynchronize(stream);
         if (status_h) (void)hipHostFree(status_h);
         if (done_h) (void)hipHostFree(done_h);
+        if (upload_stream) (void)hipStreamDestroy(upload_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -452,7 +458,7 @@ struct Pool {
         shape_rule(k);
         init[k] = p.init;
         obj_offset[k] = p.init.obj_offset;
-        const hipStream_t st = stream;
+        const hipStream_t st = upload_stream ? upload_stream : stream;
         auto put = [st](auto* dst, const auto& v) {
             if (!v.empty())
                 BATCH_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st));
@@ -470,7 +476,7 @@ struct Pool {
     // Copies p's arrays into the arenas at its ranges.
     template <typename Put>
     void upload(const Prepared& p, int64_t ro, int64_t co, int64_t za, Put&& put) {
-        hipStream_t st = stream;
+        hipStream_t st = upload_stream ? upload_stream : stream;
         put(ptrA.get<int32_t>() + ro, p.ptrA);
         put(ptrAT.get<int32_t>() + co, p.ptrAT);
         put(idxA.get<int32_t>() + za, p.idxA);
@@ -864,8 +870,23 @@ struct Server {
     BatchGeo geo{};                     // the running set's layout (direct launches)
     bool recapture = false;             // the running set changed since the graph was captured
     int32_t finished_total = 0;
+    // steps whose replays may still run (at most two), with their end events
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int inflight = 0;
+    int64_t steps = 0;
 
-    ~Server() { if (exec) (void)hipGraphExecDestroy(exec); }
+    ~Server() {
+        if (pool.stream) (void)hipStreamSynchronize(pool.stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    }
+
+    // Waits for every launched replay (before the running set changes: the
+    // replays in flight read its layout, slots and ranges).
+    void drain() {
+        BATCH_HIP(hipStreamSynchronize(pool.stream));
+        inflight = 0;
+    }
 
     // Undoes the admission of `fresh` (not yet running): slots back to the
     // pool, requests back to the head of the queue in their order.
@@ -888,6 +909,8 @@ struct Server {
         const int cap = o.max_blocks > 0 ? o.max_blocks : 1024;
         const double bias = o.lanes_bias > 0 ? o.lanes_bias : 1.0;
         pool.create(o.max_cols, o.max_rows, o.max_nnz, o.max_instances, o.pdlp, bias, cap);
+        BATCH_HIP(hipStreamCreateWithFlags(&pool.upload_stream, hipStreamNonBlocking));
+        for (auto& e : ev) BATCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
 
     int64_t submit(const supr_lp& U, const supr_lp& S, const double* Dr, const double* Dc) {
@@ -909,9 +932,48 @@ struct Server {
     // pool together, after its result is stored; a request whose admission
     // fails goes back to the head of the queue with its host data; a failed
     // capture is retried at the next step.
+    // One step.  Admitted requests are uploaded (upload stream) while the
+    // running set's replays still run; the running set changes -- finished
+    // requests retired, admitted ones started -- only after the replays in
+    // flight have drained; then this step's replays are launched, with at most
+    // two steps' replays in flight (the host does not wait for them unless it
+    // must).  Finished requests are seen through the host-coherent done flags,
+    // which only move forward.
     void step(int32_t* n_running, int32_t* n_queued, int32_t* n_finished) {
-        if (!live.empty()) {
-            std::atomic_thread_fence(std::memory_order_acquire);
+        std::atomic_thread_fence(std::memory_order_acquire);
+        bool any_done = false;
+        for (int32_t k : live)
+            if (((volatile int32_t*)pool.done_h)[k]) { any_done = true; break; }
+        std::vector<int32_t> fresh;
+        while (true) {
+            std::unique_ptr<Request> r;
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                if (queue.empty()) break;
+                r = std::move(queue.front());
+                queue.pop_front();
+            }
+            int k = -1;
+            try {
+                k = pool.add(r->prep);   // free ranges and slot: no replay in flight reads them
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(mu);
+                queue.push_front(std::move(r));
+                drain();
+                unadmit(fresh);
+                throw;
+            }
+            if (k < 0) {
+                std::lock_guard<std::mutex> lock(mu);
+                queue.push_front(std::move(r));
+                break;
+            }
+            admitted[k] = Clock::now();
+            running[k] = std::move(r);
+            fresh.push_back(k);
+        }
+        if (any_done || !fresh.empty()) drain();
+        if (any_done) {
             for (size_t q = 0; q < live.size();) {
                 const int32_t k = live[q];
                 if (!((volatile int32_t*)pool.done_h)[k]) { ++q; continue; }
@@ -943,33 +1005,6 @@ struct Server {
                 recapture = true;
             }
         }
-        std::vector<int32_t> fresh;
-        while (true) {
-            std::unique_ptr<Request> r;
-            {
-                std::lock_guard<std::mutex> lock(mu);
-                if (queue.empty()) break;
-                r = std::move(queue.front());
-                queue.pop_front();
-            }
-            int k = -1;
-            try {
-                k = pool.add(r->prep);
-            } catch (...) {
-                std::lock_guard<std::mutex> lock(mu);
-                queue.push_front(std::move(r));
-                unadmit(fresh);
-                throw;
-            }
-            if (k < 0) {
-                std::lock_guard<std::mutex> lock(mu);
-                queue.push_front(std::move(r));
-                break;
-            }
-            admitted[k] = Clock::now();
-            running[k] = std::move(r);
-            fresh.push_back(k);
-        }
         if (!fresh.empty()) {
             try {
                 pool.cold_start(fresh, pool.opt.max_iterations);
@@ -982,6 +1017,7 @@ struct Server {
             recapture = true;
         }
         if (recapture) {
+            // the set changed after a drain: nothing in flight reads the old layout
             if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
             if (!live.empty()) {
                 geo = pool.layout(live);
@@ -992,9 +1028,16 @@ struct Server {
             recapture = false;
         }
         if (!live.empty()) {
+            const int slot = (int)(steps % 2);
+            if (inflight == 2) {
+                BATCH_HIP(hipEventSynchronize(ev[slot]));
+                --inflight;
+            }
             const int reps = so.replays_per_step > 0 ? so.replays_per_step : 1;
             for (int q = 0; q < reps; ++q) pool.launch(exec, geo, (int)live.size(), pool.graph_attempts());
-            BATCH_HIP(hipStreamSynchronize(pool.stream));
+            BATCH_HIP(hipEventRecord(ev[slot], pool.stream));
+            ++inflight;
+            ++steps;
         }
         *n_running = (int32_t)live.size();
         {
