@@ -94,6 +94,10 @@ template <typename V> struct MatView {
     // products are the stored values, so the sums are the same bits.
     const double* rsc = nullptr;
     const double* csc = nullptr;
+    // ELL with one lane per row, an even row count and more rows than one
+    // grid-stride pass holds: k_primal gives each lane two adjacent rows
+    // (ell_pair_dot), so its row streams move 8 or 16 bytes per lane access.
+    int32_t pairs = 0;
 };
 
 // Iterate buffers of one precision.
@@ -360,6 +364,55 @@ __device__ __forceinline__ Acc row_dot(const MatView<V>& M, const X* __restrict_
         }
     }
     return group_sum<L, Acc>(s);
+}
+
+// Two-element vector of a value type (one 8- or 16-byte access).
+template <typename T> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
+// Row products of the adjacent ELL rows r and r + 1 (r even, M.rows even) by
+// one lane.  The layout is column-major, so the two rows' entries of a slot
+// are adjacent: one 4- or 8-byte index access and one 8- or 16-byte value (or
+// row factor) access serve both.  Each row adds the same terms in the same
+// order as row_dot with one lane per row, so each sum has the same bits.
+template <typename T>
+__device__ __forceinline__ void ell_pair_dot(const MatView<T>& M, const T* __restrict__ x, int r,
+                                             T (&s)[2]) {
+    const size_t R = (size_t)M.rows;
+    s[0] = T(0);
+    s[1] = T(0);
+    auto slot = [&](int k, int (&c)[2]) {
+        const size_t q = (size_t)k * R + r;
+        if (M.idx16) {
+            const ushort2 u = *reinterpret_cast<const ushort2*>(M.idx16 + q);
+            c[0] = u.x;
+            c[1] = u.y;
+        } else {
+            const int2 u = *reinterpret_cast<const int2*>(M.idx + q);
+            c[0] = u.x;
+            c[1] = u.y;
+        }
+    };
+    if constexpr (std::is_same<T, double>::value) {
+        if (M.csc) {
+            const double2 f = *reinterpret_cast<const double2*>(M.rsc + r);
+            for (int k = 0; k < M.ell_w; ++k) {
+                int c[2];
+                slot(k, c);
+                s[0] += (M.csc[c[0]] * f.x) * x[c[0]];
+                s[1] += (M.csc[c[1]] * f.y) * x[c[1]];
+            }
+            return;
+        }
+    }
+    for (int k = 0; k < M.ell_w; ++k) {
+        int c[2];
+        slot(k, c);
+        const auto v = *reinterpret_cast<const typename Pair<T>::type*>(M.val + (size_t)k * R + r);
+        s[0] += v.x * x[c[0]];
+        s[1] += v.y * x[c[1]];
+    }
 }
 
 // row_dot for long CSR rows with the next group's index and value loads

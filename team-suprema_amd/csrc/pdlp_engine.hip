@@ -589,6 +589,9 @@ class Solver {
     bool one_pass(const DevMatrix& M) const {
         return one_pass_ && (long)M.nnz <= (long)kPreU * M.lanes * M.rows;
     }
+    // two adjacent rows of A^T per lane in the primal launches where the
+    // layout allows it (primal_view; SUPR_ROW_PAIRS=off: one row per lane)
+    bool row_pairs_ = true;
     int passes_ = 10;
     std::vector<char> integer_;
     std::unique_ptr<DevLp> dlp_;
@@ -1069,6 +1072,7 @@ void Solver::setup_device_resident(bool mixed) {
     const int n = n_, m = m_;
     const auto t0 = Clock::now();
     if (const char* e = std::getenv("SUPR_ONE_PASS"); e && std::strcmp(e, "off") == 0) one_pass_ = false;
+    if (const char* e = std::getenv("SUPR_ROW_PAIRS"); e && std::strcmp(e, "off") == 0) row_pairs_ = false;
     // The host passes over the vectors (x0, eta0, omega0; the row kinds and
     // the bound kinds) run on two worker threads while this thread uploads
     // and lays out the matrices; they only read the forms.
@@ -1245,6 +1249,18 @@ template <typename T> MatView<T> iter_view(const DevMatrix& M);
 template <> MatView<float> iter_view<float>(const DevMatrix& M) { return M.f32(); }
 template <> MatView<double> iter_view<double>(const DevMatrix& M) { return M.f64_iter(); }
 
+// A^T's view for the primal launches: two adjacent rows per lane
+// (MatView::pairs) where A^T is ELL with one lane per row, an even row count
+// and more rows than one pass of the capped grid holds.  Each row's values
+// are the same bits either way; the step statistic's partial sums group the
+// rows differently, so the trajectory differs in the last bits.
+template <typename T> MatView<T> primal_view(const DevMatrix& AT, bool pairs_on) {
+    MatView<T> v = iter_view<T>(AT);
+    v.pairs = pairs_on && AT.ell_w > 0 && AT.lanes == 1 && AT.rows % 2 == 0 &&
+              (long)AT.rows > (long)kMaxBlocks * kBlock;
+    return v;
+}
+
 // The iteration kernels, in their one-pass form (matrix slots and epilogue
 // operands loaded before the step decision) when pre is set and the launch
 // covers every row in one pass.
@@ -1270,7 +1286,7 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     hipStream_t s = stream_.s;
     DevState* S = state_.as<DevState>();
     const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
-    const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
+    const MatView<T> A = iter_view<T>(A_), AT = primal_view<T>(AT_, row_pairs_);
     const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
     const bool pn = one_pass(AT_), pm = one_pass(A_);
     for (int k = 0; k < count; ++k) {
@@ -1631,7 +1647,7 @@ void Solver::bench_kernels(int reps, double* out) {
     auto run = [&](auto& I, auto tag, int slot) {
         using T = decltype(tag);
         const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
-        const MatView<T> A = iter_view<T>(A_), AT = iter_view<T>(AT_);
+        const MatView<T> A = iter_view<T>(A_), AT = primal_view<T>(AT_, row_pairs_);
         const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
         DevState* S = state_.as<DevState>();
         const DevState both[2] = {st, st};

@@ -194,6 +194,46 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
             const double d = (double)v - (double)xj;
             acc[0] += d * d;
         }
+    } else if (L == 1 && AT.pairs) {
+        // two adjacent rows per lane: the epilogue above on both, its streams
+        // read and written as one 8- or 16-byte access per lane
+        using V2 = typename Pair<T>::type;
+        for (int q = xcd_block() * kBlock + (int)threadIdx.x; 2 * q < AT.rows; q += (int)gridDim.x * kBlock) {
+            const int r = 2 * q;
+            T a[2];
+            ell_pair_dot(AT, y, r, a);
+            const V2 xv = *reinterpret_cast<const V2*>(x + r);
+            const T xj[2] = {xv.x, xv.y};
+            if (pending) {
+                V2 sv = *reinterpret_cast<const V2*>(B.xsum + r);
+                sv.x += eta_acc * xj[0];
+                sv.y += eta_acc * xj[1];
+                *reinterpret_cast<V2*>(B.xsum + r) = sv;
+            }
+            const V2 cv = *reinterpret_cast<const V2*>(B.c + r);
+            const T z[2] = {xj[0] - tau * (cv.x - a[0]), xj[1] - tau * (cv.y - a[1])};
+            T v[2];
+            if (B.l) {
+                const V2 lv = *reinterpret_cast<const V2*>(B.l + r);
+                const V2 uv = *reinterpret_cast<const V2*>(B.u + r);
+                v[0] = smin(smax(z[0], lv.x), uv.x);
+                v[1] = smin(smax(z[1], lv.y), uv.y);
+            } else {
+                v[0] = smax(z[0], T(0));
+                v[1] = smax(z[1], T(0));
+            }
+            V2 out;
+            out.x = v[0];
+            out.y = v[1];
+            *reinterpret_cast<V2*>(xn + r) = out;
+            if constexpr (sizeof(T) == 4)
+                reinterpret_cast<float4*>(B.xd)[q] = make_float4(v[0], v[0] - xj[0], v[1], v[1] - xj[1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double d = (double)v[h] - (double)xj[h];
+                acc[0] += d * d;
+            }
+        }
     } else if (staged_rows<L>(AT.rows)) {
         for_row_tiles<L, 1, T>(
             AT.rows, xcd_block(), gridDim.x,

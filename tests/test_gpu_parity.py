@@ -667,6 +667,47 @@ def test_one_pass_kernels_give_the_same_bits(monkeypatch, golden, mode):
               f"{on.info['lanes_A']}/{on.info['lanes_AT']}, same bits", file=sys.stderr, flush=True)
 
 
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_row_pairs_follow_the_one_row_per_lane_solve(monkeypatch, mode):
+    """A^T in ELL with one lane per row, an even number of rows and more rows
+    than one pass of the capped grid: the primal launches give each lane two
+    adjacent rows (SUPR_ROW_PAIRS=off: one row per lane).  Each row's values
+    are the same bits; only the grouping of the step statistic's partial sums
+    differs, so 64 attempts of both schedules, with a KKT check every 8
+    accepted steps and restarts to the average, agree to rounding (the random
+    LP in FP64: identical after 2 attempts, 1e-15 after 4, 2e-13 after 64, the
+    last-bit differences growing as the trajectories drift apart; 1e-7 after
+    160) -- a wrong row, index, bound or average update would not.  Covered:
+    16-bit indices with values formed from factors and x >= 0
+    (transportation), the same with stored values, and 32-bit indices, stored
+    values and finite upper bounds (random LP with 70,000 rows)."""
+    rtol = 1e-9 if mode == P.MODE_FAST_FP64 else 1e-5
+    cases = [("transport", P.gen_transport(1000, 600, seed=9), {}),
+             ("transport_stored", P.gen_transport(1000, 600, seed=9), {"SUPR_ELL_VALUES": "stored"}),
+             ("random_wide", P.gen_random_lp(70000, 600000, 3, seed=21), {})]
+    for name, lp, env in cases:
+        s, Dr, Dc = P.scale(lp)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        kb = P.bench_kernels(lp, s, Dr, Dc, reps=2)
+        assert kb["ell_AT"] > 0 and kb["lanes_AT"] == 1 and lp.n % 2 == 0 and lp.n > 2048 * 256, name
+        run = dict(tol=1e-12, mode=mode, max_iterations=64, check_every=8)
+        on = P.solve_pdhg_gpu(lp, s, Dr, Dc, **run)
+        monkeypatch.setenv("SUPR_ROW_PAIRS", "off")
+        off = P.solve_pdhg_gpu(lp, s, Dr, Dc, **run)
+        monkeypatch.delenv("SUPR_ROW_PAIRS")
+        for k in env:
+            monkeypatch.delenv(k)
+        assert on.iterations == off.iterations == 64, name
+        assert on.restarts == off.restarts >= 1, (name, on.restarts, off.restarts)
+        dx = np.abs(on.x - off.x).max() / (1.0 + np.abs(off.x).max())
+        dy = np.abs(on.y - off.y).max() / (1.0 + np.abs(off.y).max())
+        dobj = abs(on.info["objective"] - off.info["objective"]) / (1.0 + abs(off.info["objective"]))
+        assert dx <= rtol and dy <= rtol and dobj <= rtol, (name, dx, dy, dobj)
+        print(f"[parity] row pairs {name} mode {mode}: 64 attempts, {on.restarts} restarts, max rel "
+              f"diff x {dx:.1e} y {dy:.1e} objective {dobj:.1e}", file=sys.stderr, flush=True)
+
+
 def test_device_memory_cache_reuse_and_release():
     """Device memory a call releases is kept and reused by the next call (no
     hipMalloc / hipFree between one-shot calls); results do not depend on
