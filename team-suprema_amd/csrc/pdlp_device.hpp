@@ -421,22 +421,13 @@ __device__ __forceinline__ void ell_pair_dot(const MatView<T>& M, const T* __res
 // of two (index, then gather).  Same terms in the same order as row_dot, so
 // the same bits; more registers in flight, so it is used where rows are long
 // and a kernel has few wavefronts (the dual half).
+// row_dot_pipe's loop from entry p on, with the group at p already loaded
+// into c, v when it is full (p + (U - 1) L < e), then the predicated last
+// group; added into s.
 template <int L, typename Acc, typename V, typename X>
-__device__ __forceinline__ Acc row_dot_pipe(const MatView<V>& M, const X* __restrict__ x, int row,
-                                            int lane, bool valid) {
-    if (M.ell_w > 0) return row_dot<L, Acc>(M, x, row, lane, valid);
+__device__ __forceinline__ Acc csr_pipe_run(const MatView<V>& M, const X* __restrict__ x, int p,
+                                            int e, bool full, int (&c)[4], V (&v)[4], Acc s) {
     constexpr int U = 4;
-    Acc s = Acc(0);
-    int b = 0, e = 0;
-    if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
-    int p = b + lane;
-    bool full = p + (U - 1) * L < e;
-    int c[U];
-    V v[U];
-    if (full) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
-    }
     while (full) {
         X xv[U];
 #pragma unroll
@@ -475,7 +466,25 @@ __device__ __forceinline__ Acc row_dot_pipe(const MatView<V>& M, const X* __rest
         for (int u = 0; u < U; ++u)
             if (ok[u]) s += (Acc)vv[u] * (Acc)xv[u];
     }
-    return group_sum<L, Acc>(s);
+    return s;
+}
+
+template <int L, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc row_dot_pipe(const MatView<V>& M, const X* __restrict__ x, int row,
+                                            int lane, bool valid) {
+    if (M.ell_w > 0) return row_dot<L, Acc>(M, x, row, lane, valid);
+    constexpr int U = 4;
+    int b = 0, e = 0;
+    if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
+    const int p = b + lane;
+    const bool full = p + (U - 1) * L < e;
+    int c[U];
+    V v[U];
+    if (full) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
+    }
+    return group_sum<L, Acc>(csr_pipe_run<L, Acc>(M, x, p, e, full, c, v, Acc(0)));
 }
 
 // Two right-hand sides sharing the index stream (FP64, KKT check).
@@ -752,59 +761,6 @@ __device__ __forceinline__ Acc csr_dot_from(const MatView<V>& M, const X* __rest
     return s;
 }
 
-// row_dot_pipe's CSR loop from entry p on, added into s.
-template <int L, typename Acc, typename V, typename X>
-__device__ __forceinline__ Acc csr_dot_from_pipe(const MatView<V>& M, const X* __restrict__ x,
-                                                 int p, int e, Acc s) {
-    constexpr int U = 4;
-    bool full = p + (U - 1) * L < e;
-    int c[U];
-    V v[U];
-    if (full) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
-    }
-    while (full) {
-        X xv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
-        const int pn = p + U * L;
-        const bool nfull = pn + (U - 1) * L < e;
-        int cn[U];
-        V vn[U];
-        if (nfull) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) { cn[u] = M.idx[pn + u * L]; vn[u] = M.val[pn + u * L]; }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
-        p = pn;
-        full = nfull;
-        if (nfull) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) { c[u] = cn[u]; v[u] = vn[u]; }
-        }
-    }
-    if (p < e) {
-        bool ok[U];
-        int cc[U];
-        V vv[U];
-        X xv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            ok[u] = p + u * L < e;
-            cc[u] = ok[u] ? M.idx[p + u * L] : 0;
-            vv[u] = ok[u] ? M.val[p + u * L] : V(0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = ok[u] ? x[cc[u]] : X(0);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (ok[u]) s += (Acc)vv[u] * (Acc)xv[u];
-    }
-    return s;
-}
-
 // The row product from the prefetched slots: row_dot's terms in row_dot's
 // order (Pipe: the CSR rest as row_dot_pipe runs it).  Every lane of the
 // group gets the sum.
@@ -847,6 +803,24 @@ __device__ __forceinline__ Acc row_dot_pre(const MatView<V>& M, const X* __restr
                 s += (Acc)M.val[q] * (Acc)x[c];
             }
         }
+    } else if constexpr (Pipe) {
+        // the prefetched group is row_dot_pipe's first group: when it is full
+        // the loop starts from it (its gathers issued beside the next group's
+        // index and value loads); otherwise it is the row's last, partial group
+        if (P.ok[kPreU - 1]) {
+            int c[kPreU];
+            V v[kPreU];
+#pragma unroll
+            for (int u = 0; u < kPreU; ++u) { c[u] = P.c[u]; v[u] = P.v[u]; }
+            s = csr_pipe_run<L, Acc>(M, x, P.next - kPreU * L, P.end, true, c, v, s);
+        } else {
+            X xv[kPreU];
+#pragma unroll
+            for (int u = 0; u < kPreU; ++u) xv[u] = P.ok[u] ? x[P.c[u]] : X(0);
+#pragma unroll
+            for (int u = 0; u < kPreU; ++u)
+                if (P.ok[u]) s += (Acc)P.v[u] * (Acc)xv[u];
+        }
     } else {
         X xv[kPreU];
 #pragma unroll
@@ -854,8 +828,7 @@ __device__ __forceinline__ Acc row_dot_pre(const MatView<V>& M, const X* __restr
 #pragma unroll
         for (int u = 0; u < kPreU; ++u)
             if (P.ok[u]) s += (Acc)P.v[u] * (Acc)xv[u];
-        if constexpr (Pipe) s = csr_dot_from_pipe<L, Acc>(M, x, P.next, P.end, s);
-        else s = csr_dot_from<L, Acc>(M, x, P.next, P.end, s);
+        s = csr_dot_from<L, Acc>(M, x, P.next, P.end, s);
     }
     return group_sum<L, Acc>(s);
 }

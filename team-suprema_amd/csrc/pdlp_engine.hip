@@ -589,6 +589,16 @@ class Solver {
     bool one_pass(const DevMatrix& M) const {
         return one_pass_ && (long)M.nnz <= (long)kPreU * M.lanes * M.rows;
     }
+    // The FP64 dual launches also take the one-pass form on long CSR rows:
+    // the prefetched group starts the pipelined row loop (row_dot_pre), so
+    // the state, row-pointer, first index / value and epilogue loads overlap
+    // instead of following one another.  Only on grids of at most 1,024
+    // blocks (four wavefronts per SIMD), where the prefetch's registers cost
+    // no residency.
+    bool one_pass_dual(const DevMatrix& M, bool fp64) const {
+        return one_pass(M) ||
+               (one_pass_ && fp64 && M.ell_w == 0 && grid_rows(M.rows, M.lanes) <= 1024);
+    }
     // two adjacent rows of A^T per lane in the primal launches where the
     // layout allows it (primal_view; SUPR_ROW_PAIRS=off: one row per lane)
     bool row_pairs_ = true;
@@ -1288,7 +1298,7 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     const IterBufs<T> B = I.view(ineq_.as<uint8_t>());
     const MatView<T> A = iter_view<T>(A_), AT = primal_view<T>(AT_, row_pairs_);
     const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
-    const bool pn = one_pass(AT_), pm = one_pass(A_);
+    const bool pn = one_pass(AT_), pm = one_pass_dual(A_, sizeof(T) == 8);
     for (int k = 0; k < count; ++k) {
         const int p = k & 1;
         SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(pn, gn, s, S, p, AT, B, part_m_, gm, dx2_slot_, part_n_)));
@@ -1656,7 +1666,7 @@ void Solver::bench_kernels(int reps, double* out) {
             SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(one_pass(AT_), gn, s, S, 0, AT, B, part_m_, gm, dx2_slot_, part_n_)));
         });
         out[slot + 1] = time_kernel([&] {
-            SUPR_LANES(A_.lanes, (launch_dual<LL, T>(one_pass(A_), gm, s, S, 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
+            SUPR_LANES(A_.lanes, (launch_dual<LL, T>(one_pass_dual(A_, sizeof(T) == 8), gm, s, S, 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
         });
         SUPR_HIP(hipGetLastError());
     };

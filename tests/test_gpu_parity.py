@@ -644,10 +644,13 @@ def test_one_pass_kernels_give_the_same_bits(monkeypatch, golden, mode):
     Same terms in the same order, so the same solve bit for bit: ELL A^T with
     several lanes per row (random LP), ELL A^T with values formed from factors
     (transportation), CSR on both sides with rows longer than one prefetched
-    group (Netlib, power-law), bounded columns and inequality rows, and lanes
-    pinned from 1 to 64."""
+    group (Netlib, power-law), bounded columns and inequality rows, lanes
+    pinned from 1 to 64, and an FP64 dual over rows of 200 and 1,000 entries
+    whose prefetched group starts the pipelined row loop (transportation
+    200 x 1,000)."""
     cases = [("random", P.gen_random_lp(400, 1000, 6, seed=13), {}),
              ("transport", P.gen_transport(60, 80, seed=4), {}),
+             ("transport_long_rows", P.gen_transport(200, 1000, seed=4), {}),
              ("power_law", P.gen_power_law_lp(1000, 4000, seed=3), {}),
              ("power_law_l1", P.gen_power_law_lp(1000, 4000, seed=3), dict(spmv_lanes_A=1, spmv_lanes_AT=2)),
              ("random_l64", P.gen_random_lp(400, 1000, 6, seed=13), dict(spmv_lanes_A=64, spmv_lanes_AT=4))]
