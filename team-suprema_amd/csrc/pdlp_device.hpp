@@ -423,7 +423,7 @@ __device__ __forceinline__ void ell_pair_dot(const MatView<T>& M, const T* __res
 // and a kernel has few wavefronts (the dual half).
 // row_dot_pipe's loop from entry p on, with the group at p already loaded
 // into c, v when it is full (p + (U - 1) L < e), then the predicated last
-// group; added into s.
+// group; added into s (the one-pass entry, row_dot_pre).
 template <int L, typename Acc, typename V, typename X>
 __device__ __forceinline__ Acc csr_pipe_run(const MatView<V>& M, const X* __restrict__ x, int p,
                                             int e, bool full, int (&c)[4], V (&v)[4], Acc s) {
@@ -469,22 +469,64 @@ __device__ __forceinline__ Acc csr_pipe_run(const MatView<V>& M, const X* __rest
     return s;
 }
 
+// (row_dot_pipe keeps its own copy of csr_pipe_run's loop: called through
+// csr_pipe_run, the grid-stride FP64 dual kernel took 68 instead of 59
+// VGPRs, a wavefront per SIMD less.)
 template <int L, typename Acc, typename V, typename X>
 __device__ __forceinline__ Acc row_dot_pipe(const MatView<V>& M, const X* __restrict__ x, int row,
                                             int lane, bool valid) {
     if (M.ell_w > 0) return row_dot<L, Acc>(M, x, row, lane, valid);
     constexpr int U = 4;
+    Acc s = Acc(0);
     int b = 0, e = 0;
     if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
-    const int p = b + lane;
-    const bool full = p + (U - 1) * L < e;
+    int p = b + lane;
+    bool full = p + (U - 1) * L < e;
     int c[U];
     V v[U];
     if (full) {
 #pragma unroll
         for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; v[u] = M.val[p + u * L]; }
     }
-    return group_sum<L, Acc>(csr_pipe_run<L, Acc>(M, x, p, e, full, c, v, Acc(0)));
+    while (full) {
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+        const int pn = p + U * L;
+        const bool nfull = pn + (U - 1) * L < e;
+        int cn[U];
+        V vn[U];
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { cn[u] = M.idx[pn + u * L]; vn[u] = M.val[pn + u * L]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += (Acc)v[u] * (Acc)xv[u];
+        p = pn;
+        full = nfull;
+        if (nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) { c[u] = cn[u]; v[u] = vn[u]; }
+        }
+    }
+    if (p < e) {
+        bool ok[U];
+        int cc[U];
+        V vv[U];
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = p + u * L < e;
+            cc[u] = ok[u] ? M.idx[p + u * L] : 0;
+            vv[u] = ok[u] ? M.val[p + u * L] : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = ok[u] ? x[cc[u]] : X(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) s += (Acc)vv[u] * (Acc)xv[u];
+    }
+    return group_sum<L, Acc>(s);
 }
 
 // Two right-hand sides sharing the index stream (FP64, KKT check).
@@ -759,6 +801,52 @@ __device__ __forceinline__ Acc csr_dot_from(const MatView<V>& M, const X* __rest
             if (ok[u]) s += (Acc)v[u] * (Acc)xv[u];
     }
     return s;
+}
+
+// Long CSR rows in one pass (the FP64 dual over few, long rows): a lane's
+// entries p, p + L, p + 2L, ... in chunks of kFlatK, each chunk's index and
+// value loads issued together and then its gathers together, so a lane waits
+// about two memory latencies per chunk; the first chunk is loaded before the
+// state.  The terms are added in row_dot_pipe's order, so the same bits.
+constexpr int kFlatK = 32;
+template <typename V> struct FlatSlots {
+    int c[kFlatK];   // -1: past the row's end
+    V v[kFlatK];
+};
+
+template <int L, typename V>
+__device__ __forceinline__ void flat_load(const MatView<V>& M, int p, int e, FlatSlots<V>& F) {
+#pragma unroll
+    for (int u = 0; u < kFlatK; ++u) {
+        const bool ok = p + u * L < e;
+        F.c[u] = ok ? M.idx[p + u * L] : -1;
+        F.v[u] = ok ? M.val[p + u * L] : V(0);
+    }
+}
+
+template <typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc flat_sum(const FlatSlots<V>& F, const X* __restrict__ x, Acc s) {
+    X xv[kFlatK];
+#pragma unroll
+    for (int u = 0; u < kFlatK; ++u) xv[u] = F.c[u] >= 0 ? x[F.c[u]] : X(0);
+#pragma unroll
+    for (int u = 0; u < kFlatK; ++u)
+        if (F.c[u] >= 0) s += (Acc)F.v[u] * (Acc)xv[u];
+    return s;
+}
+
+// The row product from the first chunk F (loaded at p) and the rest of the
+// row up to e; every lane of the group gets the sum.
+template <int L, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc row_dot_flat(const MatView<V>& M, const X* __restrict__ x, int p,
+                                            int e, const FlatSlots<V>& F) {
+    Acc s = flat_sum<Acc>(F, x, Acc(0));
+    for (int q = p + kFlatK * L; q < e; q += kFlatK * L) {
+        FlatSlots<V> G;
+        flat_load<L>(M, q, e, G);
+        s = flat_sum<Acc>(G, x, s);
+    }
+    return group_sum<L, Acc>(s);
 }
 
 // The row product from the prefetched slots: row_dot's terms in row_dot's

@@ -253,10 +253,12 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
 //   axn = A xn, ad = A (xn - x);  [pending] ysum += eta_acc y;
 //   yn = proj(y + sigma (b - (axn + ad)));  partials of (yn-y)^2 and
 //   (yn-y)'ad.  The last block also sums k_primal's partials into *dx2_slot.
-// Pre: a one-pass launch; the lane's first matrix slots and its row's
+// Pre: 1, a one-pass launch; the lane's first matrix slots and its row's
 // epilogue operands (both y and A x buffers, b, ysum, the row kind) are
-// loaded before the state.  The same arithmetic, so the same bits.
-template <int L, typename T, bool Pre>
+// loaded before the state.  2 (FP64, CSR): the same with the lane's first
+// chunk of entries (row_dot_flat) instead of the slots.  The same arithmetic,
+// so the same bits.
+template <int L, typename T, int Pre>
 __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S, int p1,
                                                  MatView<T> A, IterBufs<T> B,
                                                  const double* __restrict__ part_n, int nb_n,
@@ -268,11 +270,19 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
     PreSlots<T> P;
     T py[2] = {T(0), T(0)}, pax[2] = {T(0), T(0)}, pb = T(0), ps = T(0);
     uint8_t pineq = 0;
-    if constexpr (Pre) {
+    FlatSlots<T> F;
+    int fp = 0, fe = 0;
+    if constexpr (Pre != 0) {
         prow = xcd_block() * (kBlock / L) + (int)threadIdx.x / L;
         plane = (int)threadIdx.x % L;
         pvalid = prow < A.rows;
-        pre_slots<L>(A, prow, plane, pvalid, P);
+        if constexpr (Pre == 2) {
+            if (pvalid) { fp = A.ptr[prow]; fe = A.ptr[prow + 1]; }
+            fp += plane;
+            flat_load<L>(A, fp, fe, F);
+        } else {
+            pre_slots<L>(A, prow, plane, pvalid, P);
+        }
         if (pvalid && plane == 0) {
             py[0] = B.y[0][prow];
             py[1] = B.y[1][prow];
@@ -327,9 +337,12 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
         acc[0] += d * d;
         acc[1] += d * (double)ad;
     };
-    if constexpr (Pre) {
+    if constexpr (Pre != 0) {
         T a, ad = T(0);
-        if constexpr (kDiff) {
+        if constexpr (Pre == 2) {
+            static_assert(!kDiff, "the flat one-pass dual is an FP64 form");
+            a = row_dot_flat<L, T>(A, xn, fp, fe, F);
+        } else if constexpr (kDiff) {
             row_dot_diff_pre<L>(A, reinterpret_cast<const float2*>(B.xd), prow, P, a, ad);
         } else {
             a = row_dot_pre<L, true, T>(A, xn, prow, P);

@@ -646,8 +646,8 @@ def test_one_pass_kernels_give_the_same_bits(monkeypatch, golden, mode):
     (transportation), CSR on both sides with rows longer than one prefetched
     group (Netlib, power-law), bounded columns and inequality rows, lanes
     pinned from 1 to 64, and an FP64 dual over rows of 200 and 1,000 entries
-    whose prefetched group starts the pipelined row loop (transportation
-    200 x 1,000)."""
+    in the flat one-pass form (each lane's entries in chunks of 32, the first
+    loaded before the state; transportation 200 x 1,000)."""
     cases = [("random", P.gen_random_lp(400, 1000, 6, seed=13), {}),
              ("transport", P.gen_transport(60, 80, seed=4), {}),
              ("transport_long_rows", P.gen_transport(200, 1000, seed=4), {}),
