@@ -803,7 +803,7 @@ __device__ __forceinline__ Acc csr_dot_from(const MatView<V>& M, const X* __rest
     return s;
 }
 
-// Long CSR rows in one pass (the FP64 dual over few, long rows): a lane's
+// Long CSR rows in one pass (the dual over few, long rows): a lane's
 // entries p, p + L, p + 2L, ... in chunks of kFlatK, each chunk's index and
 // value loads issued together and then its gathers together, so a lane waits
 // about two memory latencies per chunk; the first chunk is loaded before the
@@ -847,6 +847,32 @@ __device__ __forceinline__ Acc row_dot_flat(const MatView<V>& M, const X* __rest
         s = flat_sum<Acc>(G, x, s);
     }
     return group_sum<L, Acc>(s);
+}
+
+// row_dot_diff in the flat form (FP32 loop: A xn and A (xn - x) from the
+// interleaved pairs), the terms in row_dot_diff's order.
+template <int L>
+__device__ __forceinline__ void row_dot_diff_flat(const MatView<float>& M,
+                                                  const float2* __restrict__ xd, int p, int e,
+                                                  const FlatSlots<float>& F, float& s1, float& sd) {
+    s1 = 0.f;
+    sd = 0.f;
+    auto chunk = [&](const FlatSlots<float>& G) {
+        float2 w[kFlatK];
+#pragma unroll
+        for (int u = 0; u < kFlatK; ++u) w[u] = G.c[u] >= 0 ? xd[G.c[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kFlatK; ++u)
+            if (G.c[u] >= 0) { s1 += G.v[u] * w[u].x; sd += G.v[u] * w[u].y; }
+    };
+    chunk(F);
+    for (int q = p + kFlatK * L; q < e; q += kFlatK * L) {
+        FlatSlots<float> G;
+        flat_load<L>(M, q, e, G);
+        chunk(G);
+    }
+    s1 = group_sum<L, float>(s1);
+    sd = group_sum<L, float>(sd);
 }
 
 // The row product from the prefetched slots: row_dot's terms in row_dot's

@@ -590,14 +590,14 @@ class Solver {
         return one_pass_ && (long)M.nnz <= (long)kPreU * M.lanes * M.rows;
     }
     // The dual launches' form (launch_dual): the one-pass form where one
-    // prefetched group holds a mean row (1); for long CSR rows in FP64 the
-    // flat one-pass form (2: each lane's entries in chunks of 32, the first
-    // loaded before the state, every chunk's gathers issued together), on
-    // grids of at most 1,024 blocks (four wavefronts per SIMD), where its
-    // registers cost no residency; otherwise the grid-stride kernel (0).
-    int one_pass_dual(const DevMatrix& M, bool fp64) const {
+    // prefetched group holds a mean row (1); for long CSR rows the flat
+    // one-pass form (2: each lane's entries in chunks of 32, the first loaded
+    // before the state, every chunk's gathers issued together), on grids of
+    // at most 1,024 blocks (four wavefronts per SIMD), where its registers
+    // cost no residency; otherwise the grid-stride kernel (0).
+    int one_pass_dual(const DevMatrix& M) const {
         if (one_pass(M)) return 1;
-        return one_pass_ && fp64 && M.ell_w == 0 && grid_rows(M.rows, M.lanes) <= 1024 ? 2 : 0;
+        return one_pass_ && M.ell_w == 0 && grid_rows(M.rows, M.lanes) <= 1024 ? 2 : 0;
     }
     // two adjacent rows of A^T per lane in the primal launches where the
     // layout allows it (primal_view; SUPR_ROW_PAIRS=off: one row per lane)
@@ -1283,15 +1283,12 @@ void launch_primal(bool pre, int grid, hipStream_t s, DevState* S, int p, const 
         hipLaunchKernelGGL((k_primal<L, T, false>), dim3(grid), dim3(kBlock), 0, s, S, p, AT, B, part_m, nb_m, dx2, part_n);
 }
 // pre: 0 the grid-stride dual, 1 the one-pass form, 2 the one-pass form
-// over long CSR rows (FP64; row_dot_flat)
+// over long CSR rows (row_dot_flat, row_dot_diff_flat)
 template <int L, typename T>
 void launch_dual(int pre, int grid, hipStream_t s, DevState* S, int p1, const MatView<T>& A,
                  const IterBufs<T>& B, const double* part_n, int nb_n, double* dx2, double* part_m) {
     if (pre == 2 && one_pass_rows<L>(A.rows) && A.ell_w == 0) {
-        if constexpr (sizeof(T) == 8)
-            hipLaunchKernelGGL((k_dual<L, T, 2>), dim3(grid), dim3(kBlock), 0, s, S, p1, A, B, part_n, nb_n, dx2, part_m);
-        else
-            throw std::logic_error("the flat one-pass dual is an FP64 form");
+        hipLaunchKernelGGL((k_dual<L, T, 2>), dim3(grid), dim3(kBlock), 0, s, S, p1, A, B, part_n, nb_n, dx2, part_m);
     } else if (pre != 0 && one_pass_rows<L>(A.rows)) {
         hipLaunchKernelGGL((k_dual<L, T, 1>), dim3(grid), dim3(kBlock), 0, s, S, p1, A, B, part_n, nb_n, dx2, part_m);
     } else {
@@ -1307,7 +1304,7 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     const MatView<T> A = iter_view<T>(A_), AT = primal_view<T>(AT_, row_pairs_);
     const int gn = grid_rows(n_, AT_.lanes), gm = grid_rows(m_, A_.lanes);
     const bool pn = one_pass(AT_);
-    const int pm = one_pass_dual(A_, sizeof(T) == 8);
+    const int pm = one_pass_dual(A_);
     for (int k = 0; k < count; ++k) {
         const int p = k & 1;
         SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(pn, gn, s, S, p, AT, B, part_m_, gm, dx2_slot_, part_n_)));
@@ -1675,7 +1672,7 @@ void Solver::bench_kernels(int reps, double* out) {
             SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(one_pass(AT_), gn, s, S, 0, AT, B, part_m_, gm, dx2_slot_, part_n_)));
         });
         out[slot + 1] = time_kernel([&] {
-            SUPR_LANES(A_.lanes, (launch_dual<LL, T>(one_pass_dual(A_, sizeof(T) == 8), gm, s, S, 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
+            SUPR_LANES(A_.lanes, (launch_dual<LL, T>(one_pass_dual(A_), gm, s, S, 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
         });
         SUPR_HIP(hipGetLastError());
     };

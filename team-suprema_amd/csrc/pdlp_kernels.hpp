@@ -255,9 +255,9 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
 //   (yn-y)'ad.  The last block also sums k_primal's partials into *dx2_slot.
 // Pre: 1, a one-pass launch; the lane's first matrix slots and its row's
 // epilogue operands (both y and A x buffers, b, ysum, the row kind) are
-// loaded before the state.  2 (FP64, CSR): the same with the lane's first
-// chunk of entries (row_dot_flat) instead of the slots.  The same arithmetic,
-// so the same bits.
+// loaded before the state.  2 (CSR): the same with the lane's first chunk
+// of entries (row_dot_flat, row_dot_diff_flat) instead of the slots.  The
+// same arithmetic, so the same bits.
 template <int L, typename T, int Pre>
 __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S, int p1,
                                                  MatView<T> A, IterBufs<T> B,
@@ -340,8 +340,10 @@ __global__ __launch_bounds__(kBlock) void k_dual(const DevState* __restrict__ S,
     if constexpr (Pre != 0) {
         T a, ad = T(0);
         if constexpr (Pre == 2) {
-            static_assert(!kDiff, "the flat one-pass dual is an FP64 form");
-            a = row_dot_flat<L, T>(A, xn, fp, fe, F);
+            if constexpr (kDiff)
+                row_dot_diff_flat<L>(A, reinterpret_cast<const float2*>(B.xd), fp, fe, F, a, ad);
+            else
+                a = row_dot_flat<L, T>(A, xn, fp, fe, F);
         } else if constexpr (kDiff) {
             row_dot_diff_pre<L>(A, reinterpret_cast<const float2*>(B.xd), prow, P, a, ad);
         } else {
