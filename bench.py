@@ -72,6 +72,23 @@ def nonneg_bounds(lp, s):
     return bool(np.all((l == 0) & ~np.signbit(l)) and np.all(np.isposinf(u)))
 
 
+def dual_form(rows, nnz, lanes, ell):
+    """The dual kernel's third template argument for a launch over A with
+    `rows` rows, `nnz` entries and `lanes` lanes per row (the engine's rule:
+    Solver::one_pass_dual and launch_dual): 1 the one-pass form (a mean row
+    fits one prefetched group of 4 per lane), 2 the flat one-pass form (long
+    CSR rows, at most 1,024 blocks), 0 the grid-stride kernel."""
+    staged = lanes >= 2 and rows >= (1 << 17)
+    if not (lanes <= 64 and not staged and rows <= 2048 * (256 // lanes)):
+        return 0
+    if nnz <= 4 * lanes * rows:
+        return 1
+    g = min(max(1, -(-rows // (256 // lanes))), 2048)
+    if g >= 8:
+        g = -(-g // 8) * 8
+    return 2 if not ell and g <= 1024 else 0
+
+
 def kernel_bytes(lp, s, ell_AT, ell_A, nonneg=False, store_aty=False, rebuilt=False, compressed=True):
     """Algorithmic bytes of one launch of each iteration kernel (every array
     read or written once), value/vector size s (4: FP32 loop, 8: FP64 loop).
@@ -834,12 +851,14 @@ def main():
         return
     if args.pmc_summary:
         lanes = [32, 1]
-        # the 1M-variable LP's launches take the grid-stride kernels (template
-        # flag false: A^T's rows exceed one pass, A's rows are long)
+        # the 1M-variable LP's primal launches take the grid-stride kernel
+        # (template flag false: A^T's rows exceed one pass), its dual launches
+        # the flat one-pass form over A's 2,000 rows of 1,000
+        df = dual_form(2000, 2000000, lanes[0], False)
         print(json.dumps(pmc_summary(*args.pmc_summary, {
-            "k_primal_f32": f"k_primal<{lanes[1]}, float, false>", "k_dual_f32": f"k_dual<{lanes[0]}, float, false>",
+            "k_primal_f32": f"k_primal<{lanes[1]}, float, false>", "k_dual_f32": f"k_dual<{lanes[0]}, float, {df}>",
             "k_primal_f64": f"k_primal<{lanes[1]}, double, false>",
-            "k_dual_f64": f"k_dual<{lanes[0]}, double, false>"}), indent=1))
+            "k_dual_f64": f"k_dual<{lanes[0]}, double, {df}>"}), indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -965,7 +984,7 @@ def main():
                              rocprof=f"k_primal<{lat}, double, false>",
                              rebuilt_values=bool(kb.get("rebuilt_AT_fp64", 0) > 0)),
         "k_dual_f64": dict(ms=kb["dual_ms_fp64"], bytes=u64[1], format_bytes=dual64,
-                           rocprof=f"k_dual<{la}, double, false>"),
+                           rocprof=f"k_dual<{la}, double, {dual_form(lp.m, lp.nnz, la, kb['ell_A'] > 0)}>"),
     }
     for k in kernels.values():
         k["GBps"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
