@@ -1392,6 +1392,32 @@ struct CheckPlan {
     bool slack_due = false;               // the last unit ended at an expected check
     static constexpr int kMaxUnit = 32;   // attempts per unit without a check group
     static constexpr int kSlack = 4;      // attempts in the unit after an expected check
+    static constexpr int kRateSpan = 128; // attempts the rejection rate is taken over
+    // (attempts, accepted) of each state the host has seen, one per check
+    // group count
+    std::vector<std::pair<int32_t, int32_t>> seen;
+    int32_t seen_groups = -1;
+
+    // The rejection rate of the last kRateSpan or more attempts (the loop's
+    // whole rate until it has that many).  The rate drifts down as the step
+    // size settles: on the 1M-variable LP the whole loop's 5% placed the late
+    // checks' groups 4 attempts after the check came due (4 no-op attempts
+    // per check), where the recent rate is 0.
+    double rejection(const DevState& hs) {
+        if (hs.groups != seen_groups) {
+            seen.push_back({hs.attempts, hs.accepted});
+            seen_groups = hs.groups;
+        }
+        int32_t a0 = 0, c0 = 0;
+        for (size_t j = seen.size(); j-- > 0;)
+            if (seen[j].first <= hs.attempts - kRateSpan) {
+                a0 = seen[j].first;
+                c0 = seen[j].second;
+                break;
+            }
+        const int32_t da = hs.attempts - a0;
+        return da > 0 ? std::min(0.5, (double)(da - (hs.accepted - c0)) / da) : 0.0;
+    }
 
     // The next unit from the last known state: attempts (even) and whether
     // a check group follows them.
@@ -1402,7 +1428,7 @@ struct CheckPlan {
         }
         const int64_t jp = std::max<int64_t>(0, hs.groups - base_groups);
         const int64_t at = jp == 0 ? 0 : group_at[(size_t)std::min<int64_t>(jp, (int64_t)group_at.size()) - 1];
-        const double rej = hs.attempts > 0 ? std::min(0.5, (double)(hs.attempts - hs.accepted) / hs.attempts) : 0.0;
+        const double rej = rejection(hs);
         const int64_t ce = check_every;
         const int64_t due_acc = hs.need_check ? hs.accepted : ((int64_t)hs.accepted / ce + 1) * ce;
         int64_t target = at + (int64_t)std::ceil((double)(due_acc - hs.accepted) / (1.0 - rej));
