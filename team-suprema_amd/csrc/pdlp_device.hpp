@@ -652,67 +652,6 @@ __device__ __forceinline__ void row_dot_diff(const MatView<float>& M, const floa
     sd = group_sum<L, float>(sd);
 }
 
-// row_dot_diff with the next group's loads issued before this group's
-// gathers are consumed (see row_dot_pipe); the same terms in the same order.
-template <int L>
-__device__ __forceinline__ void row_dot_diff_pipe(const MatView<float>& M,
-                                                  const float2* __restrict__ xd, int row, int lane,
-                                                  bool valid, float& s1, float& sd) {
-    if (M.ell_w > 0) { row_dot_diff<L>(M, xd, row, lane, valid, s1, sd); return; }
-    constexpr int U = 4;
-    s1 = 0.f; sd = 0.f;
-    int b = 0, e = 0;
-    if (valid) { b = M.ptr[row]; e = M.ptr[row + 1]; }
-    int p = b + lane;
-    bool full = p + (U - 1) * L < e;
-    int c[U];
-    float a[U];
-    if (full) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) { c[u] = M.idx[p + u * L]; a[u] = M.val[p + u * L]; }
-    }
-    while (full) {
-        float2 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = xd[c[u]];
-        const int pn = p + U * L;
-        const bool nfull = pn + (U - 1) * L < e;
-        int cn[U];
-        float an[U];
-        if (nfull) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) { cn[u] = M.idx[pn + u * L]; an[u] = M.val[pn + u * L]; }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) { s1 += a[u] * v[u].x; sd += a[u] * v[u].y; }
-        p = pn;
-        full = nfull;
-        if (nfull) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) { c[u] = cn[u]; a[u] = an[u]; }
-        }
-    }
-    if (p < e) {
-        bool ok[U];
-        int cc[U];
-        float aa[U];
-        float2 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            ok[u] = p + u * L < e;
-            cc[u] = ok[u] ? M.idx[p + u * L] : 0;
-            aa[u] = ok[u] ? M.val[p + u * L] : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = ok[u] ? xd[cc[u]] : make_float2(0.f, 0.f);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (ok[u]) { s1 += aa[u] * v[u].x; sd += aa[u] * v[u].y; }
-    }
-    s1 = group_sum<L, float>(s1);
-    sd = group_sum<L, float>(sd);
-}
-
 // ---------------------------------------------- one-pass launches --
 //
 // When a launch's grid covers every row in a single pass (no grid-stride
