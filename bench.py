@@ -29,7 +29,8 @@ width); beside it:
     column, m/n in [0.2, 0.8], nnz log-uniform over [1e4, 2e7]) through the
     continuous-batching server, every result verified; p50/p99 latency and
     solves/s, against one-at-a-time plain FP64 solves of the same requests
-    (their latency from a FIFO queue over the same arrivals);
+    (their latency from a FIFO queue over the same arrivals); "serving-solve":
+    the server's runs of the narrow stream alone (for tracing its kernels);
   * "c0": the random equality LP m = 2,000, n = 5,000 in each mode, beside the
     reference's own CPU PDLP run to completion;
   * "large": the power-law LP m = 500,000, n = 2,000,000 (35.1M nonzeros):
@@ -338,14 +339,16 @@ def bench_batched(count=512, sample=8):
 
 
 def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=None, span=False,
-                  saturated=True):
+                  saturated=True, baselines=True, server_options=None):
     """Open-loop Poisson stream through the server against one-at-a-time
     plain FP64 solves of the same requests.  plain_sample_attempts: time the
     plain solves on that many attempts each and scale by the request's
     attempt count in the server (for requests too large to solve plainly in
     a bench run; the output says so).  span: the requests' nnz are spread
     log-evenly over [1e4, nnz_hi] (both ends included), in random order.
-    saturated: also serve every request submitted at once."""
+    saturated: also serve every request submitted at once.  baselines:
+    also the plain FP64 and CPU baselines (off: the server's runs alone, for
+    tracing its kernels).  server_options: more P.Server options."""
     import threading
     rng = np.random.default_rng(seed)
     if span:
@@ -366,7 +369,7 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
     caps = dict(max_cols=1 << 24, max_rows=1 << 24, max_nnz=1 << 27, max_instances=64, max_blocks=1024)
 
     def serve(open_loop):
-        with P.Server(mode=P.MODE_MIXED, tol=TOL, **caps) as srv:
+        with P.Server(mode=P.MODE_MIXED, tol=TOL, **caps, **(server_options or {})) as srv:
             tickets, done, res = [None] * count, [None] * count, [None] * count
             lock = threading.Lock()
             t0 = time.perf_counter()
@@ -420,6 +423,10 @@ def bench_serving(count=10, qps=1.5, nnz_hi=1e6, seed=7, plain_sample_attempts=N
     ol = serve(True)
     sat = serve(False) if saturated else None
     base = sat if sat is not None else ol
+    if not baselines:
+        log(f"serving: " + (f"saturated {sat['solves_per_s']:.2f} solves/s; " if sat else "")
+            + f"open loop {qps} qps p50 {ol['latency_p50']:.2f} s p99 {ol['latency_p99']:.2f} s")
+        return {"request_nnz": [int(r[0].nnz) for r in reqs], "saturated": sat, "open_loop": ol}
     svc = []
     for i in range(count):
         t0 = time.perf_counter()
@@ -826,7 +833,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the power-law LP leg (about 3 minutes)")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--no-prep", action="store_true")
-    ap.add_argument("--only", choices=["c0", "batched", "batched-solve", "serving", "serving-full", "large", "large-mixed",
+    ap.add_argument("--only", choices=["c0", "batched", "batched-solve", "serving", "serving-solve", "serving-full",
+                                       "large", "large-mixed",
                                        "large-kernels", "ingest", "prep", "one-shot"],
                     help="run only this section and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -880,7 +888,11 @@ def main():
                                          saturated=False, plain_sample_attempts=1000)
     if args.only and args.only != "one-shot":
         fn = {"c0": bench_c0, "batched": bench_batched, "batched-solve": bench_batched_solve,
-              "serving": bench_serving, "large": bench_large,
+              "serving": bench_serving,
+              # the server's runs alone, replays launched kernel by kernel (the
+              # run to trace its kernels; the same bits as the graph replays)
+              "serving-solve": lambda: bench_serving(baselines=False, server_options=dict(graph_attempts=-16)),
+              "large": bench_large,
               "large-mixed": lambda: bench_large(P.MODE_MIXED), "ingest": bench_ingest,
               "prep": bench_prep, "large-kernels": bench_large_kernels,
               "serving-full": serving_full}[args.only]
