@@ -1077,21 +1077,38 @@ __device__ __forceinline__ void decide_step(DevState* st, double dx2, double dy2
 // Loads the state S[p] into LDS, applies the pending step decision (if the
 // previous launch ran an attempt), and publishes the result to S[p ^ 1]
 // (the publisher block only).  Returns the block's copy.  Wavefront 0 alone
-// loads the state, the primal sum and the dual partials (issued together:
-// one memory latency), reduces the partials without a block barrier, and
-// decides; one barrier then hands the state to the other wavefronts.
+// loads the state, the primal sum and the dual partials, reduces the
+// partials without a block barrier, and decides; one barrier then hands the
+// state to the other wavefronts.  The state goes from global memory straight
+// into LDS (global_load_lds, one 4-byte word per lane: two wavefront
+// instructions for its 74 words), so the copy holds no registers and the
+// partials' loads, issued right behind it, are in flight at the same time:
+// one memory latency for both.  (Copied through thread 0's registers, the
+// state's loads had to land before the partials' were issued, and copying it
+// by registers beside the partials' loads took the primal kernels past 100
+// VGPRs.)
 __device__ __forceinline__ void load_and_decide(DevState* __restrict__ S, int p,
                                                 const double* __restrict__ part_m, int nb_m,
                                                 const double* __restrict__ dx2_slot,
                                                 DevState& Q, bool publisher) {
     if (threadIdx.x < 64) {
+        constexpr int W = (int)(sizeof(DevState) / 4);
+        static_assert(sizeof(DevState) % 4 == 0 && W > 64 && W <= 128,
+                      "the state is copied in two rounds of one word per lane");
+        const int lane = (int)threadIdx.x;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(S + p);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&Q);
+        // lane l's word lands at dst + l (the LDS base is wavefront-uniform)
+        __builtin_amdgcn_global_load_lds(src + lane, dst, 4, 0, 0);
+        if (lane < W - 64) __builtin_amdgcn_global_load_lds(src + 64 + lane, dst + 64, 4, 0, 0);
         double dx2 = 0.0;
-        if (threadIdx.x == 0) {
-            Q = S[p];
-            dx2 = *dx2_slot;
-        }
+        if (lane == 0) dx2 = *dx2_slot;
         double sm[2];
         wave_sum_partials<2>(part_m, nb_m, sm);
+        // every memory operation of the wavefront, the copy's LDS writes
+        // included, has completed before lane 0 reads the state
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
         if (threadIdx.x == 0) {
             if (Q.decide_pending) {
                 decide_step(&Q, dx2, sm[0], sm[1]);
