@@ -602,6 +602,9 @@ class Solver {
     // two adjacent rows of A^T per lane in the primal launches where the
     // layout allows it (primal_view; SUPR_ROW_PAIRS=off: one row per lane)
     bool row_pairs_ = true;
+    // FP64 row pairs in one pass take k_primal_pairs (SUPR_EARLY_SLOTS=off:
+    // k_primal's row-pair path; the same bits)
+    bool early_slots_ = true;
     int passes_ = 10;
     std::vector<char> integer_;
     std::unique_ptr<DevLp> dlp_;
@@ -1083,6 +1086,7 @@ void Solver::setup_device_resident(bool mixed) {
     const auto t0 = Clock::now();
     if (const char* e = std::getenv("SUPR_ONE_PASS"); e && std::strcmp(e, "off") == 0) one_pass_ = false;
     if (const char* e = std::getenv("SUPR_ROW_PAIRS"); e && std::strcmp(e, "off") == 0) row_pairs_ = false;
+    if (const char* e = std::getenv("SUPR_EARLY_SLOTS"); e && std::strcmp(e, "off") == 0) early_slots_ = false;
     // The host passes over the vectors (x0, eta0, omega0; the row kinds and
     // the bound kinds) run on two worker threads while this thread uploads
     // and lays out the matrices; they only read the forms.
@@ -1274,9 +1278,18 @@ template <typename T> MatView<T> primal_view(const DevMatrix& AT, bool pairs_on)
 // The iteration kernels, in their one-pass form (matrix slots and epilogue
 // operands loaded before the step decision) when pre is set and the launch
 // covers every row in one pass.
+// early: FP64 row pairs in one pass of the grid take k_primal_pairs (the
+// pair's first index slots loaded before the decision prologue; the same bits
+// as k_primal's row-pair path, SUPR_EARLY_SLOTS=off)
 template <int L, typename T>
-void launch_primal(bool pre, int grid, hipStream_t s, DevState* S, int p, const MatView<T>& AT,
+void launch_primal(bool pre, bool early, int grid, hipStream_t s, DevState* S, int p, const MatView<T>& AT,
                    const IterBufs<T>& B, const double* part_m, int nb_m, const double* dx2, double* part_n) {
+    if constexpr (sizeof(T) == 8 && L == 1) {
+        if (early && AT.pairs && (long)AT.rows <= 2L * grid * kBlock) {
+            hipLaunchKernelGGL((k_primal_pairs<T>), dim3(grid), dim3(kBlock), 0, s, S, p, AT, B, part_m, nb_m, dx2, part_n);
+            return;
+        }
+    }
     if (pre && one_pass_rows<L>(AT.rows))
         hipLaunchKernelGGL((k_primal<L, T, true>), dim3(grid), dim3(kBlock), 0, s, S, p, AT, B, part_m, nb_m, dx2, part_n);
     else
@@ -1307,7 +1320,7 @@ void Solver::enqueue_attempts(const Iter<T>& I, int count) {
     const int pm = one_pass_dual(A_);
     for (int k = 0; k < count; ++k) {
         const int p = k & 1;
-        SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(pn, gn, s, S, p, AT, B, part_m_, gm, dx2_slot_, part_n_)));
+        SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(pn, early_slots_, gn, s, S, p, AT, B, part_m_, gm, dx2_slot_, part_n_)));
         SUPR_LANES(A_.lanes, (launch_dual<LL, T>(pm, gm, s, S, p ^ 1, A, B, part_n_, gn, dx2_slot_, part_m_)));
     }
     SUPR_HIP(hipGetLastError());
@@ -1695,7 +1708,7 @@ void Solver::bench_kernels(int reps, double* out) {
         const DevState both[2] = {st, st};
         SUPR_HIP(hipMemcpyAsync(S, both, sizeof(both), hipMemcpyHostToDevice, s));
         out[slot] = time_kernel([&] {
-            SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(one_pass(AT_), gn, s, S, 0, AT, B, part_m_, gm, dx2_slot_, part_n_)));
+            SUPR_LANES(AT_.lanes, (launch_primal<LL, T>(one_pass(AT_), early_slots_, gn, s, S, 0, AT, B, part_m_, gm, dx2_slot_, part_n_)));
         });
         out[slot + 1] = time_kernel([&] {
             SUPR_LANES(A_.lanes, (launch_dual<LL, T>(one_pass_dual(A_), gm, s, S, 1, A, B, part_n_, gn, dx2_slot_, part_m_)));

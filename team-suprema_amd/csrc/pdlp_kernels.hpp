@@ -228,10 +228,14 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
             *reinterpret_cast<V2*>(xn + r) = out;
             if constexpr (sizeof(T) == 4)
                 reinterpret_cast<float4*>(B.xd)[q] = make_float4(v[0], v[0] - xj[0], v[1], v[1] - xj[1]);
+            // an explicit fma: written acc += d * d, the two squares' sum is
+            // contracted by the compiler's choice of which multiply to fuse,
+            // and that choice differed between kernels with the same source
+            // (k_primal_pairs fused the first square, this loop the second)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const double d = (double)v[h] - (double)xj[h];
-                acc[0] += d * d;
+                acc[0] = fma(d, d, acc[0]);
             }
         }
     } else if (staged_rows<L>(AT.rows)) {
@@ -244,6 +248,138 @@ __global__ __launch_bounds__(kBlock) void k_primal(DevState* __restrict__ S, int
             const T a = row_dot<L, T>(AT, y, row, lane, valid);
             if (valid && lane == 0) epilogue(row, a);
         });
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part_n[blockIdx.x] = acc[0];
+}
+
+// The primal half over an ELL A^T with one lane per row in one pass of row
+// pairs (MatView::pairs and rows <= 2 x grid x kBlock; the 1M-variable
+// transportation LP's A^T): lane q takes rows 2q and 2q + 1, and loads the
+// pair's first two index slots before the decision prologue, so the gathers
+// of y start as soon as the decision is known instead of one memory latency
+// later; the rest of the pair's operands are loaded after the decision,
+// beside the gathers.  The terms, epilogue and partial sums are those of
+// k_primal's row-pair path (one pair per lane), so the same bits.  A kernel
+// of its own: only this path sets its register count (64 VGPRs in FP64; the
+// FP32 loop keeps k_primal's path, where the early slots took it to 66, a
+// wavefront per SIMD less).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_primal_pairs(DevState* __restrict__ S, int p,
+                                                         MatView<T> AT, IterBufs<T> B,
+                                                         const double* __restrict__ part_m, int nb_m,
+                                                         const double* __restrict__ dx2_slot,
+                                                         double* __restrict__ part_n) {
+    using V2 = typename Pair<T>::type;
+    __shared__ DevState Q;
+    const size_t R = (size_t)AT.rows;
+    const int q = xcd_block() * kBlock + (int)threadIdx.x;
+    const int r = 2 * q;
+    const bool valid = r < AT.rows;
+    int c0[2] = {0, 0}, c1[2] = {0, 0};   // slots 0 and 1 of rows r, r + 1
+    auto slot = [&](int k, int (&c)[2]) {
+        if (AT.idx16) {
+            const ushort2 u = *reinterpret_cast<const ushort2*>(AT.idx16 + (size_t)k * R + r);
+            c[0] = u.x;
+            c[1] = u.y;
+        } else {
+            const int2 u = *reinterpret_cast<const int2*>(AT.idx + (size_t)k * R + r);
+            c[0] = u.x;
+            c[1] = u.y;
+        }
+    };
+    if (valid && AT.ell_w > 0) slot(0, c0);
+    if (valid && AT.ell_w > 1) slot(1, c1);
+    const bool publisher = blockIdx.x == gridDim.x - 1;
+    load_and_decide(S, p, part_m, nb_m, dx2_slot, Q, publisher);
+    if (!Q.active) return;
+    if (publisher && threadIdx.x == 0) {
+        S[p ^ 1].decide_pending = 1;   // k_dual will run
+        publish_step_powers(S, p, Q);
+    }
+    const int cur = Q.cur;
+    const bool pending = Q.pending != 0;
+    const T eta_acc = (T)Q.eta_acc;
+    const T tau = (T)(Q.eta / Q.omega);
+    const T* __restrict__ x = B.x[cur];
+    T* __restrict__ xn = B.x[cur ^ 1];
+    const T* __restrict__ y = B.y[cur];
+    double acc[1] = {0.0};
+    if (valid) {
+        // ell_pair_dot's terms in its order: slot 0, slot 1, then the rest
+        T a[2] = {T(0), T(0)};
+        bool rebuilt = false;
+        if constexpr (std::is_same<T, double>::value) rebuilt = AT.csc != nullptr;
+        if (rebuilt) {
+            if constexpr (std::is_same<T, double>::value) {
+                const double2 f = *reinterpret_cast<const double2*>(AT.rsc + r);
+                if (AT.ell_w > 0) {
+                    a[0] += (AT.csc[c0[0]] * f.x) * y[c0[0]];
+                    a[1] += (AT.csc[c0[1]] * f.y) * y[c0[1]];
+                }
+                if (AT.ell_w > 1) {
+                    a[0] += (AT.csc[c1[0]] * f.x) * y[c1[0]];
+                    a[1] += (AT.csc[c1[1]] * f.y) * y[c1[1]];
+                }
+                for (int k = 2; k < AT.ell_w; ++k) {
+                    int c[2];
+                    slot(k, c);
+                    a[0] += (AT.csc[c[0]] * f.x) * y[c[0]];
+                    a[1] += (AT.csc[c[1]] * f.y) * y[c[1]];
+                }
+            }
+        } else {
+            auto vals = [&](int k) { return *reinterpret_cast<const V2*>(AT.val + (size_t)k * R + r); };
+            if (AT.ell_w > 0) {
+                const V2 v = vals(0);
+                a[0] += v.x * y[c0[0]];
+                a[1] += v.y * y[c0[1]];
+            }
+            if (AT.ell_w > 1) {
+                const V2 v = vals(1);
+                a[0] += v.x * y[c1[0]];
+                a[1] += v.y * y[c1[1]];
+            }
+            for (int k = 2; k < AT.ell_w; ++k) {
+                int c[2];
+                slot(k, c);
+                const V2 v = vals(k);
+                a[0] += v.x * y[c[0]];
+                a[1] += v.y * y[c[1]];
+            }
+        }
+        // k_primal's row-pair epilogue
+        const V2 xv = *reinterpret_cast<const V2*>(x + r);
+        const T xj[2] = {xv.x, xv.y};
+        if (pending) {
+            V2 sv = *reinterpret_cast<const V2*>(B.xsum + r);
+            sv.x += eta_acc * xj[0];
+            sv.y += eta_acc * xj[1];
+            *reinterpret_cast<V2*>(B.xsum + r) = sv;
+        }
+        const V2 cv = *reinterpret_cast<const V2*>(B.c + r);
+        const T z[2] = {xj[0] - tau * (cv.x - a[0]), xj[1] - tau * (cv.y - a[1])};
+        T v[2];
+        if (B.l) {
+            const V2 lv = *reinterpret_cast<const V2*>(B.l + r);
+            const V2 uv = *reinterpret_cast<const V2*>(B.u + r);
+            v[0] = smin(smax(z[0], lv.x), uv.x);
+            v[1] = smin(smax(z[1], lv.y), uv.y);
+        } else {
+            v[0] = smax(z[0], T(0));
+            v[1] = smax(z[1], T(0));
+        }
+        V2 out;
+        out.x = v[0];
+        out.y = v[1];
+        *reinterpret_cast<V2*>(xn + r) = out;
+        if constexpr (sizeof(T) == 4)
+            reinterpret_cast<float4*>(B.xd)[q] = make_float4(v[0], v[0] - xj[0], v[1], v[1] - xj[1]);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double d = (double)v[h] - (double)xj[h];
+            acc[0] = fma(d, d, acc[0]);   // see k_primal's row-pair path
+        }
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) part_n[blockIdx.x] = acc[0];

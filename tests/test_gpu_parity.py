@@ -711,6 +711,36 @@ def test_row_pairs_follow_the_one_row_per_lane_solve(monkeypatch, mode):
               f"diff x {dx:.1e} y {dy:.1e} objective {dobj:.1e}", file=sys.stderr, flush=True)
 
 
+@pytest.mark.parametrize("mode", [P.MODE_MIXED, P.MODE_FAST_FP64])
+def test_early_index_slots_give_the_same_bits(monkeypatch, mode):
+    """FP64 row pairs in one pass of the grid run k_primal_pairs, which loads
+    each pair's first two index slots before the decision prologue;
+    SUPR_EARLY_SLOTS=off runs k_primal's row-pair path.  The same terms, the
+    same epilogue and the same partial sums (each lane's two squares added by
+    an explicit fma in both), so the same solve bit for bit: 16-bit indices
+    with values formed from factors (transportation, to convergence), stored
+    values, and 32-bit indices with finite upper bounds (random LP, to an
+    iteration limit)."""
+    cases = [("transport", P.gen_transport(1000, 600, seed=9), {}, 200000),
+             ("transport_stored", P.gen_transport(1000, 600, seed=9), {"SUPR_ELL_VALUES": "stored"}, 3000),
+             ("random_wide", P.gen_random_lp(70000, 600000, 3, seed=21), {}, 1500)]
+    for name, lp, env, limit in cases:
+        s, Dr, Dc = P.scale(lp)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        on = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, max_iterations=limit)
+        monkeypatch.setenv("SUPR_EARLY_SLOTS", "off")
+        off = P.solve_pdhg_gpu(lp, s, Dr, Dc, tol=TOL, mode=mode, max_iterations=limit)
+        monkeypatch.delenv("SUPR_EARLY_SLOTS")
+        for k in env:
+            monkeypatch.delenv(k)
+        assert on.iterations == off.iterations and on.restarts == off.restarts, name
+        assert np.array_equal(on.x, off.x) and np.array_equal(on.y, off.y), name
+        assert on.info["objective"] == off.info["objective"], name
+        print(f"[parity] early slots {name} mode {mode}: {on.iterations} attempts, same bits",
+              file=sys.stderr, flush=True)
+
+
 def test_device_memory_cache_reuse_and_release():
     """Device memory a call releases is kept and reused by the next call (no
     hipMalloc / hipFree between one-shot calls); results do not depend on
