@@ -90,6 +90,19 @@ def dual_form(rows, nnz, lanes, ell):
     return 2 if not ell and g <= 1024 else 0
 
 
+def primal_kernel(n, lanes, ell, fp64):
+    """The rocprof name of the primal kernel for an A^T with n rows, more
+    than one pass of the 2,048-block grid holds at `lanes` lanes per row (the
+    1M-variable transportation LP's; the engine's rule: primal_view and
+    launch_primal): FP64 row pairs that fit one pass of pairs run
+    k_primal_pairs, the rest k_primal's grid-stride form (template flag
+    false)."""
+    pairs = ell and lanes == 1 and n % 2 == 0 and n > 2048 * 256
+    if fp64 and pairs and n <= 2 * 2048 * 256:
+        return "k_primal_pairs<double>"
+    return f"k_primal<{lanes}, {'double' if fp64 else 'float'}, false>"
+
+
 def kernel_bytes(lp, s, ell_AT, ell_A, nonneg=False, store_aty=False, rebuilt=False, compressed=True):
     """Algorithmic bytes of one launch of each iteration kernel (every array
     read or written once), value/vector size s (4: FP32 loop, 8: FP64 loop).
@@ -859,13 +872,14 @@ def main():
         return
     if args.pmc_summary:
         lanes = [32, 1]
-        # the 1M-variable LP's primal launches take the grid-stride kernel
-        # (template flag false: A^T's rows exceed one pass), its dual launches
-        # the flat one-pass form over A's 2,000 rows of 1,000
+        # the 1M-variable LP's primal launches take k_primal_pairs (FP64) and
+        # k_primal's grid-stride row-pair path (FP32), its dual launches the
+        # flat one-pass form over A's 2,000 rows of 1,000
         df = dual_form(2000, 2000000, lanes[0], False)
         print(json.dumps(pmc_summary(*args.pmc_summary, {
-            "k_primal_f32": f"k_primal<{lanes[1]}, float, false>", "k_dual_f32": f"k_dual<{lanes[0]}, float, {df}>",
-            "k_primal_f64": f"k_primal<{lanes[1]}, double, false>",
+            "k_primal_f32": primal_kernel(1000000, lanes[1], True, False),
+            "k_dual_f32": f"k_dual<{lanes[0]}, float, {df}>",
+            "k_primal_f64": primal_kernel(1000000, lanes[1], True, True),
             "k_dual_f64": f"k_dual<{lanes[0]}, double, {df}>"}), indent=1))
         return
     if args.gpus != 1:
@@ -993,7 +1007,7 @@ def main():
         "k_primal_f32": dict(ms=kb["primal_ms_fp32"], bytes=u32[0], format_bytes=prim32),
         "k_dual_f32": dict(ms=kb["dual_ms_fp32"], bytes=u32[1], format_bytes=dual32),
         "k_primal_f64": dict(ms=kb["primal_ms_fp64"], bytes=u64[0], format_bytes=prim64,
-                             rocprof=f"k_primal<{lat}, double, false>",
+                             rocprof=primal_kernel(lp.n, lat, kb["ell_AT"] > 0, True),
                              rebuilt_values=bool(kb.get("rebuilt_AT_fp64", 0) > 0)),
         "k_dual_f64": dict(ms=kb["dual_ms_fp64"], bytes=u64[1], format_bytes=dual64,
                            rocprof=f"k_dual<{la}, double, {dual_form(lp.m, lp.nnz, la, kb['ell_A'] > 0)}>"),
