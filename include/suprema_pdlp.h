@@ -166,7 +166,12 @@ typedef int (*supr_stop_fn)(void* ctx);
  * half's gathers of x then share cache lines); x_out comes back in the
  * caller's column order.  SUPR_COLUMN_ORDER=given turns this off
  * (also for supr_engine_create and supr_bench_kernels); SUPR_MODE_PLAIN_FP64,
- * the reference engine's design, always solves the columns as given. */
+ * the reference engine's design, always solves the columns as given.  When
+ * A^T is ELL with one lane per row, an even row count and 2^19 or more rows,
+ * the primal half takes two adjacent rows per lane: each row's values are
+ * the same bits, but the step statistic's partial sums group the rows
+ * differently, so the trajectory differs from one row per lane in the last
+ * bits (SUPR_ROW_PAIRS=off). */
 int supr_solve_pdhg_gpu(const supr_lp* unscaled_form, const supr_lp* scaled_form,
                         const double* Dr, const double* Dc, const supr_pdlp_options* opt,
                         supr_stop_fn stop, void* stop_ctx, double* x_out, double* y_out,
