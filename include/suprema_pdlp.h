@@ -167,7 +167,7 @@ typedef int (*supr_stop_fn)(void* ctx);
  * caller's column order.  SUPR_COLUMN_ORDER=given turns this off
  * (also for supr_engine_create and supr_bench_kernels); SUPR_MODE_PLAIN_FP64,
  * the reference engine's design, always solves the columns as given.  When
- * A^T is ELL with one lane per row, an even row count and 2^19 or more rows,
+ * A^T is ELL with one lane per row, an even row count and more than 2^19 rows,
  * the primal half takes two adjacent rows per lane: each row's values are
  * the same bits, but the step statistic's partial sums group the rows
  * differently, so the trajectory differs from one row per lane in the last
